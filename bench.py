@@ -1,0 +1,169 @@
+#!/usr/bin/env python3
+"""Benchmark of the GPU pricing engine on one MI355X.
+
+Workload (one "step"): the up-and-out barrier call ((S-100)^+ at T=1, knocked
+out when S >= 200 on any of 253 daily monitoring points) under GBM s0=180,
+sigma=0.25, r=0.05, 4,194,304 paths x 252 steps, priced with its Greeks by CRN
+bump-and-reprice exactly as greeks_gbm does (price, S+-1.8, sigma+-0.01,
+r+-1e-4: seven scenario pricings sharing one random stream).  The metric is
+scenario path-steps per second: 7 x paths x steps per step, divided by the
+wall time of the timed steps.  Inputs are a contract, a model and a seed, so
+nothing crosses PCIe inside the timed region except the program tables
+(a few KB) and the results.
+
+Prints one JSON line.  The roofline object uses HIP events recorded by the
+engine on its own stream around every path-kernel launch; cpu_baseline times
+the oracle (the C++ f64 restatement of the reference pricer) on this host's
+cores on a bounded number of paths.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "path-steps/s on 1 MI355X vs host CPU; adjoint vs bump Greeks/s at equal stderr"
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(n_paths: int, threads: int):
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as O
+    import kontracts_amd as K
+
+    c = K.up_and_out_call("X", 100.0, 200.0, 1.0)
+    cfg = O.cfg(n_paths, 42, 252, 0.05, threads=threads)
+    t0 = time.perf_counter()
+    O.greeks_gbm(c.to_json(), "X", 180.0, 0.25, cfg, 1.8, 0.01, 1e-4)
+    dt = time.perf_counter() - t0
+    return 7 * n_paths * 252 / dt, dt
+
+
+def traffic_from_profiles():
+    path = os.path.join(ROOT, "profiles", "path_kernel_gbm_barrier_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f).get("bytes_per_launch")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--paths", type=int, default=4_194_304)
+    ap.add_argument("--cpu-paths", type=int, default=65_536)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--extras", action="store_true", help="also time configs 1 and 3 (single pricings)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU: this build has no multi-GPU path (paths would shard by index with one "
+              "f64 statistics reduction per contract); use --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import kontracts_amd as K
+    from kontracts_amd import _native as N
+    from kontracts_amd import roofline as R
+
+    eng = N.engine(0)
+    contract = K.up_and_out_call("X", 100.0, 200.0, 1.0)
+    model = K.GBM(180.0, 0.25, 0.05, asset="X")
+    bumps = (1.8, 0.01, 1e-4)
+    n, steps = args.paths, 252
+
+    def one_step():
+        return contract.greeks(model, n_paths=n, seed=42, steps_per_year=steps, bumps=bumps)
+
+    for _ in range(args.warmup):
+        one_step()
+    eng.set_profiling(True)
+    eng.kernel_time(reset=True)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        g = one_step()
+    t1 = time.perf_counter()
+    k_ms, k_launches, k_path_steps = eng.kernel_time(reset=True)
+    eng.set_profiling(False)
+
+    wall = t1 - t0
+    ms_per_step = 1e3 * wall / args.steps
+    scen_path_steps = 7.0 * n * steps
+    value = scen_path_steps * args.steps / wall
+    log(f"greeks: price {g.price:.6f} delta {g.delta:.6f} gamma {g.gamma:.6e} vega {g.vega:.6f} rho {g.rho:.6f}")
+    log(f"{ms_per_step:.3f} ms/step, {value:.4e} scenario path-steps/s; kernels {k_launches} launches "
+        f"{k_ms / max(k_launches, 1):.3f} ms avg")
+
+    avg_launch_s = (k_ms / max(k_launches, 1)) / 1e3
+    per_launch_path_steps = k_path_steps / max(k_launches, 1)
+    ops = per_launch_path_steps * R.gbm_step(monitored=True)
+    achieved = ops / avg_launch_s / 1e12 if avg_launch_s > 0 else 0.0
+    peak = R.PEAK_LANE_OPS / 1e12
+    roof = {"bound": "valu", "achieved": round(achieved, 3), "peak": round(peak, 3), "unit": "Tlane-op/s",
+            "frac": round(achieved / peak, 4), "traffic": traffic_from_profiles(),
+            "kernel": "path_kernel<GBM,NS=8>", "avg_launch_ms": round(avg_launch_s * 1e3, 4),
+            "lane_ops_per_path_step": R.gbm_step(monitored=True),
+            "path_steps_per_launch": per_launch_path_steps}
+
+    cpu = None
+    if not args.no_cpu:
+        threads = min(len(os.sched_getaffinity(0)), 16)
+        cv, dt = cpu_baseline(args.cpu_paths, threads)
+        cpu = {"value": round(cv, 1), "unit": "path-steps/s", "cores": threads, "kind": "port",
+               "sample": f"oracle greeks_gbm restatement (7 CRN pricings, f64, recursive AST evaluation) on "
+                         f"{args.cpu_paths} paths x 252 steps of the same contract, {dt:.1f} s"}
+        log(f"cpu baseline {cv:.4e} path-steps/s on {threads} threads ({dt:.1f} s)")
+
+    extras = {}
+    if args.extras:
+        grid = [i / 252 for i in range(253)]
+        c1 = K.european_call("X", 100.0, 1.0)
+        m1 = K.GBM(100.0, 0.2, 0.05, asset="X")
+        c1.price(m1, n_paths=1_000_000, grid=grid)
+        t = time.perf_counter()
+        r1 = c1.price(m1, n_paths=1_000_000, grid=grid)
+        extras["config1_european_gbm_path_steps_per_s"] = 1_000_000 * 252 / (time.perf_counter() - t)
+        extras["config1_price"] = r1.price
+        h = K.heston(100, 0.04, 2.0, 0.04, 0.3, -0.5, 0.05, asset="X")
+        c1.price(h, n_paths=2_097_152, steps_per_year=365)
+        t = time.perf_counter()
+        r3 = c1.price(h, n_paths=2_097_152, steps_per_year=365)
+        extras["config3_heston_forward_path_steps_per_s"] = 2_097_152 * 365 / (time.perf_counter() - t)
+        extras["config3_price"] = r3.price
+
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "path-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 paths, f64 payoff/stderr reduction",
+        "data": "synthetic (contract + model + seed; counter-based Philox normals generated on the GPU)",
+        "config": {"workload": "up-and-out call (S-100)^+ KO at S>=200, GBM s0=180 sigma=0.25 r=0.05, "
+                               f"{n} paths x 252 steps, CRN bump Greeks (7 scenario pricings)",
+                   "paths": n, "steps": steps, "scenarios": 7, "parallelism": "single GPU"},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "greeks": {"price": g.price, "delta": g.delta, "gamma": g.gamma, "vega": g.vega, "rho": g.rho,
+                   "price_std_error": g.std_errors[0]},
+    }
+    if extras:
+        line["extras"] = extras
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,226 @@
+/*
+ * kontracts.h — C ABI of the MI355X Monte Carlo pricing engine.
+ *
+ * This is the drop-in boundary for kontract's path-simulation hot path.  Every
+ * entry point replaces one reference function; the citation on each names the
+ * Rust function (file:line in the kontract crate) whose argument meaning,
+ * defaults and error behaviour it keeps.
+ *
+ * Conventions
+ *   - Plain C types only: pointers, sizes, doubles.  The caller owns every host
+ *     buffer; the engine owns device memory.
+ *   - Contracts cross the boundary as serde's externally tagged JSON, exactly as
+ *     Contract::to_json writes it (src/ast.rs:210-218), e.g.
+ *       {"When":[{"At":1.0},{"Scale":[{"Max":[{"Sub":[{"Spot":"X"},{"Const":100.0}]},
+ *        {"Const":0.0}]},{"One":"USD"}]}]}
+ *   - kt_status values 1..7 are KontractError's variants (src/error.rs:7-28) in
+ *     declaration order; the message is available through kt_last_error.
+ *   - Calls are synchronous: each returns after the engine's HIP stream has
+ *     drained.  An engine is not thread-safe; use one engine per host thread.
+ *   - Results depend only on the inputs and the seed (counter-based RNG, fixed
+ *     reduction order), not on launch shape or scheduling.
+ */
+#ifndef KONTRACTS_H
+#define KONTRACTS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define KT_ABI_VERSION 1
+#define KT_MAX_ASSETS 8
+
+typedef enum kt_status {
+  KT_OK = 0,
+  KT_ERR_UNKNOWN_ASSET = 1,       /* KontractError::UnknownAsset */
+  KT_ERR_TIME_OUT_OF_RANGE = 2,   /* KontractError::TimeOutOfRange */
+  KT_ERR_STEP_OUT_OF_RANGE = 3,   /* KontractError::StepOutOfRange */
+  KT_ERR_INCONSISTENT_PATH = 4,   /* KontractError::InconsistentPath */
+  KT_ERR_SERDE = 5,               /* KontractError::Serde */
+  KT_ERR_MALFORMED_CONTRACT = 6,  /* KontractError::MalformedContract */
+  KT_ERR_UNSUPPORTED = 7,         /* KontractError::Unsupported */
+  KT_ERR_INVALID_ARGUMENT = 100,  /* bad pointer / size at the C boundary */
+  KT_ERR_DEVICE = 101,            /* HIP runtime or kernel failure */
+  KT_ERR_OUT_OF_MEMORY = 102
+} kt_status;
+
+typedef struct kt_engine kt_engine;
+
+/* Model kinds: the three simulators on the hot path. */
+enum {
+  KT_MODEL_GBM = 0,            /* Gbm (src/simulator.rs:188-253) */
+  KT_MODEL_HESTON = 1,         /* HestonSimulator (src/simulator.rs:453-584) */
+  KT_MODEL_CORRELATED_GBM = 2  /* CorrelatedGbmN (src/multi_asset.rs:49-184) */
+};
+
+/* Heston variance schemes.  FLOOR is the reference's: v <- max(v_next, 0) after
+ * every step (src/simulator.rs:531-533).  FULL_TRUNCATION uses max(v, 0) inside
+ * the drift and diffusion and keeps v unfloored (Lord, Koekkoek, van Dijk 2010). */
+enum { KT_HESTON_FLOOR = 0, KT_HESTON_FULL_TRUNCATION = 1 };
+
+/* A simulator.  Replaces `&dyn Simulator` (src/simulator.rs:70-178).
+ *   GBM:            n_assets = 1; s0, mu (= r - q, bindings.rs:518-521), sigma.
+ *   HESTON:         n_assets = 1; s0 = spot, mu = r (drift), v0..rho below.
+ *   CORRELATED_GBM: n_assets = k (2..8); s0/mu/sigma per factor (GbmFactor,
+ *                   src/fx.rs:149-170) and corr = k*k row-major correlation. */
+typedef struct kt_model {
+  int32_t kind;
+  int32_t n_assets;
+  const char* const* asset_names; /* n_assets NUL-terminated names */
+  const double* s0;               /* [n_assets] */
+  const double* mu;               /* [n_assets] */
+  const double* sigma;            /* [n_assets] (unused for HESTON) */
+  const double* corr;             /* [n_assets*n_assets] (CORRELATED_GBM only) */
+  double v0, kappa, theta, sigma_v, rho; /* HESTON only */
+  int32_t heston_scheme;          /* KT_HESTON_FLOOR (default) or FULL_TRUNCATION */
+  int32_t reserved;
+} kt_model;
+
+/* McConfig (src/pricer.rs:30-54).  grid/n_grid, when set, replace the compiled
+ * time grid, which is the reference's price_on_paths route on an explicit grid
+ * (src/pricer.rs:230-241).  antithetic/control_variate must be 0 (the variance
+ * reduction branch, src/pricer.rs:117-228, is not on this engine's path). */
+typedef struct kt_mc_config {
+  uint64_t n_paths;
+  uint64_t seed;
+  uint64_t steps_per_year;
+  double rate;
+  const double* grid;
+  uint64_t n_grid;
+  uint32_t antithetic;
+  uint32_t control_variate;
+} kt_mc_config;
+
+/* PriceResult (src/pricer.rs:58-71). */
+typedef struct kt_price_result {
+  double price;
+  double sample_std;
+  double std_error;
+  double ci95_low;
+  double ci95_high;
+  uint64_t n_paths;
+} kt_price_result;
+
+/* BumpSizes (src/greeks.rs:22-39); defaults spot 1e-2, vol 1e-2, rate 1e-4. */
+typedef struct kt_bumps {
+  double spot;
+  double vol;
+  double rate;
+} kt_bumps;
+
+/* Greeks (src/greeks.rs:43-54) plus the Monte Carlo standard error of each
+ * estimate, computed from the per-path finite-difference samples. */
+typedef struct kt_greeks {
+  double price, delta, gamma, vega, rho;
+  double std_error[5]; /* same order: price, delta, gamma, vega, rho */
+} kt_greeks;
+
+/* Engine lifetime. `device` is the HIP device ordinal. */
+kt_status kt_engine_create(int device, kt_engine** out);
+void kt_engine_destroy(kt_engine* engine);
+/* Message of the last failing call on this engine ("" if none). */
+const char* kt_last_error(const kt_engine* engine);
+/* Message of the last failing engine-less call on this thread. */
+const char* kt_last_error_global(void);
+
+/* price_gbm (src/pricer.rs:97-115): compile, build the grid
+ * (time_grid_forced with the model's fine-grid requirement), simulate, evaluate
+ * cashflows per path and summarize.  Replaces PyContract::price
+ * (src/bindings.rs:350-379) underneath the Python API. */
+kt_status kt_price(kt_engine* engine, const char* contract_json, const kt_model* model,
+                   const kt_mc_config* cfg, kt_price_result* out);
+
+/* price_batch_gbm (src/pricer.rs:250-294): one merged grid (union of dates,
+ * max horizon, OR of fine flags), one simulation shared by all contracts, one
+ * result per contract.  n_contracts == 0 returns KT_OK with nothing written. */
+kt_status kt_price_batch(kt_engine* engine, const char* const* contract_jsons, uint64_t n_contracts,
+                         const kt_model* model, const kt_mc_config* cfg, kt_price_result* out);
+
+/* greeks_gbm (src/greeks.rs:60-98): CRN bump-and-reprice of a one-asset GBM with
+ * drift = cfg->rate; the seven repricings share one simulation of the
+ * normals on the GPU.  bumps == NULL means BumpSizes::default(). */
+kt_status kt_greeks_gbm(kt_engine* engine, const char* contract_json, const char* asset, double s0,
+                        double sigma, const kt_mc_config* cfg, const kt_bumps* bumps, kt_greeks* out);
+
+/* CRN bump-and-reprice for any model, the greeks_gbm pattern (central
+ * differences, common random numbers; src/greeks.rs:69-89) applied to the
+ * model's parameters.  Parameters: GBM {s0, sigma, mu}; HESTON {s0, v0, kappa,
+ * theta, sigma_v, rho}.  h[i] are absolute bumps (n_params entries).
+ * out[0] = price, out[1+i] = dPrice/dparam_i; stderr_out (nullable) holds the
+ * standard error of each.  Reprices with separate launches at the same seed. */
+kt_status kt_greeks_bump(kt_engine* engine, const char* contract_json, const kt_model* model,
+                         const kt_mc_config* cfg, const double* h, uint64_t n_params, double* out,
+                         double* stderr_out);
+
+/* Pathwise (adjoint) Greeks for Lipschitz payoffs: contracts whose conditions
+ * depend on time only (European, Asian, basket; no barrier, no digital).  A
+ * contract with a price-dependent condition returns KT_ERR_UNSUPPORTED: use
+ * kt_greeks_bump for those.  Parameters as in kt_greeks_bump; out[0] = price,
+ * out[1+i] = dPrice/dparam_i; stderr_out (nullable) as above.  No reference
+ * counterpart (the reference has bump Greeks only, src/greeks.rs). */
+kt_status kt_greeks_adjoint(kt_engine* engine, const char* contract_json, const kt_model* model,
+                            const kt_mc_config* cfg, double* out, double* stderr_out, uint64_t n_out);
+
+/* price_american_lsm (src/lsm.rs:80-124): Longstaff-Schwartz on the grid
+ * [0, dates...], ITM regression on {1, S, .., S^(n_basis-1)} with normal
+ * equations accumulated and solved in f64 (poly_fit/solve_linear,
+ * src/lsm.rs:259-288, src/numerics.rs:190-236).  The exercised payoff must
+ * have all its flows at the exercise date (no when/until/anytime inside). */
+kt_status kt_price_american(kt_engine* engine, const char* payoff_json, const double* exercise_dates,
+                            uint64_t n_dates, const kt_model* model, const kt_mc_config* cfg, int32_t n_basis,
+                            kt_price_result* out);
+
+/* One contract of a heterogeneous portfolio: its own model and MC config.
+ * Semantically a loop of price_gbm calls (src/pricer.rs:97-115). */
+typedef struct kt_job {
+  const char* contract_json;
+  kt_model model;
+  kt_mc_config cfg;
+} kt_job;
+
+/* Prices n_jobs contracts with different models, grids and path counts in one
+ * persistent launch over a (contract, path-block) work queue.  Each result is
+ * identical to kt_price on that job alone. */
+kt_status kt_price_portfolio(kt_engine* engine, const kt_job* jobs, uint64_t n_jobs, kt_price_result* out);
+
+/* Kernel timing: when enabled, HIP events on the engine's stream bracket every
+ * path-kernel launch group; the totals accumulate until reset.  path_steps
+ * counts paths x grid intervals x scenarios simulated by those launches. */
+kt_status kt_engine_set_profiling(kt_engine* engine, int32_t enabled);
+kt_status kt_engine_kernel_time(kt_engine* engine, double* total_ms, uint64_t* n_launches, double* path_steps,
+                                int32_t reset);
+
+/* ---- Host-only (no GPU needed) -------------------------------------------
+ * compile() + Plan::time_grid_forced (src/compiler.rs:54-85).  Writes up to
+ * cap grid points and dates; *n_grid / *n_dates receive the full counts.
+ * assets receives the sorted asset names joined by '\n'. */
+kt_status kt_plan(const char* contract_json, uint64_t steps_per_year, int32_t force_fine, double* grid,
+                  uint64_t* n_grid, double* dates, uint64_t* n_dates, uint64_t cap, int32_t* needs_fine_grid,
+                  double* horizon, char* assets, uint64_t assets_cap);
+
+/* Text listing of the per-step op program the engine would run for a
+ * contract on a grid (diagnostics; host only). */
+kt_status kt_describe_program(const char* contract_json, const kt_model* model, const kt_mc_config* cfg,
+                              char* buf, uint64_t cap);
+
+/* ---- Parity hooks (used by the tests) ---------------------------------------
+ * Philox4x32-10 on the GPU: n counters (4 words each) under one key. */
+kt_status kt_debug_philox(kt_engine* engine, const uint32_t* ctr, const uint32_t* key, uint64_t n,
+                          uint32_t* out);
+/* FP32 inverse normal on the GPU applied to raw 32-bit words. */
+kt_status kt_debug_normals(kt_engine* engine, const uint32_t* words, uint64_t n, float* out);
+/* kt_price that also returns every path's present value (pv_out[n_paths]) and,
+ * when normals_out != NULL, the FP32 normals the GPU drew, laid out
+ * [path][grid point][component].  grid_out receives the grid (capacity
+ * grid_cap) and *n_grid its length. */
+kt_status kt_debug_paths(kt_engine* engine, const char* contract_json, const kt_model* model,
+                         const kt_mc_config* cfg, double* pv_out, float* normals_out, double* grid_out,
+                         uint64_t grid_cap, uint64_t* n_grid, kt_price_result* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* KONTRACTS_H */

@@ -1,0 +1,632 @@
+// kt_engine.cpp — the C ABI (include/kontracts.h): validation, model tables,
+// device buffers, kernel launches and result assembly.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/kontracts.h"
+#include "kt_compiler.h"
+#include "kt_engine_internal.h"
+#include "kt_launch.h"
+
+using namespace kt;
+
+namespace kt {
+
+thread_local std::string g_last_error;
+
+const double kZ95 = 1.959963984540054;  // src/pricer.rs:74
+const double kLog2e = 1.4426950408889634;
+
+// ---------------------------------------------------------------------------
+// Models
+// ---------------------------------------------------------------------------
+ModelDesc model_from_abi(const kt_model* m) {
+  if (!m) raise(KT_ERR_INVALID_ARGUMENT, "null model");
+  ModelDesc d;
+  d.kind = m->kind;
+  int k = m->n_assets;
+  if (m->kind == KT_MODEL_GBM || m->kind == KT_MODEL_HESTON) {
+    if (k != 1) raise(KT_ERR_INVALID_ARGUMENT, "GBM and Heston models have exactly one asset");
+  } else if (m->kind == KT_MODEL_CORRELATED_GBM) {
+    // CorrelatedGbmN::new (src/multi_asset.rs:62-83)
+    if (k < 2) raise(KT_ERR_MALFORMED_CONTRACT, "CorrelatedGbmN: at least 2 assets required");
+    if (k > KT_MAX_ASSETS) raise(KT_ERR_UNSUPPORTED, "CorrelatedGbmN: at most 8 assets on this engine");
+    if (!m->corr) raise(KT_ERR_MALFORMED_CONTRACT, "CorrelatedGbmN: correlation matrix must be kxk");
+  } else {
+    raise(KT_ERR_INVALID_ARGUMENT, "unknown model kind");
+  }
+  if (!m->asset_names || !m->s0 || !m->mu) raise(KT_ERR_INVALID_ARGUMENT, "null model arrays");
+  if (m->kind != KT_MODEL_HESTON && !m->sigma) raise(KT_ERR_INVALID_ARGUMENT, "null sigma");
+  for (int i = 0; i < k; ++i) {
+    if (!m->asset_names[i]) raise(KT_ERR_INVALID_ARGUMENT, "null asset name");
+    d.names.push_back(m->asset_names[i]);
+    d.s0.push_back(m->s0[i]);
+    d.mu.push_back(m->mu[i]);
+    d.sigma.push_back(m->kind == KT_MODEL_HESTON ? 0.0 : m->sigma[i]);
+  }
+  if (m->kind == KT_MODEL_CORRELATED_GBM) {
+    for (int i = 0; i < k; ++i)
+      if (d.sigma[i] < 0.0) raise(KT_ERR_MALFORMED_CONTRACT, "CorrelatedGbmN: all volatilities must be >= 0");
+    // cholesky_lower with the PSD floor (src/numerics.rs:158-183)
+    d.chol.assign((size_t)k * k, 0.0);
+    for (int i = 0; i < k; ++i)
+      for (int j = 0; j <= i; ++j) {
+        double sum = m->corr[i * k + j];
+        for (int q = 0; q < j; ++q) sum -= d.chol[i * k + q] * d.chol[j * k + q];
+        if (i == j) {
+          d.chol[i * k + j] = std::sqrt(std::max(sum, 0.0));
+        } else {
+          double piv = d.chol[j * k + j];
+          d.chol[i * k + j] = piv > 0.0 ? sum / piv : 0.0;
+        }
+      }
+  }
+  d.v0 = m->v0;
+  d.kappa = m->kappa;
+  d.theta = m->theta;
+  d.sigma_v = m->sigma_v;
+  d.rho = m->rho;
+  d.scheme = m->heston_scheme;
+  return d;
+}
+
+// Parameter checks done by the reference simulators before simulating
+// (Gbm::simulate src/simulator.rs:221-225, HestonSimulator::simulate :556-580).
+void validate_model(const ModelDesc& d) {
+  if (d.kind == KT_MODEL_GBM && d.sigma[0] < 0.0) raise(KT_ERR_MALFORMED_CONTRACT, "GBM sigma must be non-negative");
+  if (d.kind == KT_MODEL_HESTON) {
+    if (!(d.rho >= -1.0 && d.rho <= 1.0)) raise(KT_ERR_MALFORMED_CONTRACT, "Heston rho must be in [-1, 1]");
+    if (d.v0 < 0.0) raise(KT_ERR_MALFORMED_CONTRACT, "Heston v0 must be non-negative");
+    if (d.sigma_v < 0.0) raise(KT_ERR_MALFORMED_CONTRACT, "Heston sigma_v must be non-negative");
+    if (d.theta < 0.0) raise(KT_ERR_MALFORMED_CONTRACT, "Heston theta must be non-negative");
+    if (d.kappa < 0.0) raise(KT_ERR_MALFORMED_CONTRACT, "Heston kappa must be non-negative");
+    if (d.scheme != KT_HESTON_FLOOR && d.scheme != KT_HESTON_FULL_TRUNCATION)
+      raise(KT_ERR_INVALID_ARGUMENT, "unknown Heston scheme");
+  }
+}
+
+int model_normals(const ModelDesc& d) {
+  return d.kind == KT_MODEL_GBM ? 1 : d.kind == KT_MODEL_HESTON ? 2 : (int)d.names.size();
+}
+
+int model_kernel_kind(const ModelDesc& d) {
+  return d.kind == KT_MODEL_GBM ? MK_GBM : d.kind == KT_MODEL_HESTON ? MK_HESTON : MK_BASKET;
+}
+
+// Per grid point model coefficients (see kt_launch.h).
+std::vector<float> model_table(const ModelDesc& d, const std::vector<double>& grid, int* stride) {
+  size_t n = grid.size();
+  std::vector<float> tab;
+  double prev = 0.0;
+  if (d.kind == KT_MODEL_HESTON) {
+    *stride = kHestonStride;
+    tab.assign(n * kHestonStride, 0.0f);
+    double r = d.mu[0];
+    for (size_t k = 0; k < n; ++k) {
+      double dt = grid[k] - prev;
+      prev = grid[k];
+      if (!(dt > 0.0)) continue;
+      double sdt = std::sqrt(dt);
+      float* t = &tab[k * kHestonStride];
+      t[0] = (float)(r * dt * kLog2e);
+      t[1] = (float)(-0.5 * dt * kLog2e);
+      t[2] = (float)(sdt * kLog2e);
+      t[3] = (float)(-d.kappa * dt);
+      t[4] = (float)(d.kappa * d.theta * dt);
+      t[5] = (float)(d.sigma_v * sdt);
+    }
+    return tab;
+  }
+  size_t K = d.names.size();
+  *stride = (int)(2 * K);
+  tab.assign(n * 2 * K, 0.0f);
+  for (size_t k = 0; k < n; ++k) {
+    double dt = grid[k] - prev;
+    prev = grid[k];
+    if (!(dt > 0.0)) continue;
+    for (size_t i = 0; i < K; ++i) {
+      double sg = d.sigma[i];
+      tab[(k * K + i) * 2] = (float)((d.mu[i] - 0.5 * sg * sg) * dt * kLog2e);
+      tab[(k * K + i) * 2 + 1] = (float)(sg * std::sqrt(dt) * kLog2e);
+    }
+  }
+  return tab;
+}
+
+kt_price_result result_from_stats(const KStats& s) {
+  kt_price_result r{};
+  r.n_paths = (uint64_t)s.n;
+  r.price = s.n > 0 ? s.mean : 0.0;
+  r.sample_std = s.n > 1 ? std::sqrt(s.m2 / (s.n - 1.0)) : 0.0;
+  r.std_error = s.n > 0 ? r.sample_std / std::sqrt(s.n) : 0.0;
+  r.ci95_low = r.price - kZ95 * r.std_error;
+  r.ci95_high = r.price + kZ95 * r.std_error;
+  return r;
+}
+
+void check_hip(hipError_t e, const char* what) {
+  if (e != hipSuccess) raise(KT_ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+int lowest_error_code(unsigned bits) {
+  for (int c = 1; c < 32; ++c)
+    if (bits & (1u << c)) return c;
+  return 0;
+}
+
+const char* error_text(int code) {
+  switch (code) {
+    case KT_ERR_UNKNOWN_ASSET: return "unknown asset referenced by the contract";
+    case KT_ERR_MALFORMED_CONTRACT: return "malformed contract: empty or invalid average window";
+    case KT_ERR_UNSUPPORTED:
+      return "unsupported: `or` reached on a path, or an average window extending past its evaluation step";
+    default: return "error raised on a path";
+  }
+}
+
+}  // namespace kt
+
+// ---------------------------------------------------------------------------
+// Engine
+// ---------------------------------------------------------------------------
+kt_engine::~kt_engine() {
+  (void)hipSetDevice(device);
+  for (DevBuf* b : {&arena, &partials, &stats, &errw, &pv, &z, &scratch, &paths, &aux})
+    if (b->p) (void)hipFree(b->p);
+  if (ev0) (void)hipEventDestroy(ev0);
+  if (ev1) (void)hipEventDestroy(ev1);
+  if (stream) (void)hipStreamDestroy(stream);
+}
+
+void kt_engine::prof_begin() {
+  if (!profiling) return;
+  if (!ev0) check_hip(hipEventCreate(&ev0), "hipEventCreate");
+  if (!ev1) check_hip(hipEventCreate(&ev1), "hipEventCreate");
+  check_hip(hipEventRecord(ev0, stream), "hipEventRecord");
+}
+
+void kt_engine::prof_end(double path_steps) {
+  if (!profiling) return;
+  check_hip(hipEventRecord(ev1, stream), "hipEventRecord");
+  check_hip(hipEventSynchronize(ev1), "hipEventSynchronize");
+  float ms = 0.0f;
+  check_hip(hipEventElapsedTime(&ms, ev0, ev1), "hipEventElapsedTime");
+  prof_ms += ms;
+  prof_path_steps += path_steps;
+  prof_launches += 1;
+}
+
+void* kt_engine::ensure(DevBuf& b, size_t bytes) {
+  if (bytes <= b.cap) return b.p;
+  if (b.p) check_hip(hipFree(b.p), "hipFree");
+  b.p = nullptr;
+  b.cap = 0;
+  size_t want = std::max<size_t>(bytes, 4096);
+  check_hip(hipMalloc(&b.p, want), "hipMalloc");
+  b.cap = want;
+  return b.p;
+}
+
+// Upload a set of host blobs into the arena; returns device pointers.
+std::vector<void*> kt_engine::upload(const std::vector<std::pair<const void*, size_t>>& blobs) {
+  size_t total = 0;
+  std::vector<size_t> offs;
+  for (auto& b : blobs) {
+    offs.push_back(total);
+    total += (b.second + 255) & ~(size_t)255;
+  }
+  host_stage.resize(std::max<size_t>(total, 256));
+  for (size_t i = 0; i < blobs.size(); ++i)
+    if (blobs[i].second) memcpy(host_stage.data() + offs[i], blobs[i].first, blobs[i].second);
+  char* dev = (char*)ensure(arena, std::max<size_t>(total, 256));
+  check_hip(hipMemcpyAsync(dev, host_stage.data(), total, hipMemcpyHostToDevice, stream), "upload");
+  std::vector<void*> out;
+  for (size_t o : offs) out.push_back(dev + o);
+  return out;
+}
+
+namespace kt {
+
+// Run one program over n_paths paths; returns the per-output statistics.
+// pv_host / z_host (nullable) request the debug variant.
+std::vector<KStats> run_program(kt_engine* E, const Program& prog, const ModelDesc& md,
+                                const std::vector<double>& grid, double rate, uint64_t n_paths, uint64_t seed,
+                                double* pv_host, float* z_host) {
+  int stride = 0;
+  std::vector<float> tab = model_table(md, grid, &stride);
+  std::vector<double> df(grid.size());
+  for (size_t k = 0; k < grid.size(); ++k) df[k] = std::exp(-rate * grid[k]);
+  int kind = model_kernel_kind(md);
+  int K = (int)md.names.size();
+  bool debug = pv_host || z_host;
+  int ns = 0, nout = 0;
+  if (!path_variant_exists(kind, K, prog.n_slots, prog.n_outputs, debug, &ns, &nout))
+    raise(KT_ERR_UNSUPPORTED, "no path kernel variant for this program shape");
+  if (prog.n_slots > ns) raise(KT_ERR_UNSUPPORTED, "program needs more registers than the kernel variant holds");
+
+  std::vector<KOp> ops = prog.ops;
+  if (ops.empty()) ops.push_back(KOp{OP_NOP, 0, 0, 0, 0.0f});
+  auto dev = E->upload({{ops.data(), ops.size() * sizeof(KOp)},
+                        {prog.steps.data(), prog.steps.size() * sizeof(KStep)},
+                        {prog.init.data(), prog.init.size() * sizeof(KOp)},
+                        {df.data(), df.size() * sizeof(double)},
+                        {tab.data(), tab.size() * sizeof(float)}});
+
+  const uint64_t kMaxBlocksPerLaunch = 1ull << 20;
+  uint64_t total_blocks = (n_paths + kBlock - 1) / kBlock;
+  if (total_blocks > (1ull << 31)) raise(KT_ERR_INVALID_ARGUMENT, "too many paths");
+  KStats* partials = (KStats*)E->ensure(E->partials, std::max<uint64_t>(total_blocks, 1) * nout * sizeof(KStats));
+  KStats* stats = (KStats*)E->ensure(E->stats, 8 * sizeof(KStats));
+  unsigned* err = (unsigned*)E->ensure(E->errw, 256);
+  check_hip(hipMemsetAsync(err, 0, sizeof(unsigned), E->stream), "memset");
+  double* pv_dev = nullptr;
+  float* z_dev = nullptr;
+  int M = model_normals(md);
+  if (pv_host) pv_dev = (double*)E->ensure(E->pv, std::max<uint64_t>(n_paths, 1) * sizeof(double));
+  if (z_host) z_dev = (float*)E->ensure(E->z, std::max<uint64_t>(n_paths * grid.size() * M, 1) * sizeof(float));
+
+  PathArgs A{};
+  A.ops = (const KOp*)dev[0];
+  A.steps = (const KStep*)dev[1];
+  A.init = (const KOp*)dev[2];
+  A.df = (const double*)dev[3];
+  A.tab = (const float*)dev[4];
+  A.n_init = (int)prog.init.size();
+  A.n_grid = (int)grid.size();
+  A.tab_stride = stride;
+  A.heston_full_truncation = md.scheme == KT_HESTON_FULL_TRUNCATION;
+  A.seed_lo = (uint32_t)seed;
+  A.seed_hi = (uint32_t)(seed >> 32);
+  for (int i = 0; i < K && i < kMaxAssets; ++i) A.s0[i] = (float)md.s0[i];
+  for (size_t i = 0; i < md.chol.size(); ++i) A.chol[i] = (float)md.chol[i];
+  A.v0 = (float)std::max(md.v0, 0.0);
+  A.rho = (float)md.rho;
+  A.rho_perp = (float)std::sqrt(std::max(1.0 - md.rho * md.rho, 0.0));
+  A.err = err;
+
+  E->prof_begin();
+  for (uint64_t b0 = 0; b0 < total_blocks; b0 += kMaxBlocksPerLaunch) {
+    uint64_t nb = std::min(kMaxBlocksPerLaunch, total_blocks - b0);
+    A.path_base = b0 * kBlock;
+    A.n_paths = std::min<uint64_t>(nb * kBlock, n_paths - b0 * kBlock);
+    A.partials = partials + b0 * nout;
+    A.pv_out = pv_dev ? pv_dev + b0 * kBlock : nullptr;
+    A.z_out = z_dev ? z_dev + b0 * kBlock * grid.size() * M : nullptr;
+    check_hip(launch_path(kind, K, ns, nout, debug, A, (int)nb, E->stream), "path kernel launch");
+  }
+  E->prof_end((double)n_paths * (double)(grid.size() > 0 ? grid.size() - 1 : 0));
+  check_hip(launch_finalize(partials, (int)total_blocks, nout, prog.n_outputs, stats, E->stream), "finalize launch");
+  std::vector<KStats> out(prog.n_outputs);
+  unsigned err_bits = 0;
+  check_hip(hipMemcpyAsync(out.data(), stats, prog.n_outputs * sizeof(KStats), hipMemcpyDeviceToHost, E->stream),
+            "copy stats");
+  check_hip(hipMemcpyAsync(&err_bits, err, sizeof(unsigned), hipMemcpyDeviceToHost, E->stream), "copy err");
+  if (pv_host)
+    check_hip(hipMemcpyAsync(pv_host, pv_dev, n_paths * sizeof(double), hipMemcpyDeviceToHost, E->stream), "copy pv");
+  if (z_host)
+    check_hip(hipMemcpyAsync(z_host, z_dev, n_paths * grid.size() * M * sizeof(float), hipMemcpyDeviceToHost,
+                             E->stream),
+              "copy normals");
+  check_hip(hipStreamSynchronize(E->stream), "stream sync");
+  if (err_bits) {
+    int code = lowest_error_code(err_bits);
+    raise(code, error_text(code));
+  }
+  return out;
+}
+
+// Shared front half of price_gbm: parse, plan, grid, validation.
+PreparedContract prepare(const char* json, const ModelDesc& md, const kt_mc_config* cfg) {
+  if (!cfg) raise(KT_ERR_INVALID_ARGUMENT, "null config");
+  if (cfg->antithetic || cfg->control_variate)
+    raise(KT_ERR_UNSUPPORTED, "variance reduction is not on the GPU engine's path");
+  PreparedContract pc;
+  pc.con = parse_contract(json);
+  Plan plan = compile_plan(*pc.con);
+  if (cfg->grid && cfg->n_grid) {
+    pc.grid.assign(cfg->grid, cfg->grid + cfg->n_grid);
+  } else if (cfg->grid == nullptr && cfg->n_grid == 0) {
+    pc.grid = time_grid_forced(plan, cfg->steps_per_year, md.kind == KT_MODEL_HESTON);
+  } else {
+    raise(KT_ERR_INCONSISTENT_PATH, "empty time grid");
+  }
+  validate_model(md);
+  validate_grid(pc.grid);
+  return pc;
+}
+
+}  // namespace kt
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+#define KT_GUARD_BEGIN try {
+#define KT_GUARD_END(E)                                      \
+  }                                                          \
+  catch (const kt::Error& e) {                               \
+    return (kt_status)kt::set_error((E), e.code, e.msg);     \
+  }                                                          \
+  catch (const std::bad_alloc&) {                            \
+    return (kt_status)kt::set_error((E), KT_ERR_OUT_OF_MEMORY, "host allocation failed"); \
+  }                                                          \
+  catch (const std::exception& ex) {                         \
+    return (kt_status)kt::set_error((E), KT_ERR_DEVICE, ex.what()); \
+  }
+
+namespace kt {
+int set_error(kt_engine* E, int code, const std::string& msg) {
+  if (E) E->err = msg;
+  g_last_error = msg;
+  return code;
+}
+}  // namespace kt
+
+extern "C" {
+
+kt_status kt_engine_create(int device, kt_engine** out) {
+  if (!out) return (kt_status)set_error(nullptr, KT_ERR_INVALID_ARGUMENT, "null out");
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0)
+    return (kt_status)set_error(nullptr, KT_ERR_DEVICE, "no HIP device available");
+  if (device < 0 || device >= n) return (kt_status)set_error(nullptr, KT_ERR_INVALID_ARGUMENT, "bad device ordinal");
+  if (hipSetDevice(device) != hipSuccess) return (kt_status)set_error(nullptr, KT_ERR_DEVICE, "hipSetDevice failed");
+  kt_engine* E = new kt_engine();
+  E->device = device;
+  if (hipStreamCreateWithFlags(&E->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete E;
+    return (kt_status)set_error(nullptr, KT_ERR_DEVICE, "hipStreamCreate failed");
+  }
+  *out = E;
+  return KT_OK;
+}
+
+void kt_engine_destroy(kt_engine* engine) { delete engine; }
+
+const char* kt_last_error(const kt_engine* engine) { return engine ? engine->err.c_str() : ""; }
+const char* kt_last_error_global(void) { return g_last_error.c_str(); }
+
+kt_status kt_price(kt_engine* E, const char* json, const kt_model* model, const kt_mc_config* cfg,
+                   kt_price_result* out) {
+  KT_GUARD_BEGIN
+  if (!E || !out) raise(KT_ERR_INVALID_ARGUMENT, "null engine or output");
+  check_hip(hipSetDevice(E->device), "hipSetDevice");
+  ModelDesc md = model_from_abi(model);
+  PreparedContract pc = prepare(json, md, cfg);
+  *out = kt_price_result{};
+  if (cfg->n_paths == 0) return KT_OK;
+  Program prog = lower({pc.con.get()}, pc.grid, md.names);
+  auto st = run_program(E, prog, md, pc.grid, cfg->rate, cfg->n_paths, cfg->seed, nullptr, nullptr);
+  *out = result_from_stats(st[0]);
+  E->err.clear();
+  return KT_OK;
+  KT_GUARD_END(E)
+}
+
+kt_status kt_debug_paths(kt_engine* E, const char* json, const kt_model* model, const kt_mc_config* cfg,
+                         double* pv_out, float* normals_out, double* grid_out, uint64_t grid_cap, uint64_t* n_grid,
+                         kt_price_result* out) {
+  KT_GUARD_BEGIN
+  if (!E || !out || !pv_out) raise(KT_ERR_INVALID_ARGUMENT, "null engine or output");
+  check_hip(hipSetDevice(E->device), "hipSetDevice");
+  ModelDesc md = model_from_abi(model);
+  PreparedContract pc = prepare(json, md, cfg);
+  if (n_grid) *n_grid = pc.grid.size();
+  if (grid_out)
+    for (size_t i = 0; i < pc.grid.size() && i < grid_cap; ++i) grid_out[i] = pc.grid[i];
+  *out = kt_price_result{};
+  if (cfg->n_paths == 0) return KT_OK;
+  Program prog = lower({pc.con.get()}, pc.grid, md.names);
+  auto st = run_program(E, prog, md, pc.grid, cfg->rate, cfg->n_paths, cfg->seed, pv_out, normals_out);
+  *out = result_from_stats(st[0]);
+  return KT_OK;
+  KT_GUARD_END(E)
+}
+
+kt_status kt_price_batch(kt_engine* E, const char* const* jsons, uint64_t n_contracts, const kt_model* model,
+                         const kt_mc_config* cfg, kt_price_result* out) {
+  KT_GUARD_BEGIN
+  if (!E) raise(KT_ERR_INVALID_ARGUMENT, "null engine");
+  if (n_contracts == 0) return KT_OK;
+  if (!jsons || !out || !cfg) raise(KT_ERR_INVALID_ARGUMENT, "null argument");
+  if (cfg->antithetic || cfg->control_variate)
+    raise(KT_ERR_UNSUPPORTED, "variance reduction is not on the GPU engine's path");
+  check_hip(hipSetDevice(E->device), "hipSetDevice");
+  ModelDesc md = model_from_abi(model);
+  // Merged plan (src/pricer.rs:260-278).
+  std::vector<std::unique_ptr<Con>> cons;
+  Plan merged;
+  for (uint64_t i = 0; i < n_contracts; ++i) {
+    cons.push_back(parse_contract(jsons[i]));
+    Plan p = compile_plan(*cons.back());
+    merged.fixed_dates.insert(merged.fixed_dates.end(), p.fixed_dates.begin(), p.fixed_dates.end());
+    merged.horizon = std::max(merged.horizon, p.horizon);
+    merged.needs_fine_grid = merged.needs_fine_grid || p.needs_fine_grid;
+  }
+  std::vector<double> grid = time_grid_forced(merged, cfg->steps_per_year, md.kind == KT_MODEL_HESTON);
+  validate_model(md);
+  validate_grid(grid);
+  if (cfg->n_paths == 0) {
+    for (uint64_t i = 0; i < n_contracts; ++i) out[i] = kt_price_result{};
+    return KT_OK;
+  }
+  // Up to kMaxOutputs contracts share one launch; a group that does not fit
+  // the register file is split.
+  std::vector<std::pair<size_t, size_t>> todo;
+  for (size_t b = 0; b < n_contracts; b += kMaxOutputs) todo.push_back({b, std::min<size_t>(b + kMaxOutputs, n_contracts)});
+  while (!todo.empty()) {
+    auto [b, e] = todo.back();
+    todo.pop_back();
+    std::vector<const Con*> group;
+    for (size_t i = b; i < e; ++i) group.push_back(cons[i].get());
+    Program prog;
+    try {
+      prog = lower(group, grid, md.names);
+    } catch (const Error& err) {
+      if (err.code == KT_ERR_UNSUPPORTED && e - b > 1 && err.msg.find("registers") != std::string::npos) {
+        size_t mid = b + (e - b) / 2;
+        todo.push_back({mid, e});
+        todo.push_back({b, mid});
+        continue;
+      }
+      throw;
+    }
+    auto st = run_program(E, prog, md, grid, cfg->rate, cfg->n_paths, cfg->seed, nullptr, nullptr);
+    for (size_t i = b; i < e; ++i) out[i] = result_from_stats(st[i - b]);
+  }
+  E->err.clear();
+  return KT_OK;
+  KT_GUARD_END(E)
+}
+
+kt_status kt_greeks_gbm(kt_engine* E, const char* json, const char* asset, double s0, double sigma,
+                        const kt_mc_config* cfg, const kt_bumps* bumps, kt_greeks* out) {
+  KT_GUARD_BEGIN
+  if (!E || !out || !cfg || !asset) raise(KT_ERR_INVALID_ARGUMENT, "null argument");
+  check_hip(hipSetDevice(E->device), "hipSetDevice");
+  kt_bumps h = bumps ? *bumps : kt_bumps{1e-2, 1e-2, 1e-4};
+  greeks_gbm_crn(E, json, asset, s0, sigma, cfg, h, out);
+  E->err.clear();
+  return KT_OK;
+  KT_GUARD_END(E)
+}
+
+kt_status kt_greeks_bump(kt_engine* E, const char* json, const kt_model* model, const kt_mc_config* cfg,
+                         const double* h, uint64_t n_params, double* out, double* stderr_out) {
+  KT_GUARD_BEGIN
+  if (!E || !out || !cfg || !h) raise(KT_ERR_INVALID_ARGUMENT, "null argument");
+  check_hip(hipSetDevice(E->device), "hipSetDevice");
+  greeks_bump(E, json, model, cfg, h, n_params, out, stderr_out);
+  E->err.clear();
+  return KT_OK;
+  KT_GUARD_END(E)
+}
+
+kt_status kt_greeks_adjoint(kt_engine* E, const char* json, const kt_model* model, const kt_mc_config* cfg,
+                            double* out, double* stderr_out, uint64_t n_out) {
+  KT_GUARD_BEGIN
+  if (!E || !out || !cfg) raise(KT_ERR_INVALID_ARGUMENT, "null argument");
+  check_hip(hipSetDevice(E->device), "hipSetDevice");
+  greeks_adjoint(E, json, model, cfg, out, stderr_out, n_out);
+  E->err.clear();
+  return KT_OK;
+  KT_GUARD_END(E)
+}
+
+kt_status kt_price_american(kt_engine* E, const char* json, const double* dates, uint64_t n_dates,
+                            const kt_model* model, const kt_mc_config* cfg, int32_t n_basis, kt_price_result* out) {
+  KT_GUARD_BEGIN
+  if (!E || !out || !cfg) raise(KT_ERR_INVALID_ARGUMENT, "null argument");
+  check_hip(hipSetDevice(E->device), "hipSetDevice");
+  price_american(E, json, dates, n_dates, model, cfg, n_basis, out);
+  E->err.clear();
+  return KT_OK;
+  KT_GUARD_END(E)
+}
+
+kt_status kt_price_portfolio(kt_engine* E, const kt_job* jobs, uint64_t n_jobs, kt_price_result* out) {
+  KT_GUARD_BEGIN
+  if (!E) raise(KT_ERR_INVALID_ARGUMENT, "null engine");
+  if (n_jobs == 0) return KT_OK;
+  if (!jobs || !out) raise(KT_ERR_INVALID_ARGUMENT, "null argument");
+  check_hip(hipSetDevice(E->device), "hipSetDevice");
+  price_portfolio(E, jobs, n_jobs, out);
+  E->err.clear();
+  return KT_OK;
+  KT_GUARD_END(E)
+}
+
+kt_status kt_debug_philox(kt_engine* E, const uint32_t* ctr, const uint32_t* key, uint64_t n, uint32_t* out) {
+  KT_GUARD_BEGIN
+  if (!E || !ctr || !key || !out) raise(KT_ERR_INVALID_ARGUMENT, "null argument");
+  check_hip(hipSetDevice(E->device), "hipSetDevice");
+  if (n == 0) return KT_OK;
+  uint32_t* d = (uint32_t*)E->ensure(E->scratch, n * 32);
+  check_hip(hipMemcpyAsync(d, ctr, n * 16, hipMemcpyHostToDevice, E->stream), "copy");
+  check_hip(launch_debug_philox(d, key[0], key[1], n, d + 4 * n, E->stream), "launch");
+  check_hip(hipMemcpyAsync(out, d + 4 * n, n * 16, hipMemcpyDeviceToHost, E->stream), "copy");
+  check_hip(hipStreamSynchronize(E->stream), "sync");
+  return KT_OK;
+  KT_GUARD_END(E)
+}
+
+kt_status kt_debug_normals(kt_engine* E, const uint32_t* words, uint64_t n, float* out) {
+  KT_GUARD_BEGIN
+  if (!E || !words || !out) raise(KT_ERR_INVALID_ARGUMENT, "null argument");
+  check_hip(hipSetDevice(E->device), "hipSetDevice");
+  if (n == 0) return KT_OK;
+  uint32_t* d = (uint32_t*)E->ensure(E->scratch, n * 8);
+  check_hip(hipMemcpyAsync(d, words, n * 4, hipMemcpyHostToDevice, E->stream), "copy");
+  check_hip(launch_debug_normals(d, n, (float*)(d + n), E->stream), "launch");
+  check_hip(hipMemcpyAsync(out, d + n, n * 4, hipMemcpyDeviceToHost, E->stream), "copy");
+  check_hip(hipStreamSynchronize(E->stream), "sync");
+  return KT_OK;
+  KT_GUARD_END(E)
+}
+
+kt_status kt_engine_set_profiling(kt_engine* E, int32_t enabled) {
+  if (!E) return (kt_status)set_error(nullptr, KT_ERR_INVALID_ARGUMENT, "null engine");
+  E->profiling = enabled != 0;
+  return KT_OK;
+}
+
+kt_status kt_engine_kernel_time(kt_engine* E, double* total_ms, uint64_t* n_launches, double* path_steps,
+                                int32_t reset) {
+  if (!E) return (kt_status)set_error(nullptr, KT_ERR_INVALID_ARGUMENT, "null engine");
+  if (total_ms) *total_ms = E->prof_ms;
+  if (n_launches) *n_launches = E->prof_launches;
+  if (path_steps) *path_steps = E->prof_path_steps;
+  if (reset) {
+    E->prof_ms = 0.0;
+    E->prof_launches = 0;
+    E->prof_path_steps = 0.0;
+  }
+  return KT_OK;
+}
+
+kt_status kt_plan(const char* json, uint64_t spy, int32_t force_fine, double* grid, uint64_t* n_grid, double* dates,
+                  uint64_t* n_dates, uint64_t cap, int32_t* needs_fine, double* horizon, char* assets,
+                  uint64_t assets_cap) {
+  KT_GUARD_BEGIN
+  auto c = parse_contract(json);
+  Plan p = compile_plan(*c);
+  auto g = time_grid_forced(p, spy, force_fine != 0);
+  if (n_grid) *n_grid = g.size();
+  if (grid)
+    for (size_t i = 0; i < g.size() && i < cap; ++i) grid[i] = g[i];
+  if (n_dates) *n_dates = p.fixed_dates.size();
+  if (dates)
+    for (size_t i = 0; i < p.fixed_dates.size() && i < cap; ++i) dates[i] = p.fixed_dates[i];
+  if (needs_fine) *needs_fine = p.needs_fine_grid ? 1 : 0;
+  if (horizon) *horizon = p.horizon;
+  if (assets && assets_cap) {
+    std::string s;
+    for (size_t i = 0; i < p.assets.size(); ++i) {
+      if (i) s += '\n';
+      s += p.assets[i];
+    }
+    snprintf(assets, assets_cap, "%s", s.c_str());
+  }
+  return KT_OK;
+  KT_GUARD_END(nullptr)
+}
+
+kt_status kt_describe_program(const char* json, const kt_model* model, const kt_mc_config* cfg, char* buf,
+                              uint64_t cap) {
+  KT_GUARD_BEGIN
+  ModelDesc md = model_from_abi(model);
+  PreparedContract pc = prepare(json, md, cfg);
+  Program prog = lower({pc.con.get()}, pc.grid, md.names);
+  std::string s = prog.describe();
+  if (buf && cap) snprintf(buf, cap, "%s", s.c_str());
+  return KT_OK;
+  KT_GUARD_END(nullptr)
+}
+
+}  // extern "C"

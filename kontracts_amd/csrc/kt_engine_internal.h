@@ -1,0 +1,84 @@
+// kt_engine_internal.h — engine state and helpers shared by the C ABI
+// translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <memory>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../../include/kontracts.h"
+#include "kt_compiler.h"
+#include "kt_launch.h"
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t cap = 0;
+};
+
+struct kt_engine {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  DevBuf arena, partials, stats, errw, pv, z, scratch, paths, aux;
+  std::vector<char> host_stage;
+  bool profiling = false;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  double prof_ms = 0.0, prof_path_steps = 0.0;
+  uint64_t prof_launches = 0;
+  // Bracket a group of launches with events when profiling is enabled.
+  void prof_begin();
+  void prof_end(double path_steps);
+  ~kt_engine();
+  void* ensure(DevBuf& b, size_t bytes);
+  std::vector<void*> upload(const std::vector<std::pair<const void*, size_t>>& blobs);
+};
+
+namespace kt {
+
+struct ModelDesc {
+  int kind = 0;
+  std::vector<std::string> names;
+  std::vector<double> s0, mu, sigma;
+  std::vector<double> chol;  // K x K lower-triangular (correlated GBM)
+  double v0 = 0, kappa = 0, theta = 0, sigma_v = 0, rho = 0;
+  int scheme = 0;
+};
+
+struct PreparedContract {
+  std::unique_ptr<Con> con;
+  std::vector<double> grid;
+};
+
+extern const double kZ95;
+extern const double kLog2e;
+
+ModelDesc model_from_abi(const kt_model* m);
+void validate_model(const ModelDesc& d);
+int model_normals(const ModelDesc& d);
+int model_kernel_kind(const ModelDesc& d);
+std::vector<float> model_table(const ModelDesc& d, const std::vector<double>& grid, int* stride);
+kt_price_result result_from_stats(const KStats& s);
+void check_hip(hipError_t e, const char* what);
+int set_error(kt_engine* E, int code, const std::string& msg);
+int lowest_error_code(unsigned bits);
+const char* error_text(int code);
+
+PreparedContract prepare(const char* json, const ModelDesc& md, const kt_mc_config* cfg);
+std::vector<KStats> run_program(kt_engine* E, const Program& prog, const ModelDesc& md,
+                                const std::vector<double>& grid, double rate, uint64_t n_paths, uint64_t seed,
+                                double* pv_host, float* z_host);
+
+// Implemented in kt_greeks.cpp / kt_lsm.cpp / kt_portfolio.cpp.
+void greeks_gbm_crn(kt_engine* E, const char* json, const char* asset, double s0, double sigma,
+                    const kt_mc_config* cfg, const kt_bumps& h, kt_greeks* out);
+void greeks_bump(kt_engine* E, const char* json, const kt_model* model, const kt_mc_config* cfg, const double* h,
+                 uint64_t n_params, double* out, double* stderr_out);
+void greeks_adjoint(kt_engine* E, const char* json, const kt_model* model, const kt_mc_config* cfg, double* out,
+                    double* stderr_out, uint64_t n_out);
+void price_american(kt_engine* E, const char* json, const double* dates, uint64_t n_dates, const kt_model* model,
+                    const kt_mc_config* cfg, int n_basis, kt_price_result* out);
+void price_portfolio(kt_engine* E, const kt_job* jobs, uint64_t n_jobs, kt_price_result* out);
+
+}  // namespace kt

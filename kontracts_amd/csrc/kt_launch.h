@@ -1,0 +1,58 @@
+// kt_launch.h — kernel arguments and launch entry points shared by the engine
+// (host) and the gfx950 kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "kt_program.h"
+
+namespace kt {
+
+enum ModelKind { MK_GBM = 0, MK_HESTON = 1, MK_BASKET = 2 };
+
+// Per grid point model tables (floats, tab_stride per point):
+//   GBM, basket: for each asset (a, b): log2(S) += a + b * w, where
+//     a = (mu - sigma^2/2) dt log2(e), b = sigma sqrt(dt) log2(e)   (0 if dt <= 0)
+//   Heston: {c0, c1, c2, e0, d1, d2, 0, 0}:
+//     x += c0 + c1 vp + c2 sqrt(vp) z1          (x = log2(S/S0))
+//     v' = v + e0 vp + d1 + d2 sqrt(vp) (rho z1 + rho' z2)
+//     c0 = r dt log2e, c1 = -dt log2e / 2, c2 = sqrt(dt) log2e,
+//     e0 = -kappa dt, d1 = kappa theta dt, d2 = sigma_v sqrt(dt)
+constexpr int kHestonStride = 8;
+
+struct PathArgs {
+  const KOp* ops;
+  const KStep* steps;
+  const KOp* init;
+  const double* df;   // [n_grid] discount factors exp(-r t_k)
+  const float* tab;   // [n_grid][tab_stride]
+  int n_init;
+  int n_grid;
+  int tab_stride;
+  int heston_full_truncation;
+  uint64_t n_paths;    // paths in this launch
+  uint64_t path_base;  // global index of the launch's first path
+  uint32_t seed_lo, seed_hi;
+  float s0[kMaxAssets];
+  float chol[kMaxAssets * kMaxAssets];  // lower-triangular, row-major K x K
+  float v0, rho, rho_perp;
+  KStats* partials;  // [n_blocks][NOUT]
+  unsigned* err;     // bit e set <=> error code e raised on a live path
+  double* pv_out;    // debug: per-path pv of output 0 (nullable)
+  float* z_out;      // debug: normals [path][grid point][component] (nullable)
+};
+
+constexpr int kBlock = 256;
+
+// Path kernel variants available: returns false if no variant matches.
+bool path_variant_exists(int model, int k_assets, int n_slots, int n_outputs, bool debug, int* ns_chosen,
+                         int* nout_chosen);
+hipError_t launch_path(int model, int k_assets, int ns, int nout, bool debug, const PathArgs& a, int n_blocks,
+                       hipStream_t s);
+hipError_t launch_finalize(const KStats* partials, int n_blocks, int nout, int n_used, KStats* out,
+                           hipStream_t s);
+hipError_t launch_debug_philox(const uint32_t* ctr, uint32_t k0, uint32_t k1, uint64_t n, uint32_t* out,
+                               hipStream_t s);
+hipError_t launch_debug_normals(const uint32_t* words, uint64_t n, float* out, hipStream_t s);
+
+}  // namespace kt

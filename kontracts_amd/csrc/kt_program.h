@@ -1,0 +1,81 @@
+// kt_program.h — the flat per-path op program shared by the host compiler and
+// the gfx950 path kernels.
+//
+// A contract is lowered (kt_lower.cpp) into one short op list per time-grid
+// step.  The path kernel runs the list for step k after advancing the model to
+// grid point k.  Every op reads and writes a small register file of float
+// slots that lives in VGPRs; slot and op indices are wave-uniform, so the
+// hardware indexes the register file with s_set_gpr_idx and never touches
+// scratch memory.
+//
+// Slot conventions: slot 0 holds 0.0, slot 1 holds 1.0.  Booleans are 0.0/1.0.
+#pragma once
+#include <stdint.h>
+
+namespace kt {
+
+enum OpCode : uint8_t {
+  OP_NOP = 0,
+  OP_CONST,   // r[d] = imm
+  OP_SPOT,    // r[d] = spot of program asset a at this grid point
+  OP_NEG,     // r[d] = -r[a]
+  OP_ADD,     // r[d] = r[a] + r[b]
+  OP_SUB,
+  OP_MUL,
+  OP_DIV,
+  OP_MAX,     // fmaxf: a NaN operand yields the other (Rust f64::max)
+  OP_MIN,
+  OP_ADDI,    // r[d] = r[a] + imm
+  OP_SUBI,    // r[d] = r[a] - imm
+  OP_RSUBI,   // r[d] = imm - r[a]
+  OP_MULI,
+  OP_DIVI,    // r[d] = r[a] / imm
+  OP_RDIVI,   // r[d] = imm / r[a]
+  OP_MAXI,
+  OP_MINI,
+  OP_GE,      // r[d] = r[a] >= r[b]
+  OP_GT,
+  OP_LE,
+  OP_LT,
+  OP_GEI,     // r[d] = r[a] >= imm
+  OP_GTI,
+  OP_LEI,
+  OP_LTI,
+  OP_AND,     // r[d] = r[a] * r[b]
+  OP_OR,      // r[d] = max(r[a], r[b])
+  OP_NOT,     // r[d] = 1 - r[a]
+  OP_WHEN,    // acquisition: n = r[a]*(1-r[d])*r[b]; r[d] += n; r[d+1] = n
+  OP_UNTIL,   // knock-out:   r[d] = max(r[d], r[a]*r[b])
+  OP_EMIT,    // pv[out] += sign * r[a] * r[b] * df[k]   (out = d, sign = imm)
+  OP_RMAX,    // r[d] = max(r[d], r[a])
+  OP_RMIN,    // r[d] = min(r[d], r[a])
+  OP_AVG,     // Kahan: r[d] (sum) and r[d+1] (compensation) accumulate r[a]
+  OP_FLAGERR, // if r[a] != 0 on any live path: raise error code (int)imm
+  OP_COUNT_
+};
+
+struct KOp {
+  uint8_t code, d, a, b;
+  float imm;
+};
+static_assert(sizeof(KOp) == 8, "KOp must pack into 8 bytes");
+
+// Per grid point: where its op list starts and how long it is.
+struct KStep {
+  int32_t begin;
+  int32_t count;
+};
+
+// Result statistics of one output over a set of paths: count, mean and the
+// sum of squared deviations (combined with Chan et al.'s pairwise formula).
+struct KStats {
+  double n, mean, m2;
+};
+
+constexpr int kSlotZero = 0;
+constexpr int kSlotOne = 1;
+constexpr int kMaxSlots = 32;   // register file size of the path kernels
+constexpr int kMaxOutputs = 8;  // pv accumulators per path in one launch
+constexpr int kMaxAssets = 8;
+
+}  // namespace kt

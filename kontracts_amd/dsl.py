@@ -1,0 +1,657 @@
+"""Contract algebra DSL and the pricing API, mirroring the reference's Python
+package (python/kontract/_kontract.pyi:11-240, src/bindings.rs:55-472).
+
+Contracts are plain trees; ``to_json`` writes serde's externally tagged form
+(src/ast.rs:210-218), which is what crosses the C ABI.  Pricing goes to the
+GPU engine through ``kontracts_amd._native``.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import json
+import math
+from typing import Iterable, Sequence
+
+from . import _native as N
+
+USD = "USD"
+EUR = "EUR"
+GBP = "GBP"
+JPY = "JPY"
+
+__version__ = "0.1.0"
+
+
+# ---------------------------------------------------------------------------
+# JSON (serde_json writes floats with ryu: shortest round-trip digits,
+# exponents without '+' or leading zeros)
+# ---------------------------------------------------------------------------
+def _fmt_float(x: float) -> str:
+    x = float(x)
+    if not math.isfinite(x):
+        raise ValueError("non-finite numbers have no JSON form")
+    s = repr(x)
+    if "e" in s:
+        mant, exp = s.split("e")
+        sign = "-" if exp.startswith("-") else ""
+        digits = exp.lstrip("+-").lstrip("0") or "0"
+        if "." in mant and mant.endswith(".0"):
+            mant = mant[:-2]
+        return f"{mant}e{sign}{digits}"
+    return s
+
+
+def _dump(node) -> str:
+    if isinstance(node, dict):
+        return "{" + ",".join(json.dumps(k) + ":" + _dump(v) for k, v in node.items()) + "}"
+    if isinstance(node, list):
+        return "[" + ",".join(_dump(v) for v in node) + "]"
+    if isinstance(node, bool):
+        return "true" if node else "false"
+    if isinstance(node, float):
+        return _fmt_float(node)
+    if node is None:
+        return "null"
+    if isinstance(node, str):
+        return json.dumps(node, ensure_ascii=False)
+    if isinstance(node, int):
+        return _fmt_float(float(node))
+    raise TypeError(f"cannot serialise {type(node)}")
+
+
+def _floats_as_float(node):
+    """json.loads gives ints for integral numbers; the AST holds f64."""
+    if isinstance(node, dict):
+        return {k: _floats_as_float(v) for k, v in node.items()}
+    if isinstance(node, list):
+        return [_floats_as_float(v) for v in node]
+    if isinstance(node, int) and not isinstance(node, bool):
+        return float(node)
+    return node
+
+
+# ---------------------------------------------------------------------------
+# Observable / Condition / Contract
+# ---------------------------------------------------------------------------
+def _obs(x) -> "Observable":
+    if isinstance(x, Observable):
+        return x
+    if isinstance(x, (int, float)) and not isinstance(x, bool):
+        return Observable({"Const": float(x)})
+    raise TypeError("expected a number or an Observable")
+
+
+class Observable:
+    """A real-valued process (src/ast.rs:34-71)."""
+
+    __slots__ = ("node",)
+    __array_priority__ = 1000
+
+    def __init__(self, node):
+        self.node = node
+
+    # payoff helpers
+    def clip(self, floor: float) -> "Observable":
+        return self.max(float(floor))
+
+    def max(self, other) -> "Observable":
+        return Observable({"Max": [self.node, _obs(other).node]})
+
+    def min(self, other) -> "Observable":
+        return Observable({"Min": [self.node, _obs(other).node]})
+
+    # temporal observables (src/ast.rs:57-70)
+    def average(self) -> "Observable":
+        return average(self)
+
+    def average_over(self, from_year: float, to_year: float) -> "Observable":
+        return average_over(self, from_year, to_year)
+
+    def running_max(self) -> "Observable":
+        return running_max(self)
+
+    def running_min(self) -> "Observable":
+        return running_min(self)
+
+    # pipeline style
+    def add(self, o):
+        return self + o
+
+    def sub(self, o):
+        return self - o
+
+    def mul(self, o):
+        return Observable({"Mul": [self.node, _obs(o).node]})
+
+    def div(self, o):
+        return self / o
+
+    def scale(self, contract: "Contract") -> "Contract":
+        return Contract({"Scale": [self.node, contract.node]})
+
+    def __add__(self, o):
+        return Observable({"Add": [self.node, _obs(o).node]})
+
+    def __radd__(self, o):
+        return Observable({"Add": [_obs(o).node, self.node]})
+
+    def __sub__(self, o):
+        return Observable({"Sub": [self.node, _obs(o).node]})
+
+    def __rsub__(self, o):
+        return Observable({"Sub": [_obs(o).node, self.node]})
+
+    def __truediv__(self, o):
+        return Observable({"Div": [self.node, _obs(o).node]})
+
+    def __rtruediv__(self, o):
+        return Observable({"Div": [_obs(o).node, self.node]})
+
+    def __neg__(self):
+        return Observable({"Neg": self.node})
+
+    def __mul__(self, o):
+        if isinstance(o, Contract):
+            return Contract({"Scale": [self.node, o.node]})
+        return Observable({"Mul": [self.node, _obs(o).node]})
+
+    def __rmul__(self, o):
+        return Observable({"Mul": [_obs(o).node, self.node]})
+
+    def __ge__(self, o):
+        return Condition({"Ge": [self.node, _obs(o).node]})
+
+    def __gt__(self, o):
+        return Condition({"Gt": [self.node, _obs(o).node]})
+
+    def __le__(self, o):
+        return Condition({"Le": [self.node, _obs(o).node]})
+
+    def __lt__(self, o):
+        return Condition({"Lt": [self.node, _obs(o).node]})
+
+    def __eq__(self, other):
+        return isinstance(other, Observable) and other.node == self.node
+
+    __hash__ = None
+
+    def __repr__(self):
+        return f"Observable({_dump(self.node)})"
+
+
+class Condition:
+    """A boolean process (src/ast.rs:142-161)."""
+
+    __slots__ = ("node",)
+
+    def __init__(self, node):
+        self.node = node
+
+    def __and__(self, o: "Condition"):
+        return Condition({"And": [self.node, o.node]})
+
+    def __or__(self, o: "Condition"):
+        return Condition({"Or": [self.node, o.node]})
+
+    def __invert__(self):
+        return Condition({"Not": self.node})
+
+    def __eq__(self, other):
+        return isinstance(other, Condition) and other.node == self.node
+
+    __hash__ = None
+
+    def __repr__(self):
+        return f"Condition({_dump(self.node)})"
+
+
+class PriceResult:
+    """src/pricer.rs:58-71."""
+
+    __slots__ = ("price", "std_error", "sample_std", "ci95_low", "ci95_high", "n_paths")
+
+    def __init__(self, r: N.KtPriceResult):
+        self.price = r.price
+        self.std_error = r.std_error
+        self.sample_std = r.sample_std
+        self.ci95_low = r.ci95_low
+        self.ci95_high = r.ci95_high
+        self.n_paths = int(r.n_paths)
+
+    def paths_for_tolerance(self, tol: float) -> int:
+        """src/pricer.rs:81-86."""
+        if tol <= 0.0 or self.sample_std == 0.0:
+            return 0
+        return int(math.ceil((1.959963984540054 * self.sample_std / tol) ** 2))
+
+    def __repr__(self):
+        return (f"PriceResult(price={self.price:.6f}, std_error={self.std_error:.6f}, "
+                f"ci95=[{self.ci95_low:.6f}, {self.ci95_high:.6f}], n_paths={self.n_paths})")
+
+
+class Greeks:
+    """src/greeks.rs:43-54, plus the Monte Carlo standard error of each value."""
+
+    __slots__ = ("price", "delta", "gamma", "vega", "rho", "std_errors")
+
+    def __init__(self, g: N.KtGreeks):
+        self.price, self.delta, self.gamma, self.vega, self.rho = g.price, g.delta, g.gamma, g.vega, g.rho
+        self.std_errors = list(g.std_error)
+
+    def __repr__(self):
+        return (f"Greeks(price={self.price:.6f}, delta={self.delta:.6f}, gamma={self.gamma:.6f}, "
+                f"vega={self.vega:.6f}, rho={self.rho:.6f})")
+
+
+class Contract:
+    """Peyton-Jones combinators (src/ast.rs:187-206)."""
+
+    __slots__ = ("node",)
+
+    def __init__(self, node):
+        self.node = node
+
+    def __matmul__(self, cond: Condition) -> "Contract":
+        return Contract({"When": [cond.node, self.node]})
+
+    def when(self, cond: Condition) -> "Contract":
+        return self @ cond
+
+    def __add__(self, other: "Contract") -> "Contract":
+        return Contract({"And": [self.node, other.node]})
+
+    def __neg__(self) -> "Contract":
+        return Contract({"Give": self.node})
+
+    def until(self, cond: Condition) -> "Contract":
+        return Contract({"Until": [cond.node, self.node]})
+
+    def anytime(self, cond: Condition) -> "Contract":
+        return Contract({"Anytime": [cond.node, self.node]})
+
+    def or_(self, other: "Contract") -> "Contract":
+        return Contract({"Or": [self.node, other.node]})
+
+    def give(self) -> "Contract":
+        return -self
+
+    def to_json(self) -> str:
+        return _dump(self.node)
+
+    @staticmethod
+    def from_json(s: str) -> "Contract":
+        try:
+            node = json.loads(s)
+        except json.JSONDecodeError as e:
+            raise N.KontractError(5, str(e)) from None
+        node = _floats_as_float(node)
+        # Validate the shape with the engine's own reader (host only); only
+        # deserialisation errors matter here, as in Contract::from_json.
+        try:
+            plan(Contract(node))
+        except N.KontractError as e:
+            if e.code == 5:
+                raise
+        return Contract(node)
+
+    def __eq__(self, other):
+        return isinstance(other, Contract) and other.node == self.node
+
+    __hash__ = None
+
+    def __repr__(self):
+        return f"Contract({self.to_json()})"
+
+    # --- pricing (src/bindings.rs:350-467) --------------------------------
+    def price(self, model, n_paths: int = 100_000, seed: int = 42, steps_per_year: int = 50,
+              antithetic: bool = False, control_variate: bool = False, grid: Sequence[float] | None = None,
+              device: int | None = None) -> PriceResult:
+        m = _model_struct(model)
+        cfg = _cfg(n_paths, seed, steps_per_year, model.rate, grid, antithetic, control_variate)
+        eng = N.engine(device)
+        out = N.KtPriceResult()
+        N.check(N.lib().kt_price(eng.handle, self.to_json().encode(), C.byref(m.s), C.byref(cfg.s),
+                                 C.byref(out)), eng.handle)
+        return PriceResult(out)
+
+    def greeks(self, model: "GBM", n_paths: int = 200_000, seed: int = 42, steps_per_year: int = 50,
+               bumps: tuple[float, float, float] | None = None, device: int | None = None) -> Greeks:
+        if not isinstance(model, GBM):
+            raise TypeError("greeks expects a GBM model")
+        cfg = _cfg(n_paths, seed, steps_per_year, model.r, None)
+        b = N.KtBumps(*(bumps or (1e-2, 1e-2, 1e-4)))
+        eng = N.engine(device)
+        out = N.KtGreeks()
+        N.check(N.lib().kt_greeks_gbm(eng.handle, self.to_json().encode(), model.asset.encode(), model.s0,
+                                      model.sigma, C.byref(cfg.s), C.byref(b), C.byref(out)), eng.handle)
+        return Greeks(out)
+
+    def price_american(self, model, exercise_dates: Sequence[float], n_paths: int = 100_000, seed: int = 42,
+                       n_basis: int = 3, device: int | None = None) -> PriceResult:
+        m = _model_struct(model)
+        cfg = _cfg(n_paths, seed, 50, model.rate, None)
+        dates = (C.c_double * max(len(exercise_dates), 1))(*[float(d) for d in exercise_dates])
+        eng = N.engine(device)
+        out = N.KtPriceResult()
+        N.check(N.lib().kt_price_american(eng.handle, self.to_json().encode(), dates, len(exercise_dates),
+                                          C.byref(m.s), C.byref(cfg.s), int(n_basis), C.byref(out)), eng.handle)
+        return PriceResult(out)
+
+
+# ---------------------------------------------------------------------------
+# Constructors (src/ast.rs:226-448, src/bindings.rs:696-790)
+# ---------------------------------------------------------------------------
+def zero() -> Contract:
+    return Contract("Zero")
+
+
+def one(ccy: str) -> Contract:
+    return Contract({"One": ccy})
+
+
+def give(c: Contract) -> Contract:
+    return -c
+
+
+def spot(name: str) -> Observable:
+    return Observable({"Spot": name})
+
+
+S = spot
+
+
+def const_(x: float) -> Observable:
+    return Observable({"Const": float(x)})
+
+
+konst = const_
+
+
+def at(t: float) -> Condition:
+    return Condition({"At": float(t)})
+
+
+def bool_(b: bool) -> Condition:
+    return Condition({"Bool": bool(b)})
+
+
+def average(obs: Observable) -> Observable:
+    return Observable({"Average": {"obs": _obs(obs).node, "from_year": None, "to_year": None}})
+
+
+def average_over(obs: Observable, from_year: float, to_year: float) -> Observable:
+    return Observable({"Average": {"obs": _obs(obs).node, "from_year": float(from_year),
+                                   "to_year": float(to_year)}})
+
+
+def running_max(obs: Observable) -> Observable:
+    return Observable({"RunningMax": _obs(obs).node})
+
+
+def running_min(obs: Observable) -> Observable:
+    return Observable({"RunningMin": _obs(obs).node})
+
+
+def scale(obs, c: Contract) -> Contract:
+    return Contract({"Scale": [_obs(obs).node, c.node]})
+
+
+def when(cond: Condition, c: Contract) -> Contract:
+    return Contract({"When": [cond.node, c.node]})
+
+
+def anytime(cond: Condition, c: Contract) -> Contract:
+    return Contract({"Anytime": [cond.node, c.node]})
+
+
+def until(cond: Condition, c: Contract) -> Contract:
+    return Contract({"Until": [cond.node, c.node]})
+
+
+def and_(a: Contract, b: Contract) -> Contract:
+    return a + b
+
+
+def or_(a: Contract, b: Contract) -> Contract:
+    return a.or_(b)
+
+
+# Product catalogue (src/products.rs:16-84)
+def zero_coupon_bond(ccy: str, t: float) -> Contract:
+    return one(ccy) @ at(t)
+
+
+def european_call(asset: str, k: float, t: float, ccy: str = USD) -> Contract:
+    return ((spot(asset) - const_(k)).max(const_(0.0)) * one(ccy)) @ at(t)
+
+
+def european_put(asset: str, k: float, t: float, ccy: str = USD) -> Contract:
+    return ((const_(k) - spot(asset)).max(const_(0.0)) * one(ccy)) @ at(t)
+
+
+def forward(asset: str, k: float, t: float, ccy: str = USD) -> Contract:
+    return ((spot(asset) - const_(k)) * one(ccy)) @ at(t)
+
+
+def straddle(asset: str, k: float, t: float, ccy: str = USD) -> Contract:
+    return european_call(asset, k, t, ccy) + european_put(asset, k, t, ccy)
+
+
+def bull_call_spread(asset: str, k_low: float, k_high: float, t: float, ccy: str = USD) -> Contract:
+    return european_call(asset, k_low, t, ccy) + (-european_call(asset, k_high, t, ccy))
+
+
+def cash_or_nothing_call(asset: str, k: float, payout: float, t: float, ccy: str = USD) -> Contract:
+    return when(at(t), when(spot(asset) >= const_(k), scale(const_(payout), one(ccy))))
+
+
+def up_and_out_call(asset: str, k: float, barrier: float, t: float, ccy: str = USD) -> Contract:
+    return until(spot(asset) >= const_(barrier), european_call(asset, k, t, ccy))
+
+
+def down_and_out_call(asset: str, k: float, barrier: float, t: float, ccy: str = USD) -> Contract:
+    return until(spot(asset) <= const_(barrier), european_call(asset, k, t, ccy))
+
+
+# ---------------------------------------------------------------------------
+# Models (src/bindings.rs:478-567, 793-836)
+# ---------------------------------------------------------------------------
+class GBM:
+    """``GBM(s0, sigma, r, q=0.0, asset="ASSET")``: drift r - q, discount r."""
+
+    def __init__(self, s0: float, sigma: float, r: float, q: float = 0.0, asset: str = "ASSET"):
+        self.s0, self.sigma, self.r, self.q, self.asset = float(s0), float(sigma), float(r), float(q), asset
+
+    @property
+    def rate(self) -> float:
+        return self.r
+
+    def __repr__(self):
+        return f"GBM(asset={self.asset!r}, s0={self.s0}, sigma={self.sigma}, r={self.r}, q={self.q})"
+
+
+class Model:
+    """A simulator built by a factory (``heston``, ``correlated_gbm``)."""
+
+    def __init__(self, kind: int, rate: float, label: str, **params):
+        self.kind, self.rate, self.label = kind, float(rate), label
+        self.params = params
+
+    def __repr__(self):
+        return f"Model({self.label}, rate={self.rate})"
+
+
+class GbmFactor:
+    """``GbmFactor(name, s0, mu, sigma)`` (src/fx.rs:149-170)."""
+
+    def __init__(self, name: str, s0: float, mu: float, sigma: float):
+        self.name, self.s0, self.mu, self.sigma = name, float(s0), float(mu), float(sigma)
+
+    def __repr__(self):
+        return f"GbmFactor(name={self.name!r}, s0={self.s0}, mu={self.mu}, sigma={self.sigma})"
+
+
+def heston(spot: float, v0: float, kappa: float, theta: float, sigma_v: float, rho: float, r: float,
+           asset: str = "ASSET", scheme: str = "floor") -> Model:
+    """Heston model (src/bindings.rs:547-567).  ``scheme="floor"`` is the
+    reference's variance floor after each step; ``"full_truncation"`` is the
+    alternative scheme."""
+    if scheme not in ("floor", "full_truncation"):
+        raise ValueError("scheme must be 'floor' or 'full_truncation'")
+    return Model(N.KT_MODEL_HESTON, r, "Heston", asset=asset, spot=float(spot), v0=float(v0), kappa=float(kappa),
+                 theta=float(theta), sigma_v=float(sigma_v), rho=float(rho), r=float(r),
+                 scheme=0 if scheme == "floor" else 1)
+
+
+def correlated_gbm(factors: Sequence[GbmFactor], corr: Sequence[Sequence[float]], r: float) -> Model:
+    """N correlated GBMs (src/bindings.rs:826-836, src/multi_asset.rs:62-83)."""
+    factors = list(factors)
+    k = len(factors)
+    if k < 2:
+        raise N.KontractError(6, "CorrelatedGbmN: at least 2 assets required")
+    if len(corr) != k or any(len(row) != k for row in corr):
+        raise N.KontractError(6, f"CorrelatedGbmN: correlation matrix must be {k}x{k}")
+    if any(f.sigma < 0 for f in factors):
+        raise N.KontractError(6, "CorrelatedGbmN: all volatilities must be >= 0")
+    return Model(N.KT_MODEL_CORRELATED_GBM, r, "CorrelatedGbmN", factors=factors,
+                 corr=[[float(x) for x in row] for row in corr])
+
+
+class _Struct:
+    """Keeps a ctypes struct and the buffers it points to alive together."""
+
+    def __init__(self, s, keep):
+        self.s = s
+        self.keep = keep
+
+
+def _dbl(xs) -> C.Array:
+    xs = list(xs)
+    return (C.c_double * max(len(xs), 1))(*xs)
+
+
+def _model_struct(model) -> _Struct:
+    if isinstance(model, GBM):
+        names = (C.c_char_p * 1)(model.asset.encode())
+        s0, mu, sg = _dbl([model.s0]), _dbl([model.r - model.q]), _dbl([model.sigma])
+        s = N.KtModel(N.KT_MODEL_GBM, 1, names, s0, mu, sg, None, 0, 0, 0, 0, 0, 0, 0)
+        return _Struct(s, (names, s0, mu, sg))
+    if isinstance(model, Model):
+        p = model.params
+        if model.kind == N.KT_MODEL_HESTON:
+            names = (C.c_char_p * 1)(p["asset"].encode())
+            s0, mu, sg = _dbl([p["spot"]]), _dbl([p["r"]]), _dbl([0.0])
+            s = N.KtModel(N.KT_MODEL_HESTON, 1, names, s0, mu, sg, None, p["v0"], p["kappa"], p["theta"],
+                          p["sigma_v"], p["rho"], p["scheme"], 0)
+            return _Struct(s, (names, s0, mu, sg))
+        if model.kind == N.KT_MODEL_CORRELATED_GBM:
+            fs = p["factors"]
+            k = len(fs)
+            names = (C.c_char_p * k)(*[f.name.encode() for f in fs])
+            s0, mu, sg = _dbl([f.s0 for f in fs]), _dbl([f.mu for f in fs]), _dbl([f.sigma for f in fs])
+            corr = _dbl([x for row in p["corr"] for x in row])
+            s = N.KtModel(N.KT_MODEL_CORRELATED_GBM, k, names, s0, mu, sg, corr, 0, 0, 0, 0, 0, 0, 0)
+            return _Struct(s, (names, s0, mu, sg, corr))
+    raise TypeError("expected a GBM or Model")
+
+
+def _cfg(n_paths, seed, steps_per_year, rate, grid, antithetic=False, control_variate=False) -> _Struct:
+    if n_paths < 0 or seed < 0 or steps_per_year < 0:
+        raise OverflowError("can't convert negative int to unsigned")
+    g = _dbl(grid) if grid is not None else None
+    s = N.KtMcConfig(int(n_paths), int(seed) & 0xFFFFFFFFFFFFFFFF, int(steps_per_year), float(rate),
+                     g, len(grid) if grid is not None else 0, 1 if antithetic else 0, 1 if control_variate else 0)
+    return _Struct(s, (g,))
+
+
+# ---------------------------------------------------------------------------
+# Batch, portfolio, host-only plan
+# ---------------------------------------------------------------------------
+def price_batch(contracts: Iterable[Contract], model, n_paths: int = 100_000, seed: int = 42,
+                steps_per_year: int = 50, device: int | None = None) -> list[PriceResult]:
+    """price_batch_gbm (src/pricer.rs:250-294): one shared simulation."""
+    contracts = list(contracts)
+    if not contracts:
+        return []
+    m = _model_struct(model)
+    cfg = _cfg(n_paths, seed, steps_per_year, model.rate, None)
+    arr = (C.c_char_p * len(contracts))(*[c.to_json().encode() for c in contracts])
+    out = (N.KtPriceResult * len(contracts))()
+    eng = N.engine(device)
+    N.check(N.lib().kt_price_batch(eng.handle, arr, len(contracts), C.byref(m.s), C.byref(cfg.s), out), eng.handle)
+    return [PriceResult(r) for r in out]
+
+
+def price_portfolio(jobs: Sequence[tuple], device: int | None = None) -> list[PriceResult]:
+    """Heterogeneous portfolio: jobs are (contract, model, n_paths, seed,
+    steps_per_year) tuples, priced in one persistent launch."""
+    jobs = list(jobs)
+    if not jobs:
+        return []
+    keep = []
+    arr = (N.KtJob * len(jobs))()
+    for i, (con, model, n_paths, seed, spy) in enumerate(jobs):
+        m = _model_struct(model)
+        cfg = _cfg(n_paths, seed, spy, model.rate, None)
+        js = con.to_json().encode()
+        keep += [m, cfg, js]
+        arr[i].contract_json = js
+        arr[i].model = m.s
+        arr[i].cfg = cfg.s
+    out = (N.KtPriceResult * len(jobs))()
+    eng = N.engine(device)
+    N.check(N.lib().kt_price_portfolio(eng.handle, arr, len(jobs), out), eng.handle)
+    return [PriceResult(r) for r in out]
+
+
+class Plan:
+    """compile() result (src/compiler.rs:25-34) plus its grid."""
+
+    def __init__(self, assets, fixed_dates, horizon, needs_fine_grid):
+        self.assets, self.fixed_dates, self.horizon, self.needs_fine_grid = assets, fixed_dates, horizon, \
+            needs_fine_grid
+
+    def time_grid(self, steps_per_year: int) -> list[float]:
+        return self.time_grid_forced(steps_per_year, False)
+
+    def time_grid_forced(self, steps_per_year: int, force_fine: bool) -> list[float]:
+        return _plan_raw(self._json, steps_per_year, force_fine)[0]
+
+
+def _plan_raw(js: str, spy: int, force: bool):
+    L = N.lib()
+    cap = 1 << 16
+    grid = (C.c_double * cap)()
+    dates = (C.c_double * cap)()
+    ng, nd = C.c_uint64(), C.c_uint64()
+    fine, hz = C.c_int32(), C.c_double()
+    assets = C.create_string_buffer(1 << 16)
+    N.check(L.kt_plan(js.encode(), int(spy), 1 if force else 0, grid, C.byref(ng), dates, C.byref(nd), cap,
+                      C.byref(fine), C.byref(hz), assets, len(assets)))
+    if ng.value > cap or nd.value > cap:
+        raise N.KontractError(7, "grid too large for this helper")
+    names = assets.value.decode()
+    return (list(grid[: ng.value]), list(dates[: nd.value]), hz.value, bool(fine.value),
+            names.split("\n") if names else [])
+
+
+def plan(contract: Contract) -> Plan:
+    """compile (src/compiler.rs:69-85), host only."""
+    js = contract.to_json()
+    _, dates, hz, fine, assets = _plan_raw(js, 0, False)
+    p = Plan(assets, dates, hz, fine)
+    p._json = js
+    return p
+
+
+compile = plan  # the reference's name
+
+
+def describe_program(contract: Contract, model, n_paths: int = 1, seed: int = 42, steps_per_year: int = 50,
+                     grid: Sequence[float] | None = None) -> str:
+    """Text listing of the op program the GPU engine runs for a contract."""
+    m = _model_struct(model)
+    cfg = _cfg(n_paths, seed, steps_per_year, model.rate, grid)
+    buf = C.create_string_buffer(1 << 20)
+    N.check(N.lib().kt_describe_program(contract.to_json().encode(), C.byref(m.s), C.byref(cfg.s), buf, len(buf)))
+    return buf.value.decode()
