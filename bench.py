@@ -61,10 +61,22 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--paths", type=int, default=4_194_304)
-    ap.add_argument("--cpu-paths", type=int, default=65_536)
+    ap.add_argument("--cpu-paths", type=int, default=262_144)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--extras", action="store_true", help="also time configs 1 and 3 (single pricings)")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run the plain copy kernel on MIB MiB (profiler byte-counter calibration)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import ctypes as C
+        from kontracts_amd import _native as N
+        eng = N.engine(0)
+        ms = C.c_double()
+        nbytes = args.copy_calibration << 20
+        N.check(N.lib().kt_debug_copy(eng.handle, nbytes, 5, C.byref(ms)), eng.handle)
+        print(json.dumps({"copy_bytes": nbytes, "reps": 5, "avg_ms": ms.value,
+                          "GBps": 2 * nbytes / (ms.value / 1e3) / 1e9}))
+        return
     if args.gpus != 1:
         print("bench.py runs on one GPU: this build has no multi-GPU path (paths would shard by index with one "
               "f64 statistics reduction per contract); use --gpus 1", file=sys.stderr)

@@ -212,6 +212,10 @@ kt_status kt_debug_philox(kt_engine* engine, const uint32_t* ctr, const uint32_t
                           uint32_t* out);
 /* FP32 inverse normal on the GPU applied to raw 32-bit words. */
 kt_status kt_debug_normals(kt_engine* engine, const uint32_t* words, uint64_t n, float* out);
+/* Streaming device-to-device copy of `bytes` (multiple of 16) through a plain
+ * copy kernel, repeated `reps` times; *ms receives the average kernel time.
+ * Calibrates the profiler's HBM byte counters against a known size. */
+kt_status kt_debug_copy(kt_engine* engine, uint64_t bytes, int32_t reps, double* ms);
 /* kt_price that also returns every path's present value (pv_out[n_paths]) and,
  * when normals_out != NULL, the FP32 normals the GPU drew, laid out
  * [path][grid point][component].  grid_out receives the grid (capacity

@@ -1039,13 +1039,71 @@ Program lower(const std::vector<const Con*>& contracts, const std::vector<double
   for (size_t i = 0; i < contracts.size(); ++i) L.add_contract(*contracts[i], (int)i);
   Program p = L.finish();
   p.n_outputs = (int)contracts.size();
+  classify_groups(p);
   return p;
+}
+
+// A point's op list is a single monitor when it is exactly
+//   spot s <- asset a ; t <- s cmp imm ; until(ko, ONE, t)      (knock-out)
+//   spot s <- asset a ; avg|rmax|rmin(acc, s)                    (accumulator)
+// The fast path does not write the spot slot: nothing else reads it at that point.
+static bool single_monitor(const Program& p, const KStep& st, KGroup& g) {
+  if (st.count == 0) {
+    g = KGroup{G_NONE, 0, 0, 0.0f};
+    return true;
+  }
+  const KOp* o = &p.ops[st.begin];
+  if (st.count == 3 && o[0].code == OP_SPOT && o[2].code == OP_UNTIL && o[2].a == kSlotOne &&
+      o[1].a == o[0].d && o[2].b == o[1].d && o[1].d != o[2].d) {
+    int kind = -1;
+    switch (o[1].code) {
+      case OP_GEI: kind = G_KO_GE; break;
+      case OP_GTI: kind = G_KO_GT; break;
+      case OP_LEI: kind = G_KO_LE; break;
+      case OP_LTI: kind = G_KO_LT; break;
+      default: break;
+    }
+    if (kind < 0) return false;
+    g = KGroup{kind, o[2].d, o[0].a, o[1].imm};
+    return true;
+  }
+  if (st.count == 2 && o[0].code == OP_SPOT && o[1].a == o[0].d &&
+      (o[1].code == OP_AVG || o[1].code == OP_RMAX || o[1].code == OP_RMIN)) {
+    int kind = o[1].code == OP_AVG ? G_AVG : o[1].code == OP_RMAX ? G_RMAX : G_RMIN;
+    g = KGroup{kind, o[1].d, o[0].a, 0.0f};
+    return true;
+  }
+  return false;
+}
+
+void classify_groups(Program& p) {
+  const int n = (int)p.steps.size();
+  p.groups.clear();
+  for (int g0 = 0; g0 < n; g0 += 4) {
+    KGroup grp{G_GENERIC, 0, 0, 0.0f};
+    if (g0 + 4 <= n) {
+      KGroup first;
+      bool ok = single_monitor(p, p.steps[g0], first);
+      for (int s = 1; ok && s < 4; ++s) {
+        const KStep& a = p.steps[g0];
+        const KStep& b = p.steps[g0 + s];
+        ok = a.count == b.count && (a.count == 0 || a.begin == b.begin);
+      }
+      if (ok) grp = first;
+    }
+    p.groups.push_back(grp);
+  }
 }
 
 std::string Program::describe() const {
   std::ostringstream os;
   os << "slots " << n_slots << ", outputs " << n_outputs << ", ops " << ops.size() << ", max/step "
      << max_ops_per_step << (time_only ? ", time-only" : ", price-dependent") << "\n";
+  int ng[9] = {0};
+  for (const KGroup& g : groups) ng[g.kind]++;
+  os << "groups: generic " << ng[G_GENERIC] << ", none " << ng[G_NONE] << ", knock-out "
+     << ng[G_KO_GE] + ng[G_KO_GT] + ng[G_KO_LE] + ng[G_KO_LT] << ", avg " << ng[G_AVG] << ", rmax/rmin "
+     << ng[G_RMAX] + ng[G_RMIN] << "\n";
   os << "init:";
   for (const KOp& o : init) os << " " << op_name(o.code) << "(" << (int)o.d << "," << o.imm << ")";
   os << "\n";

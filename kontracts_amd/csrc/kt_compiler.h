@@ -64,6 +64,7 @@ struct Program {
   std::vector<KOp> init;       // run once before grid point 0
   std::vector<KOp> ops;        // concatenated per-step lists
   std::vector<KStep> steps;    // one per grid point
+  std::vector<KGroup> groups;  // one per 4 grid points
   int n_slots = 2;             // slots used (<= kMaxSlots)
   int n_outputs = 1;           // pv accumulators used
   bool time_only = true;       // no price-dependent condition anywhere
@@ -76,6 +77,9 @@ struct Program {
 // index).  Raises Error on contracts the engine cannot run.
 Program lower(const std::vector<const Con*>& contracts, const std::vector<double>& grid,
               const std::vector<std::string>& model_assets);
+
+// Classify groups of 4 grid points (fills prog.groups).
+void classify_groups(Program& prog);
 
 // For LSM: true iff every flow of the contract happens at its acquisition
 // step (no When/Anytime/Until/Or anywhere in the tree).

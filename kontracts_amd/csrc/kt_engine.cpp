@@ -233,35 +233,87 @@ std::vector<void*> kt_engine::upload(const std::vector<std::pair<const void*, si
 
 namespace kt {
 
-// Run one program over n_paths paths; returns the per-output statistics.
-// pv_host / z_host (nullable) request the debug variant.
-std::vector<KStats> run_program(kt_engine* E, const Program& prog, const ModelDesc& md,
-                                const std::vector<double>& grid, double rate, uint64_t n_paths, uint64_t seed,
-                                double* pv_host, float* z_host) {
+// Run one program over n_paths paths under one or kScenarios scenarios (same
+// model kind, different parameters and discount rates); returns the
+// per-output statistics.  pv_host / z_host (nullable) request the debug
+// variant (one scenario only).
+std::vector<KStats> run_program_scen(kt_engine* E, const Program& prog, const std::vector<ModelDesc>& models,
+                                     const std::vector<double>& rates, const std::vector<std::vector<double>>& stat_w,
+                                     const std::vector<double>& grid, uint64_t n_paths, uint64_t seed,
+                                     double* pv_host, float* z_host) {
+  const int SC = (int)models.size();
+  const ModelDesc& md = models[0];
   int stride = 0;
-  std::vector<float> tab = model_table(md, grid, &stride);
-  std::vector<double> df(grid.size());
-  for (size_t k = 0; k < grid.size(); ++k) df[k] = std::exp(-rate * grid[k]);
+  std::vector<float> tab;
+  if (SC == 1) {
+    tab = model_table(md, grid, &stride);
+  } else {
+    if (md.kind != KT_MODEL_GBM) raise(KT_ERR_UNSUPPORTED, "scenario kernel is GBM only");
+    std::vector<std::vector<float>> per;
+    for (const ModelDesc& m : models) per.push_back(model_table(m, grid, &stride));
+    tab.resize(grid.size() * SC * 2);
+    for (size_t k = 0; k < grid.size(); ++k)
+      for (int sc = 0; sc < SC; ++sc) {
+        tab[(k * SC + sc) * 2] = per[sc][k * 2];
+        tab[(k * SC + sc) * 2 + 1] = per[sc][k * 2 + 1];
+      }
+    stride = 2 * SC;
+  }
+  // discount tables, one per distinct rate
+  std::vector<double> distinct;
+  std::vector<int> df_index(SC);
+  for (int sc = 0; sc < SC; ++sc) {
+    auto it = std::find(distinct.begin(), distinct.end(), rates[sc]);
+    if (it == distinct.end()) {
+      df_index[sc] = (int)distinct.size();
+      distinct.push_back(rates[sc]);
+    } else {
+      df_index[sc] = (int)(it - distinct.begin());
+    }
+  }
+  std::vector<double> df(distinct.size() * grid.size());
+  for (size_t t = 0; t < distinct.size(); ++t)
+    for (size_t k = 0; k < grid.size(); ++k) df[t * grid.size() + k] = std::exp(-distinct[t] * grid[k]);
+  // knock-out thresholds log2(H/s0) per group and scenario; groups whose
+  // spot or level is not positive are interpreted instead
+  std::vector<KGroup> groups = prog.groups;
+  std::vector<float> thr(std::max<size_t>(groups.size() * SC, 1), 0.0f);
+  for (size_t g = 0; g < groups.size(); ++g) {
+    KGroup& gd = groups[g];
+    if (gd.kind < G_KO_GE || gd.kind > G_KO_LT) continue;
+    bool ok = gd.imm > 0.0f && gd.asset >= 0 && gd.asset < (int)md.s0.size();
+    for (int sc = 0; ok && sc < SC; ++sc) {
+      double s0 = models[sc].s0[gd.asset];
+      ok = s0 > 0.0;
+      if (ok) thr[g * SC + sc] = (float)std::log2((double)gd.imm / s0);
+    }
+    if (!ok) gd.kind = G_GENERIC;
+  }
+
   int kind = model_kernel_kind(md);
   int K = (int)md.names.size();
   bool debug = pv_host || z_host;
   int ns = 0, nout = 0;
-  if (!path_variant_exists(kind, K, prog.n_slots, prog.n_outputs, debug, &ns, &nout))
+  if (!path_variant_exists(kind, K, prog.n_slots, prog.n_outputs, debug, SC, &ns, &nout))
     raise(KT_ERR_UNSUPPORTED, "no path kernel variant for this program shape");
   if (prog.n_slots > ns) raise(KT_ERR_UNSUPPORTED, "program needs more registers than the kernel variant holds");
+  const int nstat = SC == 1 ? nout : kScenarioStats;
+  const int n_used = SC == 1 ? prog.n_outputs : kScenarioStats;
 
   std::vector<KOp> ops = prog.ops;
   if (ops.empty()) ops.push_back(KOp{OP_NOP, 0, 0, 0, 0.0f});
   auto dev = E->upload({{ops.data(), ops.size() * sizeof(KOp)},
                         {prog.steps.data(), prog.steps.size() * sizeof(KStep)},
+                        {groups.data(), groups.size() * sizeof(KGroup)},
                         {prog.init.data(), prog.init.size() * sizeof(KOp)},
                         {df.data(), df.size() * sizeof(double)},
-                        {tab.data(), tab.size() * sizeof(float)}});
+                        {tab.data(), tab.size() * sizeof(float)},
+                        {thr.data(), thr.size() * sizeof(float)}});
 
   const uint64_t kMaxBlocksPerLaunch = 1ull << 20;
   uint64_t total_blocks = (n_paths + kBlock - 1) / kBlock;
   if (total_blocks > (1ull << 31)) raise(KT_ERR_INVALID_ARGUMENT, "too many paths");
-  KStats* partials = (KStats*)E->ensure(E->partials, std::max<uint64_t>(total_blocks, 1) * nout * sizeof(KStats));
+  KStats* partials = (KStats*)E->ensure(E->partials, std::max<uint64_t>(total_blocks, 1) * nstat * sizeof(KStats));
   KStats* stats = (KStats*)E->ensure(E->stats, 8 * sizeof(KStats));
   unsigned* err = (unsigned*)E->ensure(E->errw, 256);
   check_hip(hipMemsetAsync(err, 0, sizeof(unsigned), E->stream), "memset");
@@ -274,11 +326,14 @@ std::vector<KStats> run_program(kt_engine* E, const Program& prog, const ModelDe
   PathArgs A{};
   A.ops = (const KOp*)dev[0];
   A.steps = (const KStep*)dev[1];
-  A.init = (const KOp*)dev[2];
-  A.df = (const double*)dev[3];
-  A.tab = (const float*)dev[4];
+  A.groups = (const KGroup*)dev[2];
+  A.init = (const KOp*)dev[3];
+  A.df = (const double*)dev[4];
+  A.tab = (const float*)dev[5];
+  A.thr = (const float*)dev[6];
   A.n_init = (int)prog.init.size();
   A.n_grid = (int)grid.size();
+  A.n_groups = (int)groups.size();
   A.tab_stride = stride;
   A.heston_full_truncation = md.scheme == KT_HESTON_FULL_TRUNCATION;
   A.seed_lo = (uint32_t)seed;
@@ -288,6 +343,12 @@ std::vector<KStats> run_program(kt_engine* E, const Program& prog, const ModelDe
   A.v0 = (float)std::max(md.v0, 0.0);
   A.rho = (float)md.rho;
   A.rho_perp = (float)std::sqrt(std::max(1.0 - md.rho * md.rho, 0.0));
+  for (int sc = 0; sc < SC && sc < kScenarios; ++sc) {
+    A.s0_scen[sc] = (float)models[sc].s0[0];
+    A.df_index[sc] = df_index[sc];
+  }
+  for (size_t j = 0; j < stat_w.size() && j < (size_t)kScenarioStats; ++j)
+    for (int sc = 0; sc < SC && sc < kScenarios; ++sc) A.stat_w[j][sc] = stat_w[j][sc];
   A.err = err;
 
   E->prof_begin();
@@ -295,16 +356,16 @@ std::vector<KStats> run_program(kt_engine* E, const Program& prog, const ModelDe
     uint64_t nb = std::min(kMaxBlocksPerLaunch, total_blocks - b0);
     A.path_base = b0 * kBlock;
     A.n_paths = std::min<uint64_t>(nb * kBlock, n_paths - b0 * kBlock);
-    A.partials = partials + b0 * nout;
+    A.partials = partials + b0 * nstat;
     A.pv_out = pv_dev ? pv_dev + b0 * kBlock : nullptr;
     A.z_out = z_dev ? z_dev + b0 * kBlock * grid.size() * M : nullptr;
-    check_hip(launch_path(kind, K, ns, nout, debug, A, (int)nb, E->stream), "path kernel launch");
+    check_hip(launch_path(kind, K, ns, nout, debug, SC, A, (int)nb, E->stream), "path kernel launch");
   }
-  E->prof_end((double)n_paths * (double)(grid.size() > 0 ? grid.size() - 1 : 0));
-  check_hip(launch_finalize(partials, (int)total_blocks, nout, prog.n_outputs, stats, E->stream), "finalize launch");
-  std::vector<KStats> out(prog.n_outputs);
+  E->prof_end((double)n_paths * (double)(grid.size() > 0 ? grid.size() - 1 : 0) * SC);
+  check_hip(launch_finalize(partials, (int)total_blocks, nstat, n_used, stats, E->stream), "finalize launch");
+  std::vector<KStats> out(n_used);
   unsigned err_bits = 0;
-  check_hip(hipMemcpyAsync(out.data(), stats, prog.n_outputs * sizeof(KStats), hipMemcpyDeviceToHost, E->stream),
+  check_hip(hipMemcpyAsync(out.data(), stats, n_used * sizeof(KStats), hipMemcpyDeviceToHost, E->stream),
             "copy stats");
   check_hip(hipMemcpyAsync(&err_bits, err, sizeof(unsigned), hipMemcpyDeviceToHost, E->stream), "copy err");
   if (pv_host)
@@ -319,6 +380,12 @@ std::vector<KStats> run_program(kt_engine* E, const Program& prog, const ModelDe
     raise(code, error_text(code));
   }
   return out;
+}
+
+std::vector<KStats> run_program(kt_engine* E, const Program& prog, const ModelDesc& md,
+                                const std::vector<double>& grid, double rate, uint64_t n_paths, uint64_t seed,
+                                double* pv_host, float* z_host) {
+  return run_program_scen(E, prog, {md}, {rate}, {}, grid, n_paths, seed, pv_host, z_host);
 }
 
 // Shared front half of price_gbm: parse, plan, grid, validation.
@@ -552,6 +619,28 @@ kt_status kt_debug_philox(kt_engine* E, const uint32_t* ctr, const uint32_t* key
   check_hip(launch_debug_philox(d, key[0], key[1], n, d + 4 * n, E->stream), "launch");
   check_hip(hipMemcpyAsync(out, d + 4 * n, n * 16, hipMemcpyDeviceToHost, E->stream), "copy");
   check_hip(hipStreamSynchronize(E->stream), "sync");
+  return KT_OK;
+  KT_GUARD_END(E)
+}
+
+kt_status kt_debug_copy(kt_engine* E, uint64_t bytes, int32_t reps, double* ms) {
+  KT_GUARD_BEGIN
+  if (!E || bytes % 16 || reps < 1) raise(KT_ERR_INVALID_ARGUMENT, "bad copy arguments");
+  check_hip(hipSetDevice(E->device), "hipSetDevice");
+  char* buf = (char*)E->ensure(E->aux, 2 * bytes);
+  check_hip(hipMemsetAsync(buf, 1, bytes, E->stream), "memset");
+  hipEvent_t a, b;
+  check_hip(hipEventCreate(&a), "event");
+  check_hip(hipEventCreate(&b), "event");
+  check_hip(hipEventRecord(a, E->stream), "record");
+  for (int i = 0; i < reps; ++i) check_hip(launch_debug_copy(buf, buf + bytes, bytes, E->stream), "copy launch");
+  check_hip(hipEventRecord(b, E->stream), "record");
+  check_hip(hipEventSynchronize(b), "sync");
+  float t = 0.0f;
+  check_hip(hipEventElapsedTime(&t, a, b), "elapsed");
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  if (ms) *ms = t / reps;
   return KT_OK;
   KT_GUARD_END(E)
 }

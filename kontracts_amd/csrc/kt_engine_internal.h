@@ -70,6 +70,11 @@ std::vector<KStats> run_program(kt_engine* E, const Program& prog, const ModelDe
                                 const std::vector<double>& grid, double rate, uint64_t n_paths, uint64_t seed,
                                 double* pv_host, float* z_host);
 
+std::vector<KStats> run_program_scen(kt_engine* E, const Program& prog, const std::vector<ModelDesc>& models,
+                                     const std::vector<double>& rates, const std::vector<std::vector<double>>& stat_w,
+                                     const std::vector<double>& grid, uint64_t n_paths, uint64_t seed,
+                                     double* pv_host, float* z_host);
+
 // Implemented in kt_greeks.cpp / kt_lsm.cpp / kt_portfolio.cpp.
 void greeks_gbm_crn(kt_engine* E, const char* json, const char* asset, double s0, double sigma,
                     const kt_mc_config* cfg, const kt_bumps& h, kt_greeks* out);

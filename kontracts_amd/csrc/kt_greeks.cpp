@@ -20,14 +20,19 @@ double price_once(kt_engine* E, const Con& con, const ModelDesc& md, const std::
 
 }  // namespace
 
-// greeks_gbm: drift = rate, seven repricings at the same seed.
+// greeks_gbm: drift = rate, seven repricings at the same seed.  The seven
+// scenarios run in one launch that shares every normal (common random
+// numbers); the kernel reduces the per-path samples of price, delta, gamma,
+// vega and rho, whose means are exactly the reference's finite differences of
+// the seven prices, and whose spread gives each Greek's standard error.
 void greeks_gbm_crn(kt_engine* E, const char* json, const char* asset, double s0, double sigma,
                     const kt_mc_config* cfg, const kt_bumps& h, kt_greeks* out) {
+  if (cfg->antithetic || cfg->control_variate)
+    raise(KT_ERR_UNSUPPORTED, "variance reduction is not on the GPU engine's path");
   auto con = parse_contract(json);
   Plan plan = compile_plan(*con);
   std::vector<double> grid = cfg->grid && cfg->n_grid ? std::vector<double>(cfg->grid, cfg->grid + cfg->n_grid)
                                                       : time_grid_forced(plan, cfg->steps_per_year, false);
-  validate_grid(grid);
   auto model = [&](double s, double sg, double r) {
     ModelDesc md;
     md.kind = KT_MODEL_GBM;
@@ -38,21 +43,48 @@ void greeks_gbm_crn(kt_engine* E, const char* json, const char* asset, double s0
     return md;
   };
   const double r = cfg->rate;
+  std::vector<ModelDesc> scen = {model(s0, sigma, r),         model(s0 + h.spot, sigma, r),
+                                 model(s0 - h.spot, sigma, r), model(s0, sigma + h.vol, r),
+                                 model(s0, sigma - h.vol, r),  model(s0, sigma, r + h.rate),
+                                 model(s0, sigma, r - h.rate)};
+  std::vector<double> rates = {r, r, r, r, r, r + h.rate, r - h.rate};
+  for (const ModelDesc& m : scen) validate_model(m);
+  validate_grid(grid);
   *out = kt_greeks{};
   if (cfg->n_paths == 0) return;
+  Program prog = lower({con.get()}, grid, {asset});
+  int ns = 0, nout = 0;
+  if (path_variant_exists(MK_GBM, 1, prog.n_slots, 1, false, kScenarios, &ns, &nout)) {
+    const double hs = h.spot, hv = h.vol, hr = h.rate;
+    std::vector<std::vector<double>> w = {
+        {1, 0, 0, 0, 0, 0, 0},
+        {0, 1 / (2 * hs), -1 / (2 * hs), 0, 0, 0, 0},
+        {-2 / (hs * hs), 1 / (hs * hs), 1 / (hs * hs), 0, 0, 0, 0},
+        {0, 0, 0, 1 / (2 * hv), -1 / (2 * hv), 0, 0},
+        {0, 0, 0, 0, 0, 1 / (2 * hr), -1 / (2 * hr)}};
+    auto st = run_program_scen(E, prog, scen, rates, w, grid, cfg->n_paths, cfg->seed, nullptr, nullptr);
+    double* vals[5] = {&out->price, &out->delta, &out->gamma, &out->vega, &out->rho};
+    for (int j = 0; j < 5; ++j) {
+      kt_price_result pr = result_from_stats(st[j]);
+      *vals[j] = pr.price;
+      out->std_error[j] = pr.std_error;
+    }
+    return;
+  }
+  // Programs too large for the scenario kernel: seven launches at the same seed.
+  double p[7];
   double se0 = 0.0;
-  double p = price_once(E, *con, model(s0, sigma, r), grid, r, cfg, &se0);
-  double su = price_once(E, *con, model(s0 + h.spot, sigma, r), grid, r, cfg, nullptr);
-  double sd = price_once(E, *con, model(s0 - h.spot, sigma, r), grid, r, cfg, nullptr);
-  double vu = price_once(E, *con, model(s0, sigma + h.vol, r), grid, r, cfg, nullptr);
-  double vd = price_once(E, *con, model(s0, sigma - h.vol, r), grid, r, cfg, nullptr);
-  double ru = price_once(E, *con, model(s0, sigma, r + h.rate), grid, r + h.rate, cfg, nullptr);
-  double rd = price_once(E, *con, model(s0, sigma, r - h.rate), grid, r - h.rate, cfg, nullptr);
-  out->price = p;
-  out->delta = (su - sd) / (2.0 * h.spot);
-  out->gamma = (su - 2.0 * p + sd) / (h.spot * h.spot);
-  out->vega = (vu - vd) / (2.0 * h.vol);
-  out->rho = (ru - rd) / (2.0 * h.rate);
+  for (int i = 0; i < 7; ++i) {
+    auto st = run_program(E, prog, scen[i], grid, rates[i], cfg->n_paths, cfg->seed, nullptr, nullptr);
+    kt_price_result pr = result_from_stats(st[0]);
+    p[i] = pr.price;
+    if (i == 0) se0 = pr.std_error;
+  }
+  out->price = p[0];
+  out->delta = (p[1] - p[2]) / (2.0 * h.spot);
+  out->gamma = (p[1] - 2.0 * p[0] + p[2]) / (h.spot * h.spot);
+  out->vega = (p[3] - p[4]) / (2.0 * h.vol);
+  out->rho = (p[5] - p[6]) / (2.0 * h.rate);
   const double nan = std::numeric_limits<double>::quiet_NaN();
   out->std_error[0] = se0;
   for (int i = 1; i < 5; ++i) out->std_error[i] = nan;

@@ -1,4 +1,4 @@
-// kt_kernels.hip — gfx950 path kernels: simulate, interpret the contract's op
+// kt_kernels.hip — gfx950 path kernels: simulate, evaluate the contract's op
 // program in registers, discount, and reduce in FP64, in one pass per path.
 //
 // Work layout: one path per lane, 256 lanes (4 waves) per workgroup, paths
@@ -8,10 +8,34 @@
 // (block, path_lo, path_hi, 0): normal j = k*M + c of grid point k, component
 // c, is word j%4 of block j/4, so the stream does not depend on the launch
 // shape.  Each group of 4 grid points consumes exactly M Philox blocks.
+//
+// Dispatch is per group of 4 grid points (KGroup): groups whose points all
+// run one monitor (knock-out compare, running sum, running max/min) or none
+// execute as unrolled straight-line code; the other groups interpret each
+// point's op list.  All tables are read through the constant address space,
+// so their loads are scalar (s_load) and their values wave-uniform.
+//
+// Scenarios (SC > 1, GBM only): SC repricings of one contract with different
+// model parameters share every normal of a path (common random numbers); each
+// scenario has its own log-spot, register file and present value, and the
+// kernel reduces SC-weighted combinations of them (finite-difference Greeks).
 #include "kt_device.h"
 #include "kt_launch.h"
 
+#define KT_CONST __attribute__((address_space(4)))
+
 namespace kt {
+
+template <class T>
+__device__ __forceinline__ const KT_CONST T* cst(const T* p) {
+  return (const KT_CONST T*)(p);
+}
+
+// Scalar-typed loads through the constant address space (wave-uniform index).
+__device__ __forceinline__ uint32_t ld_u32(const void* p, size_t i) { return cst((const uint32_t*)p)[i]; }
+__device__ __forceinline__ uint64_t ld_u64(const void* p, size_t i) { return cst((const uint64_t*)p)[i]; }
+__device__ __forceinline__ float ld_f32(const float* p, size_t i) { return cst(p)[i]; }
+__device__ __forceinline__ double ld_f64(const double* p, size_t i) { return cst(p)[i]; }
 
 // Wave-uniform selection of a per-asset kernel argument.
 template <int K>
@@ -23,123 +47,207 @@ __device__ __forceinline__ float pick_asset(const float (&arr)[kMaxAssets], uint
   return v;
 }
 
-template <int NS, int NOUT, int K>
-__device__ __forceinline__ void run_ops(const PathArgs& A, int begin, int count, float (&r)[NS], double (&pv)[NOUT],
-                                        const float (&x)[K], double df, bool live) {
-  const uint2* __restrict__ raw = reinterpret_cast<const uint2*>(A.ops) + begin;
+template <int K, int SC>
+__device__ __forceinline__ float scen_s0(const PathArgs& A, int sc, uint32_t asset) {
+  if constexpr (SC > 1) return A.s0_scen[sc];
+  return pick_asset<K>(A.s0, asset);
+}
+
+#define FOR_SC _Pragma("unroll") for (int sc = 0; sc < SC; ++sc)
+
+template <int NS, int NOUT, int K, int SC>
+__device__ __forceinline__ void run_ops(const PathArgs& A, int k, int begin, int count, float (&r)[SC][NS],
+                                        double (&pv)[SC][NOUT], const float (&x)[SC][K], bool live) {
   for (int i = 0; i < count; ++i) {
-    const uint2 w = raw[i];
+    const uint64_t word = ld_u64(A.ops, (size_t)begin + i);
+    const uint2 w = make_uint2((uint32_t)word, (uint32_t)(word >> 32));
     const uint32_t code = w.x & 0xffu;
     const uint32_t d = (w.x >> 8) & (NS - 1);
     const uint32_t a = (w.x >> 16) & (NS - 1);
     const uint32_t b = (w.x >> 24) & (NS - 1);
     const float imm = __uint_as_float(w.y);
-    const float va = r[a];
-    const float vb = r[b];
     switch (code) {
-      case OP_CONST: r[d] = imm; break;
+      case OP_CONST: FOR_SC r[sc][d] = imm; break;
       case OP_SPOT: {
         const uint32_t asset = (w.x >> 16) & 0xffu;
         const uint32_t ai = asset < (uint32_t)K ? asset : 0u;
-        r[d] = pick_asset<K>(A.s0, ai) * exp2_fast(x[ai]);
+        FOR_SC r[sc][d] = scen_s0<K, SC>(A, sc, ai) * exp2_fast(x[sc][ai]);
         break;
       }
-      case OP_NEG: r[d] = -va; break;
-      case OP_ADD: r[d] = va + vb; break;
-      case OP_SUB: r[d] = va - vb; break;
-      case OP_MUL: r[d] = va * vb; break;
-      case OP_DIV: r[d] = va / vb; break;
-      case OP_MAX: r[d] = fmaxf(va, vb); break;
-      case OP_MIN: r[d] = fminf(va, vb); break;
-      case OP_ADDI: r[d] = va + imm; break;
-      case OP_SUBI: r[d] = va - imm; break;
-      case OP_RSUBI: r[d] = imm - va; break;
-      case OP_MULI: r[d] = va * imm; break;
-      case OP_DIVI: r[d] = va / imm; break;
-      case OP_RDIVI: r[d] = imm / va; break;
-      case OP_MAXI: r[d] = fmaxf(va, imm); break;
-      case OP_MINI: r[d] = fminf(va, imm); break;
-      case OP_GE: r[d] = va >= vb ? 1.0f : 0.0f; break;
-      case OP_GT: r[d] = va > vb ? 1.0f : 0.0f; break;
-      case OP_LE: r[d] = va <= vb ? 1.0f : 0.0f; break;
-      case OP_LT: r[d] = va < vb ? 1.0f : 0.0f; break;
-      case OP_GEI: r[d] = va >= imm ? 1.0f : 0.0f; break;
-      case OP_GTI: r[d] = va > imm ? 1.0f : 0.0f; break;
-      case OP_LEI: r[d] = va <= imm ? 1.0f : 0.0f; break;
-      case OP_LTI: r[d] = va < imm ? 1.0f : 0.0f; break;
-      case OP_AND: r[d] = va * vb; break;
-      case OP_OR: r[d] = fmaxf(va, vb); break;
-      case OP_NOT: r[d] = 1.0f - va; break;
+      case OP_NEG: FOR_SC r[sc][d] = -r[sc][a]; break;
+      case OP_ADD: FOR_SC r[sc][d] = r[sc][a] + r[sc][b]; break;
+      case OP_SUB: FOR_SC r[sc][d] = r[sc][a] - r[sc][b]; break;
+      case OP_MUL: FOR_SC r[sc][d] = r[sc][a] * r[sc][b]; break;
+      case OP_DIV: FOR_SC r[sc][d] = r[sc][a] / r[sc][b]; break;
+      case OP_MAX: FOR_SC r[sc][d] = fmaxf(r[sc][a], r[sc][b]); break;
+      case OP_MIN: FOR_SC r[sc][d] = fminf(r[sc][a], r[sc][b]); break;
+      case OP_ADDI: FOR_SC r[sc][d] = r[sc][a] + imm; break;
+      case OP_SUBI: FOR_SC r[sc][d] = r[sc][a] - imm; break;
+      case OP_RSUBI: FOR_SC r[sc][d] = imm - r[sc][a]; break;
+      case OP_MULI: FOR_SC r[sc][d] = r[sc][a] * imm; break;
+      case OP_DIVI: FOR_SC r[sc][d] = r[sc][a] / imm; break;
+      case OP_RDIVI: FOR_SC r[sc][d] = imm / r[sc][a]; break;
+      case OP_MAXI: FOR_SC r[sc][d] = fmaxf(r[sc][a], imm); break;
+      case OP_MINI: FOR_SC r[sc][d] = fminf(r[sc][a], imm); break;
+      case OP_GE: FOR_SC r[sc][d] = r[sc][a] >= r[sc][b] ? 1.0f : 0.0f; break;
+      case OP_GT: FOR_SC r[sc][d] = r[sc][a] > r[sc][b] ? 1.0f : 0.0f; break;
+      case OP_LE: FOR_SC r[sc][d] = r[sc][a] <= r[sc][b] ? 1.0f : 0.0f; break;
+      case OP_LT: FOR_SC r[sc][d] = r[sc][a] < r[sc][b] ? 1.0f : 0.0f; break;
+      case OP_GEI: FOR_SC r[sc][d] = r[sc][a] >= imm ? 1.0f : 0.0f; break;
+      case OP_GTI: FOR_SC r[sc][d] = r[sc][a] > imm ? 1.0f : 0.0f; break;
+      case OP_LEI: FOR_SC r[sc][d] = r[sc][a] <= imm ? 1.0f : 0.0f; break;
+      case OP_LTI: FOR_SC r[sc][d] = r[sc][a] < imm ? 1.0f : 0.0f; break;
+      case OP_AND: FOR_SC r[sc][d] = r[sc][a] * r[sc][b]; break;
+      case OP_OR: FOR_SC r[sc][d] = fmaxf(r[sc][a], r[sc][b]); break;
+      case OP_NOT: FOR_SC r[sc][d] = 1.0f - r[sc][a]; break;
       case OP_WHEN: {
         const uint32_t dn = (d + 1) & (NS - 1);
-        const float n = va * (1.0f - r[d]) * vb;
-        r[d] = r[d] + n;
-        r[dn] = n;
+        FOR_SC {
+          const float n = r[sc][a] * (1.0f - r[sc][d]) * r[sc][b];
+          r[sc][d] = r[sc][d] + n;
+          r[sc][dn] = n;
+        }
         break;
       }
-      case OP_UNTIL: r[d] = fmaxf(r[d], va * vb); break;
+      case OP_UNTIL: FOR_SC r[sc][d] = fmaxf(r[sc][d], r[sc][a] * r[sc][b]); break;
       case OP_EMIT: {
         const uint32_t o = ((w.x >> 8) & 0xffu) & (NOUT - 1);
-        pv[o] += (double)(imm * (va * vb)) * df;
+        FOR_SC {
+          const double df = ld_f64(A.df, (size_t)A.df_index[sc] * A.n_grid + k);
+          pv[sc][o] += (double)(imm * (r[sc][a] * r[sc][b])) * df;
+        }
         break;
       }
-      case OP_RMAX: r[d] = fmaxf(r[d], va); break;
-      case OP_RMIN: r[d] = fminf(r[d], va); break;
+      case OP_RMAX: FOR_SC r[sc][d] = fmaxf(r[sc][d], r[sc][a]); break;
+      case OP_RMIN: FOR_SC r[sc][d] = fminf(r[sc][d], r[sc][a]); break;
       case OP_AVG: {
         const uint32_t dc = (d + 1) & (NS - 1);
-        const float y = va - r[dc];
-        const float t = r[d] + y;
-        r[dc] = (t - r[d]) - y;
-        r[d] = t;
+        FOR_SC {
+          const float y = r[sc][a] - r[sc][dc];
+          const float t = r[sc][d] + y;
+          r[sc][dc] = (t - r[sc][d]) - y;
+          r[sc][d] = t;
+        }
         break;
       }
-      case OP_FLAGERR:
-        if (live && va != 0.0f) atomicOr(A.err, 1u << ((uint32_t)imm & 31u));
+      case OP_FLAGERR: {
+        bool hit = false;
+        FOR_SC hit = hit || r[sc][a] != 0.0f;
+        if (live && hit) atomicOr(A.err, 1u << ((uint32_t)imm & 31u));
         break;
+      }
       default: break;
     }
   }
 }
 
-// Advance one path to grid point k with that point's normals zk[0..M), then
-// run the point's op list.
-template <int MODEL, int K, int NS, int NOUT, bool DEBUG>
-__device__ __forceinline__ void step_body(const PathArgs& A, int k, const float* zk, float (&x)[K], float& v,
-                                          float (&r)[NS], double (&pv)[NOUT], bool live, uint64_t local) {
-  constexpr int M = MODEL == MK_GBM ? 1 : (MODEL == MK_HESTON ? 2 : K);
+// Advance every scenario of one path to grid point k with that point's
+// normals zk[0..M).
+template <int MODEL, int K, int SC>
+__device__ __forceinline__ void advance(const PathArgs& A, int k, const float* zk, float (&x)[SC][K], float& v) {
   if constexpr (MODEL == MK_GBM) {
-    const float2 ab = reinterpret_cast<const float2*>(A.tab)[k];
-    x[0] = fmaf(ab.y, zk[0], x[0] + ab.x);
+    FOR_SC {
+      const float a = ld_f32(A.tab, ((size_t)k * SC + sc) * 2), b = ld_f32(A.tab, ((size_t)k * SC + sc) * 2 + 1);
+      x[sc][0] = fmaf(b, zk[0], x[sc][0] + a);
+    }
   } else if constexpr (MODEL == MK_HESTON) {
-    const float* t = A.tab + kHestonStride * k;
+    float t[6];
+#pragma unroll
+    for (int i = 0; i < 6; ++i) t[i] = ld_f32(A.tab, (size_t)kHestonStride * k + i);
     const float z1 = zk[0], z2 = zk[1];
     const float vp = A.heston_full_truncation ? fmaxf(v, 0.0f) : v;
     const float sv = __builtin_amdgcn_sqrtf(vp);
-    x[0] = x[0] + fmaf(t[1], vp, t[0]) + t[2] * sv * z1;
+    x[0][0] = x[0][0] + fmaf(t[1], vp, t[0]) + t[2] * sv * z1;
     const float wv = fmaf(A.rho, z1, A.rho_perp * z2);
     const float vn = fmaf(t[3], vp, v) + fmaf(t[5] * sv, wv, t[4]);
     v = A.heston_full_truncation ? vn : fmaxf(vn, 0.0f);
   } else {
-    const float2* ab = reinterpret_cast<const float2*>(A.tab) + (size_t)k * K;
 #pragma unroll
     for (int i = 0; i < K; ++i) {
       float wi = 0.0f;
 #pragma unroll
       for (int c = 0; c <= i; ++c) wi = fmaf(A.chol[i * K + c], zk[c], wi);
-      x[i] = fmaf(ab[i].y, wi, x[i] + ab[i].x);
+      const float a = ld_f32(A.tab, ((size_t)k * K + i) * 2), b = ld_f32(A.tab, ((size_t)k * K + i) * 2 + 1);
+      x[0][i] = fmaf(b, wi, x[0][i] + a);
     }
   }
+}
+
+template <int MODEL, int K, int SC, bool DEBUG>
+__device__ __forceinline__ void store_normals(const PathArgs& A, int k, const float* zk, bool live, uint64_t local) {
+  constexpr int M = MODEL == MK_GBM ? 1 : (MODEL == MK_HESTON ? 2 : K);
   if constexpr (DEBUG) {
     if (live && A.z_out) {
 #pragma unroll
       for (int c = 0; c < M; ++c) A.z_out[((size_t)local * A.n_grid + k) * M + c] = zk[c];
     }
   }
-  const KStep st = A.steps[k];
-  if (st.count > 0) run_ops<NS, NOUT, K>(A, st.begin, st.count, r, pv, x, A.df[k], live);
 }
 
-template <int MODEL, int K, int NS, int NOUT, bool DEBUG>
+// One interpreted grid point.
+template <int MODEL, int K, int NS, int NOUT, int SC, bool DEBUG>
+__device__ __forceinline__ void generic_point(const PathArgs& A, int k, const float* zk, float (&x)[SC][K],
+                                              float& v, float (&r)[SC][NS], double (&pv)[SC][NOUT], bool live,
+                                              uint64_t local) {
+  advance<MODEL, K, SC>(A, k, zk, x, v);
+  store_normals<MODEL, K, SC, DEBUG>(A, k, zk, live, local);
+  const uint64_t st = ld_u64(A.steps, (size_t)k);
+  const int begin = (int)(uint32_t)st, count = (int)(uint32_t)(st >> 32);
+  if (count > 0) run_ops<NS, NOUT, K, SC>(A, k, begin, count, r, pv, x, live);
+}
+
+// One group of 4 grid points under a single monitor, as straight-line code.
+template <int KIND, int MODEL, int K, int NS, int SC, bool DEBUG>
+__device__ __forceinline__ void fast_group(const PathArgs& A, const KGroup gd, int g, int k0, const float* z,
+                                           float (&x)[SC][K], float& v, float (&r)[SC][NS], bool live,
+                                           uint64_t local) {
+  constexpr int M = MODEL == MK_GBM ? 1 : (MODEL == MK_HESTON ? 2 : K);
+  constexpr bool KO = KIND >= G_KO_GE && KIND <= G_KO_LT;
+  const uint32_t slot = (uint32_t)gd.slot & (NS - 1);
+  const uint32_t slot1 = (slot + 1) & (NS - 1);
+  const uint32_t asset = (uint32_t)gd.asset < (uint32_t)K ? (uint32_t)gd.asset : 0u;
+  float acc[SC], comp[SC], thr[SC], s0v[SC];
+  FOR_SC {
+    acc[sc] = KIND == G_NONE ? 0.0f : r[sc][slot];
+    comp[sc] = KIND == G_AVG ? r[sc][slot1] : 0.0f;
+    // knock-out in log space: S cmp H  <=>  log2(S/s0) cmp log2(H/s0)  (s0 > 0, H > 0)
+    thr[sc] = KO ? ld_f32(A.thr, (size_t)g * SC + sc) : 0.0f;
+    s0v[sc] = scen_s0<K, SC>(A, sc, asset);
+  }
+#pragma unroll
+  for (int s = 0; s < 4; ++s) {
+    advance<MODEL, K, SC>(A, k0 + s, &z[s * M], x, v);
+    store_normals<MODEL, K, SC, DEBUG>(A, k0 + s, &z[s * M], live, local);
+    if constexpr (KIND != G_NONE) {
+      FOR_SC {
+        const float xs = K == 1 ? x[sc][0] : x[sc][asset];
+        if constexpr (KIND == G_KO_GE) acc[sc] = xs >= thr[sc] ? 1.0f : acc[sc];
+        if constexpr (KIND == G_KO_GT) acc[sc] = xs > thr[sc] ? 1.0f : acc[sc];
+        if constexpr (KIND == G_KO_LE) acc[sc] = xs <= thr[sc] ? 1.0f : acc[sc];
+        if constexpr (KIND == G_KO_LT) acc[sc] = xs < thr[sc] ? 1.0f : acc[sc];
+        if constexpr (KIND == G_AVG || KIND == G_RMAX || KIND == G_RMIN) {
+          const float S = s0v[sc] * exp2_fast(xs);
+          if constexpr (KIND == G_AVG) {
+            const float y = S - comp[sc];
+            const float t = acc[sc] + y;
+            comp[sc] = (t - acc[sc]) - y;
+            acc[sc] = t;
+          }
+          if constexpr (KIND == G_RMAX) acc[sc] = fmaxf(acc[sc], S);
+          if constexpr (KIND == G_RMIN) acc[sc] = fminf(acc[sc], S);
+        }
+      }
+    }
+  }
+  if constexpr (KIND != G_NONE) {
+    FOR_SC {
+      r[sc][slot] = acc[sc];
+      if constexpr (KIND == G_AVG) r[sc][slot1] = comp[sc];
+    }
+  }
+}
+
+template <int MODEL, int K, int NS, int NOUT, int SC, int NSTAT, bool DEBUG>
 __global__ __launch_bounds__(kBlock) void path_kernel(const PathArgs A) {
   constexpr int M = MODEL == MK_GBM ? 1 : (MODEL == MK_HESTON ? 2 : K);
   const uint64_t local = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
@@ -147,23 +255,29 @@ __global__ __launch_bounds__(kBlock) void path_kernel(const PathArgs A) {
   const uint64_t path = A.path_base + local;
   const uint32_t p_lo = (uint32_t)path, p_hi = (uint32_t)(path >> 32);
 
-  float r[NS];
+  float r[SC][NS];
+  double pv[SC][NOUT];
+  float x[SC][K];
+  FOR_SC {
 #pragma unroll
-  for (int i = 0; i < NS; ++i) r[i] = 0.0f;
-  double pv[NOUT];
+    for (int i = 0; i < NS; ++i) r[sc][i] = 0.0f;
 #pragma unroll
-  for (int j = 0; j < NOUT; ++j) pv[j] = 0.0;
-  for (int i = 0; i < A.n_init; ++i) {
-    const uint2 w = reinterpret_cast<const uint2*>(A.init)[i];
-    r[(w.x >> 8) & (NS - 1)] = __uint_as_float(w.y);
+    for (int j = 0; j < NOUT; ++j) pv[sc][j] = 0.0;
+#pragma unroll
+    for (int i = 0; i < K; ++i) x[sc][i] = 0.0f;
   }
-
-  float x[K];
-#pragma unroll
-  for (int i = 0; i < K; ++i) x[i] = 0.0f;
+  for (int i = 0; i < A.n_init; ++i) {
+    const uint64_t w = ld_u64(A.init, (size_t)i);
+    FOR_SC r[sc][((uint32_t)w >> 8) & (NS - 1)] = __uint_as_float((uint32_t)(w >> 32));
+  }
   float v = A.v0;
 
-  for (int g = 0; 4 * g < A.n_grid; ++g) {
+  for (int g = 0; g < A.n_groups; ++g) {
+    KGroup gd;
+    gd.kind = (int32_t)ld_u32(A.groups, (size_t)g * 4);
+    gd.slot = (int32_t)ld_u32(A.groups, (size_t)g * 4 + 1);
+    gd.asset = (int32_t)ld_u32(A.groups, (size_t)g * 4 + 2);
+    gd.imm = __uint_as_float(ld_u32(A.groups, (size_t)g * 4 + 3));
     float z[4 * M];
 #pragma unroll
     for (int bk = 0; bk < M; ++bk) {
@@ -173,41 +287,67 @@ __global__ __launch_bounds__(kBlock) void path_kernel(const PathArgs A) {
       z[4 * bk + 2] = normal_from_word(q.z);
       z[4 * bk + 3] = normal_from_word(q.w);
     }
-    const int kend = min(4, A.n_grid - 4 * g);
-    if constexpr (MODEL == MK_BASKET) {
-      // 4K normals exceed what indexed register access covers: unroll the
-      // four grid points so every normal index is a compile-time constant.
+    const int k0 = 4 * g;
+    switch (gd.kind) {
+      case G_NONE: fast_group<G_NONE, MODEL, K, NS, SC, DEBUG>(A, gd, g, k0, z, x, v, r, live, local); break;
+      case G_KO_GE: fast_group<G_KO_GE, MODEL, K, NS, SC, DEBUG>(A, gd, g, k0, z, x, v, r, live, local); break;
+      case G_KO_GT: fast_group<G_KO_GT, MODEL, K, NS, SC, DEBUG>(A, gd, g, k0, z, x, v, r, live, local); break;
+      case G_KO_LE: fast_group<G_KO_LE, MODEL, K, NS, SC, DEBUG>(A, gd, g, k0, z, x, v, r, live, local); break;
+      case G_KO_LT: fast_group<G_KO_LT, MODEL, K, NS, SC, DEBUG>(A, gd, g, k0, z, x, v, r, live, local); break;
+      case G_AVG: fast_group<G_AVG, MODEL, K, NS, SC, DEBUG>(A, gd, g, k0, z, x, v, r, live, local); break;
+      case G_RMAX: fast_group<G_RMAX, MODEL, K, NS, SC, DEBUG>(A, gd, g, k0, z, x, v, r, live, local); break;
+      case G_RMIN: fast_group<G_RMIN, MODEL, K, NS, SC, DEBUG>(A, gd, g, k0, z, x, v, r, live, local); break;
+      default: {
+        const int kend = min(4, A.n_grid - k0);
+        if constexpr (MODEL == MK_BASKET) {
+          // 4K normals exceed what indexed register access covers: unroll the
+          // four points so every normal index is a compile-time constant.
 #pragma unroll
-      for (int s = 0; s < 4; ++s)
-        if (s < kend) step_body<MODEL, K, NS, NOUT, DEBUG>(A, 4 * g + s, &z[s * M], x, v, r, pv, live, local);
-    } else {
-      for (int s = 0; s < kend; ++s) {
-        float zs[M];
+          for (int s = 0; s < 4; ++s)
+            if (s < kend)
+              generic_point<MODEL, K, NS, NOUT, SC, DEBUG>(A, k0 + s, &z[s * M], x, v, r, pv, live, local);
+        } else {
+          for (int s = 0; s < kend; ++s) {
+            float zs[M];
 #pragma unroll
-        for (int c = 0; c < M; ++c) zs[c] = z[s * M + c];
-        step_body<MODEL, K, NS, NOUT, DEBUG>(A, 4 * g + s, zs, x, v, r, pv, live, local);
+            for (int c = 0; c < M; ++c) zs[c] = z[s * M + c];
+            generic_point<MODEL, K, NS, NOUT, SC, DEBUG>(A, k0 + s, zs, x, v, r, pv, live, local);
+          }
+        }
       }
     }
   }
 
   if constexpr (DEBUG) {
-    if (live && A.pv_out) A.pv_out[local] = pv[0];
+    if (live && A.pv_out) A.pv_out[local] = pv[0][0];
   }
 
+  // Outputs: the pv accumulators (one scenario), or SC-weighted combinations.
+  double y[NSTAT];
+#pragma unroll
+  for (int j = 0; j < NSTAT; ++j) {
+    if constexpr (SC == 1) {
+      y[j] = pv[0][j];
+    } else {
+      double t = 0.0;
+      FOR_SC t += A.stat_w[j][sc] * pv[sc][0];
+      y[j] = t;
+    }
+  }
   // Block statistics per output: wave shifted sums, then Chan across waves.
-  __shared__ KStats sh[NOUT][kBlock / 64];
+  __shared__ KStats sh[NSTAT][kBlock / 64];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
-  for (int j = 0; j < NOUT; ++j) {
-    const KStats st = wave_stats(pv[j], live);
+  for (int j = 0; j < NSTAT; ++j) {
+    const KStats st = wave_stats(y[j], live);
     if (lane == 0) sh[j][wave] = st;
   }
   __syncthreads();
-  if (threadIdx.x < NOUT) {
+  if (threadIdx.x < NSTAT) {
     KStats acc = sh[threadIdx.x][0];
 #pragma unroll
     for (int wv = 1; wv < kBlock / 64; ++wv) acc = stats_combine(acc, sh[threadIdx.x][wv]);
-    A.partials[(size_t)blockIdx.x * NOUT + threadIdx.x] = acc;
+    A.partials[(size_t)blockIdx.x * NSTAT + threadIdx.x] = acc;
   }
 }
 
@@ -238,6 +378,14 @@ __global__ void debug_philox_kernel(const uint32_t* __restrict__ ctr, uint32_t k
   out[4 * i + 3] = q.w;
 }
 
+// Plain streaming copy (16 B per lane per iteration): the known-size reference
+// used to calibrate the HBM byte counters.
+__global__ __launch_bounds__(256) void debug_copy_kernel(const uint4* __restrict__ src, uint4* __restrict__ dst,
+                                                         uint64_t n16) {
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n16; i += (uint64_t)gridDim.x * 256)
+    dst[i] = src[i];
+}
+
 __global__ void debug_normals_kernel(const uint32_t* __restrict__ w, uint64_t n, float* __restrict__ out) {
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) out[i] = normal_from_word(w[i]);
@@ -248,30 +396,36 @@ __global__ void debug_normals_kernel(const uint32_t* __restrict__ w, uint64_t n,
 // ---------------------------------------------------------------------------
 namespace {
 
-template <int MODEL, int K, int NS, int NOUT, bool DEBUG>
+template <int MODEL, int K, int NS, int NOUT, int SC, int NSTAT, bool DEBUG>
 hipError_t go(const PathArgs& a, int n_blocks, hipStream_t s) {
-  hipLaunchKernelGGL((path_kernel<MODEL, K, NS, NOUT, DEBUG>), dim3(n_blocks), dim3(kBlock), 0, s, a);
+  hipLaunchKernelGGL((path_kernel<MODEL, K, NS, NOUT, SC, NSTAT, DEBUG>), dim3(n_blocks), dim3(kBlock), 0, s, a);
   return hipGetLastError();
 }
 
 template <int MODEL, int K>
 hipError_t go_ns(int ns, int nout, bool debug, const PathArgs& a, int n_blocks, hipStream_t s) {
-  if (debug) return go<MODEL, K, 32, 1, true>(a, n_blocks, s);
+  if (debug) return go<MODEL, K, 32, 1, 1, 1, true>(a, n_blocks, s);
   if constexpr (MODEL != MK_BASKET) {
-    if (ns == 8 && nout == 1) return go<MODEL, K, 8, 1, false>(a, n_blocks, s);
-    if (ns == 16 && nout == 1) return go<MODEL, K, 16, 1, false>(a, n_blocks, s);
-    if (ns == 16 && nout == 8) return go<MODEL, K, 16, 8, false>(a, n_blocks, s);
+    if (ns == 16 && nout == 1) return go<MODEL, K, 16, 1, 1, 1, false>(a, n_blocks, s);
+    if (ns == 16 && nout == 8) return go<MODEL, K, 16, 8, 1, 8, false>(a, n_blocks, s);
   }
-  if (ns == 32 && nout == 1) return go<MODEL, K, 32, 1, false>(a, n_blocks, s);
-  if (ns == 32 && nout == 8) return go<MODEL, K, 32, 8, false>(a, n_blocks, s);
+  if (ns == 32 && nout == 1) return go<MODEL, K, 32, 1, 1, 1, false>(a, n_blocks, s);
+  if (ns == 32 && nout == 8) return go<MODEL, K, 32, 8, 1, 8, false>(a, n_blocks, s);
   return hipErrorInvalidValue;
 }
 
 }  // namespace
 
-bool path_variant_exists(int model, int k_assets, int n_slots, int n_outputs, bool debug, int* ns_chosen,
-                         int* nout_chosen) {
+bool path_variant_exists(int model, int k_assets, int n_slots, int n_outputs, bool debug, int n_scen,
+                         int* ns_chosen, int* nout_chosen) {
   if (model == MK_BASKET ? (k_assets < 2 || k_assets > kMaxAssets) : k_assets != 1) return false;
+  if (n_scen != 1) {
+    // CRN scenario kernel: one GBM contract, kScenarios scenarios, 16 registers.
+    if (model != MK_GBM || n_scen != kScenarios || n_outputs != 1 || n_slots > 16 || debug) return false;
+    *ns_chosen = 16;
+    *nout_chosen = 1;
+    return true;
+  }
   if (n_outputs < 1 || n_outputs > 8 || n_slots > 32) return false;
   int nout = n_outputs == 1 ? 1 : 8;
   int ns;
@@ -281,15 +435,20 @@ bool path_variant_exists(int model, int k_assets, int n_slots, int n_outputs, bo
   } else if (model == MK_BASKET) {
     ns = 32;
   } else {
-    ns = n_slots <= 8 && nout == 1 ? 8 : (n_slots <= 16 ? 16 : 32);
+    ns = n_slots <= 16 ? 16 : 32;
   }
   *ns_chosen = ns;
   *nout_chosen = nout;
   return true;
 }
 
-hipError_t launch_path(int model, int k, int ns, int nout, bool debug, const PathArgs& a, int n_blocks,
+hipError_t launch_path(int model, int k, int ns, int nout, bool debug, int n_scen, const PathArgs& a, int n_blocks,
                        hipStream_t s) {
+  if (n_scen != 1) {
+    if (model == MK_GBM && n_scen == kScenarios && ns == 16 && nout == 1 && !debug)
+      return go<MK_GBM, 1, 16, 1, kScenarios, kScenarioStats, false>(a, n_blocks, s);
+    return hipErrorInvalidValue;
+  }
   switch (model) {
     case MK_GBM: return go_ns<MK_GBM, 1>(ns, nout, debug, a, n_blocks, s);
     case MK_HESTON: return go_ns<MK_HESTON, 1>(ns, nout, debug, a, n_blocks, s);
@@ -317,6 +476,13 @@ hipError_t launch_debug_philox(const uint32_t* ctr, uint32_t k0, uint32_t k1, ui
                                hipStream_t s) {
   if (n == 0) return hipSuccess;
   hipLaunchKernelGGL(debug_philox_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, ctr, k0, k1, n, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_debug_copy(const void* src, void* dst, uint64_t bytes, hipStream_t s) {
+  const uint64_t n16 = bytes / 16;
+  if (n16 == 0) return hipSuccess;
+  hipLaunchKernelGGL(debug_copy_kernel, dim3(8192), dim3(256), 0, s, (const uint4*)src, (uint4*)dst, n16);
   return hipGetLastError();
 }
 

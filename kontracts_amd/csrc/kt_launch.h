@@ -20,14 +20,22 @@ enum ModelKind { MK_GBM = 0, MK_HESTON = 1, MK_BASKET = 2 };
 //     e0 = -kappa dt, d1 = kappa theta dt, d2 = sigma_v sqrt(dt)
 constexpr int kHestonStride = 8;
 
+// CRN scenario kernel (greeks_gbm): base, S+h, S-h, sigma+h, sigma-h, r+h, r-h,
+// reduced into price, delta, gamma, vega and rho samples.
+constexpr int kScenarios = 7;
+constexpr int kScenarioStats = 5;
+
 struct PathArgs {
   const KOp* ops;
   const KStep* steps;
+  const KGroup* groups;  // [n_groups]
   const KOp* init;
-  const double* df;   // [n_grid] discount factors exp(-r t_k)
+  const double* df;   // [n_df][n_grid] discount factors exp(-r t_k); scenario sc uses table df_index[sc]
   const float* tab;   // [n_grid][tab_stride]
+  const float* thr;   // [n_groups][SC] knock-out thresholds log2(H / s0) (knock-out groups only)
   int n_init;
   int n_grid;
+  int n_groups;
   int tab_stride;
   int heston_full_truncation;
   uint64_t n_paths;    // paths in this launch
@@ -36,6 +44,9 @@ struct PathArgs {
   float s0[kMaxAssets];
   float chol[kMaxAssets * kMaxAssets];  // lower-triangular, row-major K x K
   float v0, rho, rho_perp;
+  float s0_scen[kScenarios];      // scenario kernel: spot per scenario
+  int df_index[kScenarios];       // discount table of each scenario (0 when SC == 1)
+  double stat_w[kScenarioStats][kScenarios];  // scenario kernel: output j = sum_sc w[j][sc] pv_sc
   KStats* partials;  // [n_blocks][NOUT]
   unsigned* err;     // bit e set <=> error code e raised on a live path
   double* pv_out;    // debug: per-path pv of output 0 (nullable)
@@ -45,14 +56,15 @@ struct PathArgs {
 constexpr int kBlock = 256;
 
 // Path kernel variants available: returns false if no variant matches.
-bool path_variant_exists(int model, int k_assets, int n_slots, int n_outputs, bool debug, int* ns_chosen,
-                         int* nout_chosen);
-hipError_t launch_path(int model, int k_assets, int ns, int nout, bool debug, const PathArgs& a, int n_blocks,
-                       hipStream_t s);
+bool path_variant_exists(int model, int k_assets, int n_slots, int n_outputs, bool debug, int n_scen,
+                         int* ns_chosen, int* nout_chosen);
+hipError_t launch_path(int model, int k_assets, int ns, int nout, bool debug, int n_scen, const PathArgs& a,
+                       int n_blocks, hipStream_t s);
 hipError_t launch_finalize(const KStats* partials, int n_blocks, int nout, int n_used, KStats* out,
                            hipStream_t s);
 hipError_t launch_debug_philox(const uint32_t* ctr, uint32_t k0, uint32_t k1, uint64_t n, uint32_t* out,
                                hipStream_t s);
+hipError_t launch_debug_copy(const void* src, void* dst, uint64_t bytes, hipStream_t s);
 hipError_t launch_debug_normals(const uint32_t* words, uint64_t n, float* out, hipStream_t s);
 
 }  // namespace kt

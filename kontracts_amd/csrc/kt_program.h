@@ -66,6 +66,28 @@ struct KStep {
   int32_t count;
 };
 
+// Groups of 4 grid points.  When all four points of a group run the same
+// single monitor (or nothing), the kernel executes the group as straight-line
+// code with no interpretation; every other group is interpreted point by point.
+enum GroupKind : int32_t {
+  G_GENERIC = 0,  // interpret each point's op list
+  G_NONE = 1,     // no ops at any of the four points
+  G_KO_GE = 2,    // r[slot] = max(r[slot], S_asset >= imm) at each point (until/knock-out)
+  G_KO_GT = 3,
+  G_KO_LE = 4,
+  G_KO_LT = 5,
+  G_AVG = 6,      // Kahan sum of S_asset into r[slot], r[slot+1]
+  G_RMAX = 7,     // r[slot] = max(r[slot], S_asset)
+  G_RMIN = 8,
+};
+
+struct KGroup {
+  int32_t kind;
+  int32_t slot;
+  int32_t asset;
+  float imm;
+};
+
 // Result statistics of one output over a set of paths: count, mean and the
 // sum of squared deviations (combined with Chan et al.'s pairwise formula).
 struct KStats {

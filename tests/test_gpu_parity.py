@@ -101,7 +101,11 @@ def oracle_model(model):
 
 GBM_X = K.GBM(100, 0.2, 0.05, asset="X")
 HESTON_X = K.heston(100, 0.04, 2.0, 0.04, 0.3, -0.5, 0.05, asset="X")
-HESTON_FT = K.heston(100, 0.04, 2.0, 0.04, 0.9, -0.7, 0.05, asset="X", scheme="full_truncation")
+HESTON_FT = K.heston(100, 0.04, 2.0, 0.04, 0.3, -0.7, 0.05, asset="X", scheme="full_truncation")
+# Feller condition strongly violated (2 kappa theta = 0.16 << sigma_v^2 = 0.81): v spends
+# time near 0 where sqrt's slope amplifies FP32 rounding along the path, so a small
+# fraction of paths drift apart from the f64 restatement.
+HESTON_FT_ROUGH = K.heston(100, 0.04, 2.0, 0.04, 0.9, -0.7, 0.05, asset="X", scheme="full_truncation")
 FACTORS = [K.GbmFactor(f"A{i}", 100.0, 0.03, 0.15 + 0.05 * i) for i in range(5)]
 BASKET5 = K.correlated_gbm(FACTORS, [[1.0 if i == j else 0.3 for j in range(5)] for i in range(5)], 0.03)
 BASKET_PAYOFF = K.scale((sum((K.spot(f"A{i}") for i in range(1, 5)), K.spot("A0")) / 5.0 - 100.0).clip(0.0),
@@ -113,6 +117,7 @@ PARITY_CASES = {
                        0.01),
     "heston_european_365": (K.european_call("X", 100, 1.0), HESTON_X, 365, None, 100.0, 0),
     "heston_full_truncation": (K.european_call("X", 100, 1.0), HESTON_FT, 100, None, 100.0, 0),
+    "heston_full_truncation_rough": (K.european_call("X", 100, 1.0), HESTON_FT_ROUGH, 100, None, 100.0, 0.02),
     "asian": (K.scale((K.average(K.spot("X")) - 100.0).clip(0.0), K.one("USD")) @ K.at(1.0), GBM_X, 52, None,
               100.0, 0),
     "lookback": (K.scale((K.running_max(K.spot("X")) - 95.0).clip(0.0), K.one("USD")) @ K.at(1.0), GBM_X, 100,
@@ -218,12 +223,14 @@ def test_deterministic_known_answers_on_gpu():
 
 def test_path_dependent_errors_on_gpu():
     m = K.GBM(100, 0.2, 0.05, asset="X")
+    horizon = K.one("USD") @ K.at(1.0)  # gives the contract a fine grid up to t=1
     with pytest.raises(K.KontractError) as e:
-        K.when(K.spot("X") >= 101.0, K.one("USD").or_(K.one("USD"))).price(m, n_paths=1000, steps_per_year=12)
+        (horizon + K.when(K.spot("X") >= 101.0, K.one("USD").or_(K.one("USD")))).price(m, n_paths=1000,
+                                                                                      steps_per_year=12)
     assert e.value.kind == "Unsupported"
     # `or` under a barrier that no path reaches is never evaluated
-    never = K.when(K.spot("X") >= 1e9, K.one("USD").or_(K.one("USD")))
-    assert never.price(m, n_paths=1000, steps_per_year=12).price == 0.0
+    never = horizon + K.when(K.spot("X") >= 1e9, K.one("USD").or_(K.one("USD")))
+    assert never.price(m, n_paths=1000, steps_per_year=12).price == pytest.approx(math.exp(-0.05), abs=1e-12)
     with pytest.raises(K.KontractError) as e:
         K.european_call("Y", 100, 1.0).price(m, n_paths=100)
     assert e.value.kind == "UnknownAsset"
@@ -262,3 +269,51 @@ def test_basket_perfect_correlation_is_single_asset():
     res = c.price(m, n_paths=100_000, seed=3)
     bs = O.bs_call(100, 100, 1, 0.03, 0.2)
     assert abs(res.price - bs) / bs < 0.02
+
+
+# ---------------------------------------------------------------------------
+# Greeks (src/greeks.rs): fused CRN scenario kernel
+# ---------------------------------------------------------------------------
+def test_greeks_scenario_kernel_equals_separate_pricings():
+    c = K.up_and_out_call("X", 100, 200, 1.0)
+    s0, sg, r, n, spy = 180.0, 0.25, 0.05, 65536, 252
+    hs, hv, hr = 1.8, 0.01, 1e-4
+    g = c.greeks(K.GBM(s0, sg, r, asset="X"), n_paths=n, seed=7, steps_per_year=spy, bumps=(hs, hv, hr))
+
+    def p(s, v, rr):
+        return c.price(K.GBM(s, v, rr, asset="X"), n_paths=n, seed=7, steps_per_year=spy).price
+
+    base = p(s0, sg, r)
+    su, sd = p(s0 + hs, sg, r), p(s0 - hs, sg, r)
+    assert g.price == pytest.approx(base, rel=1e-12, abs=1e-12)
+    assert g.delta == pytest.approx((su - sd) / (2 * hs), rel=1e-9, abs=1e-10)
+    assert g.gamma == pytest.approx((su - 2 * base + sd) / hs**2, rel=1e-7, abs=1e-10)
+    assert g.vega == pytest.approx((p(s0, sg + hv, r) - p(s0, sg - hv, r)) / (2 * hv), rel=1e-9, abs=1e-9)
+    assert g.rho == pytest.approx((p(s0, sg, r + hr) - p(s0, sg, r - hr)) / (2 * hr), rel=1e-7, abs=1e-7)
+    assert all(se > 0 for se in g.std_errors)
+
+
+def test_greeks_vs_black_scholes():
+    # tests/greeks.rs: delta 1 %, vega 2 %, rho 2 %; gamma 5 % with a spot bump of 1.0
+    c = K.european_call("AAPL", 100.0, 1.0)
+    m = K.GBM(100.0, 0.2, 0.05, asset="AAPL")
+    g = c.greeks(m, n_paths=500_000, seed=2024, steps_per_year=1)
+    bs = O.bs_call_greeks(100, 100, 1.0, 0.05, 0.2)
+    assert abs(g.delta - bs["delta"]) / bs["delta"] < 0.01
+    assert abs(g.vega - bs["vega"]) / bs["vega"] < 0.02
+    assert abs(g.rho - bs["rho"]) / bs["rho"] < 0.02
+    assert 0 < g.delta < 1 and g.gamma > 0 and g.vega > 0
+    g1 = c.greeks(m, n_paths=500_000, seed=2024, steps_per_year=1, bumps=(1.0, 1e-2, 1e-4))
+    assert abs(g1.gamma - bs["gamma"]) / bs["gamma"] < 0.05
+
+
+def test_greeks_vs_oracle():
+    c = K.up_and_out_call("X", 100, 200, 1.0)
+    n = 20000
+    g = c.greeks(K.GBM(180.0, 0.25, 0.05, asset="X"), n_paths=n, seed=11, steps_per_year=252,
+                 bumps=(1.8, 0.01, 1e-4))
+    o = O.greeks_gbm(c.to_json(), "X", 180.0, 0.25, O.cfg(n, 11, 252, 0.05), 1.8, 0.01, 1e-4)
+    # same Philox stream: the only differences are FP32 paths and barrier flips
+    assert abs(g.price - o["price"]) < 0.5 * g.std_errors[0]
+    assert abs(g.delta - o["delta"]) < 0.5 * g.std_errors[1] + 1e-3
+    assert abs(g.vega - o["vega"]) < 0.5 * g.std_errors[3] + 0.5
