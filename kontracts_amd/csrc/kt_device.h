@@ -10,15 +10,20 @@ namespace kt {
 
 // Philox4x32-10 (Salmon, Moraes, Dror, Shaw: "Parallel random numbers: as easy
 // as 1, 2, 3", SC'11).  Each round is two 32x32->64 products (v_mad_u64_u32)
-// and four xors; the key schedule is wave-uniform.
+// and two three-input xors (gfx950 v_bitop3_b32, truth table 0x96); the key
+// schedule is wave-uniform and lives in SGPRs.
+__device__ __forceinline__ uint32_t xor3(uint32_t a, uint32_t b, uint32_t c) {
+  return __builtin_amdgcn_bitop3_b32(a, b, c, 0x96);
+}
+
 __device__ __forceinline__ uint4 philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0,
                                                uint32_t k1) {
 #pragma unroll
   for (int r = 0; r < 10; ++r) {
     const uint64_t p0 = (uint64_t)0xD2511F53u * c0;
     const uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
-    const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
-    const uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+    const uint32_t n0 = xor3((uint32_t)(p1 >> 32), c1, k0);
+    const uint32_t n2 = xor3((uint32_t)(p0 >> 32), c3, k1);
     c1 = (uint32_t)p1;
     c3 = (uint32_t)p0;
     c0 = n0;
@@ -32,39 +37,44 @@ __device__ __forceinline__ uint4 philox4x32_10(uint32_t c0, uint32_t c1, uint32_
 // Standard normal from one 32-bit word.  The word selects the midpoint of one
 // of 2^24 equal cells of (0,1): u = (2*floor(w/256)+1)/2^25, and z = Phi^-1(u)
 // = sqrt(2) erfinv(2u-1), with erfinv from M. Giles, "Approximating the erfinv
-// function" (GPU Computing Gems Jade, 2011), single-precision version.  The
-// argument x = 2u-1 is an integer times 2^-24 and is formed exactly.
+// function" (GPU Computing Gems Jade, 2011), single-precision version; the
+// sqrt(2) is folded into the polynomial coefficients.  x = 2u-1 is an integer
+// times 2^-24 and is formed exactly; 1-x^2 is one fused multiply-add (one
+// rounding, exact cancellation near |x| = 1).  The tail branch (|z| > ~2.9,
+// 0.34% of draws) is evaluated only when some lane of the wave needs it.
+#define KT_SQ2(c) ((c) * 1.41421356237309505f)
 __device__ __forceinline__ float normal_from_word(uint32_t w) {
   const int i = (int)((w >> 7) | 1u) - (1 << 24);
   const float x = (float)i * 5.9604644775390625e-08f;  // 2^-24
-  const float t = (1.0f - x) * (1.0f + x);
-  float lw = -0.693147180559945f * __builtin_amdgcn_logf(t);  // v_log_f32 is log2
-  float p;
-  if (lw < 5.0f) {
-    lw -= 2.5f;
-    p = 2.81022636e-08f;
-    p = fmaf(p, lw, 3.43273939e-07f);
-    p = fmaf(p, lw, -3.5233877e-06f);
-    p = fmaf(p, lw, -4.39150654e-06f);
-    p = fmaf(p, lw, 0.00021858087f);
-    p = fmaf(p, lw, -0.00125372503f);
-    p = fmaf(p, lw, -0.00417768164f);
-    p = fmaf(p, lw, 0.246640727f);
-    p = fmaf(p, lw, 1.50140941f);
-  } else {
-    lw = __builtin_amdgcn_sqrtf(lw) - 3.0f;
-    p = -0.000200214257f;
-    p = fmaf(p, lw, 0.000100950558f);
-    p = fmaf(p, lw, 0.00134934322f);
-    p = fmaf(p, lw, -0.00367342844f);
-    p = fmaf(p, lw, 0.00573950773f);
-    p = fmaf(p, lw, -0.0076224613f);
-    p = fmaf(p, lw, 0.00943887047f);
-    p = fmaf(p, lw, 1.00167406f);
-    p = fmaf(p, lw, 2.83297682f);
+  const float t = fmaf(-x, x, 1.0f);
+  const float lg = __builtin_amdgcn_logf(t);  // log2(1 - x^2) <= 0
+  const float wc = fmaf(lg, -0.693147180559945f, -2.5f);  // -ln(1-x^2) - 2.5
+  float p = KT_SQ2(2.81022636e-08f);
+  p = fmaf(p, wc, KT_SQ2(3.43273939e-07f));
+  p = fmaf(p, wc, KT_SQ2(-3.5233877e-06f));
+  p = fmaf(p, wc, KT_SQ2(-4.39150654e-06f));
+  p = fmaf(p, wc, KT_SQ2(0.00021858087f));
+  p = fmaf(p, wc, KT_SQ2(-0.00125372503f));
+  p = fmaf(p, wc, KT_SQ2(-0.00417768164f));
+  p = fmaf(p, wc, KT_SQ2(0.246640727f));
+  p = fmaf(p, wc, KT_SQ2(1.50140941f));
+  const bool tail = lg <= -7.21347520444482f;  // -ln(1-x^2) >= 5
+  if (__builtin_amdgcn_ballot_w64(tail)) {
+    const float wt = __builtin_amdgcn_sqrtf(-0.693147180559945f * lg) - 3.0f;
+    float q = KT_SQ2(-0.000200214257f);
+    q = fmaf(q, wt, KT_SQ2(0.000100950558f));
+    q = fmaf(q, wt, KT_SQ2(0.00134934322f));
+    q = fmaf(q, wt, KT_SQ2(-0.00367342844f));
+    q = fmaf(q, wt, KT_SQ2(0.00573950773f));
+    q = fmaf(q, wt, KT_SQ2(-0.0076224613f));
+    q = fmaf(q, wt, KT_SQ2(0.00943887047f));
+    q = fmaf(q, wt, KT_SQ2(1.00167406f));
+    q = fmaf(q, wt, KT_SQ2(2.83297682f));
+    p = tail ? q : p;
   }
-  return 1.41421356237309505f * p * x;
+  return p * x;
 }
+#undef KT_SQ2
 
 __device__ __forceinline__ float exp2_fast(float x) { return __builtin_amdgcn_exp2f(x); }
 

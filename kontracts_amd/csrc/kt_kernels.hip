@@ -55,6 +55,14 @@ __device__ __forceinline__ float scen_s0(const PathArgs& A, int sc, uint32_t ass
 
 #define FOR_SC _Pragma("unroll") for (int sc = 0; sc < SC; ++sc)
 
+// In the greeks_gbm scenario layout (base, S+h, S-h, sigma+-h, r+-h) the two
+// spot-bumped scenarios differ from the base only by s0: they share its
+// log-spot path log2(S/s0).  xsrc(sc) is the scenario whose log-spot sc reads.
+template <int SC>
+__device__ constexpr int xsrc(int sc) {
+  return (SC == kScenarios && (sc == 1 || sc == 2)) ? 0 : sc;
+}
+
 template <int NS, int NOUT, int K, int SC>
 __device__ __forceinline__ void run_ops(const PathArgs& A, int k, int begin, int count, float (&r)[SC][NS],
                                         double (&pv)[SC][NOUT], const float (&x)[SC][K], bool live) {
@@ -71,7 +79,7 @@ __device__ __forceinline__ void run_ops(const PathArgs& A, int k, int begin, int
       case OP_SPOT: {
         const uint32_t asset = (w.x >> 16) & 0xffu;
         const uint32_t ai = asset < (uint32_t)K ? asset : 0u;
-        FOR_SC r[sc][d] = scen_s0<K, SC>(A, sc, ai) * exp2_fast(x[sc][ai]);
+        FOR_SC r[sc][d] = scen_s0<K, SC>(A, sc, ai) * exp2_fast(x[xsrc<SC>(sc)][ai]);
         break;
       }
       case OP_NEG: FOR_SC r[sc][d] = -r[sc][a]; break;
@@ -147,8 +155,10 @@ template <int MODEL, int K, int SC>
 __device__ __forceinline__ void advance(const PathArgs& A, int k, const float* zk, float (&x)[SC][K], float& v) {
   if constexpr (MODEL == MK_GBM) {
     FOR_SC {
-      const float a = ld_f32(A.tab, ((size_t)k * SC + sc) * 2), b = ld_f32(A.tab, ((size_t)k * SC + sc) * 2 + 1);
-      x[sc][0] = fmaf(b, zk[0], x[sc][0] + a);
+      if (xsrc<SC>(sc) == sc) {
+        const float a = ld_f32(A.tab, ((size_t)k * SC + sc) * 2), b = ld_f32(A.tab, ((size_t)k * SC + sc) * 2 + 1);
+        x[sc][0] = fmaf(b, zk[0], x[sc][0] + a);
+      }
     }
   } else if constexpr (MODEL == MK_HESTON) {
     float t[6];
@@ -197,21 +207,23 @@ __device__ __forceinline__ void generic_point(const PathArgs& A, int k, const fl
 }
 
 // One group of 4 grid points under a single monitor, as straight-line code.
-template <int KIND, int MODEL, int K, int NS, int SC, bool DEBUG>
+// The monitor's accumulators (acc, and comp for Kahan sums) are carried in
+// registers across consecutive fast groups and only written back to the
+// register file when the monitored slot changes or an interpreted group
+// follows.  Knock-out groups keep the running max (min) of log-spot over the
+// group and compare it once with the threshold log2(H/s0): some point
+// satisfies S >= H exactly when the group's maximum does.
+template <int KIND, int MODEL, int K, int SC, bool DEBUG>
 __device__ __forceinline__ void fast_group(const PathArgs& A, const KGroup gd, int g, int k0, const float* z,
-                                           float (&x)[SC][K], float& v, float (&r)[SC][NS], bool live,
-                                           uint64_t local) {
+                                           float (&x)[SC][K], float& v, float (&acc)[SC], float (&comp)[SC],
+                                           bool live, uint64_t local) {
   constexpr int M = MODEL == MK_GBM ? 1 : (MODEL == MK_HESTON ? 2 : K);
   constexpr bool KO = KIND >= G_KO_GE && KIND <= G_KO_LT;
-  const uint32_t slot = (uint32_t)gd.slot & (NS - 1);
-  const uint32_t slot1 = (slot + 1) & (NS - 1);
+  constexpr bool UP = KIND == G_KO_GE || KIND == G_KO_GT;
   const uint32_t asset = (uint32_t)gd.asset < (uint32_t)K ? (uint32_t)gd.asset : 0u;
-  float acc[SC], comp[SC], thr[SC], s0v[SC];
+  float ext[SC], s0v[SC];
   FOR_SC {
-    acc[sc] = KIND == G_NONE ? 0.0f : r[sc][slot];
-    comp[sc] = KIND == G_AVG ? r[sc][slot1] : 0.0f;
-    // knock-out in log space: S cmp H  <=>  log2(S/s0) cmp log2(H/s0)  (s0 > 0, H > 0)
-    thr[sc] = KO ? ld_f32(A.thr, (size_t)g * SC + sc) : 0.0f;
+    ext[sc] = UP ? -INFINITY : INFINITY;
     s0v[sc] = scen_s0<K, SC>(A, sc, asset);
   }
 #pragma unroll
@@ -220,11 +232,9 @@ __device__ __forceinline__ void fast_group(const PathArgs& A, const KGroup gd, i
     store_normals<MODEL, K, SC, DEBUG>(A, k0 + s, &z[s * M], live, local);
     if constexpr (KIND != G_NONE) {
       FOR_SC {
-        const float xs = K == 1 ? x[sc][0] : x[sc][asset];
-        if constexpr (KIND == G_KO_GE) acc[sc] = xs >= thr[sc] ? 1.0f : acc[sc];
-        if constexpr (KIND == G_KO_GT) acc[sc] = xs > thr[sc] ? 1.0f : acc[sc];
-        if constexpr (KIND == G_KO_LE) acc[sc] = xs <= thr[sc] ? 1.0f : acc[sc];
-        if constexpr (KIND == G_KO_LT) acc[sc] = xs < thr[sc] ? 1.0f : acc[sc];
+        if (KO && xsrc<SC>(sc) != sc) continue;  // shares the extreme of its source scenario
+        const float xs = K == 1 ? x[xsrc<SC>(sc)][0] : x[xsrc<SC>(sc)][asset];
+        if constexpr (KO) ext[sc] = UP ? fmaxf(ext[sc], xs) : fminf(ext[sc], xs);
         if constexpr (KIND == G_AVG || KIND == G_RMAX || KIND == G_RMIN) {
           const float S = s0v[sc] * exp2_fast(xs);
           if constexpr (KIND == G_AVG) {
@@ -239,10 +249,16 @@ __device__ __forceinline__ void fast_group(const PathArgs& A, const KGroup gd, i
       }
     }
   }
-  if constexpr (KIND != G_NONE) {
+  if constexpr (KO) {
     FOR_SC {
-      r[sc][slot] = acc[sc];
-      if constexpr (KIND == G_AVG) r[sc][slot1] = comp[sc];
+      const float thr = ld_f32(A.thr, (size_t)g * SC + sc);
+      const float e = ext[xsrc<SC>(sc)];
+      bool hit;
+      if constexpr (KIND == G_KO_GE) hit = e >= thr;
+      if constexpr (KIND == G_KO_GT) hit = e > thr;
+      if constexpr (KIND == G_KO_LE) hit = e <= thr;
+      if constexpr (KIND == G_KO_LT) hit = e < thr;
+      acc[sc] = hit ? 1.0f : acc[sc];
     }
   }
 }
@@ -271,6 +287,20 @@ __global__ __launch_bounds__(kBlock) void path_kernel(const PathArgs A) {
     FOR_SC r[sc][((uint32_t)w >> 8) & (NS - 1)] = __uint_as_float((uint32_t)(w >> 32));
   }
   float v = A.v0;
+  // monitor accumulators cached in registers (slot mslot, kind mkind; -1 = none)
+  float macc[SC], mcomp[SC];
+  FOR_SC macc[sc] = mcomp[sc] = 0.0f;
+  int mslot = -1, mkind = 0;
+  auto flush = [&]() {
+    if (mslot >= 0) {
+      const uint32_t s0i = (uint32_t)mslot & (NS - 1), s1i = (s0i + 1) & (NS - 1);
+      FOR_SC {
+        r[sc][s0i] = macc[sc];
+        if (mkind == G_AVG) r[sc][s1i] = mcomp[sc];
+      }
+      mslot = -1;
+    }
+  };
 
   for (int g = 0; g < A.n_groups; ++g) {
     KGroup gd;
@@ -288,16 +318,30 @@ __global__ __launch_bounds__(kBlock) void path_kernel(const PathArgs A) {
       z[4 * bk + 3] = normal_from_word(q.w);
     }
     const int k0 = 4 * g;
+    if (gd.kind != G_GENERIC && gd.kind != G_NONE && gd.slot != mslot) {
+      flush();
+      const uint32_t s0i = (uint32_t)gd.slot & (NS - 1), s1i = (s0i + 1) & (NS - 1);
+      FOR_SC {
+        macc[sc] = r[sc][s0i];
+        mcomp[sc] = gd.kind == G_AVG ? r[sc][s1i] : 0.0f;
+      }
+      mslot = gd.slot;
+      mkind = gd.kind;
+    }
     switch (gd.kind) {
-      case G_NONE: fast_group<G_NONE, MODEL, K, NS, SC, DEBUG>(A, gd, g, k0, z, x, v, r, live, local); break;
-      case G_KO_GE: fast_group<G_KO_GE, MODEL, K, NS, SC, DEBUG>(A, gd, g, k0, z, x, v, r, live, local); break;
-      case G_KO_GT: fast_group<G_KO_GT, MODEL, K, NS, SC, DEBUG>(A, gd, g, k0, z, x, v, r, live, local); break;
-      case G_KO_LE: fast_group<G_KO_LE, MODEL, K, NS, SC, DEBUG>(A, gd, g, k0, z, x, v, r, live, local); break;
-      case G_KO_LT: fast_group<G_KO_LT, MODEL, K, NS, SC, DEBUG>(A, gd, g, k0, z, x, v, r, live, local); break;
-      case G_AVG: fast_group<G_AVG, MODEL, K, NS, SC, DEBUG>(A, gd, g, k0, z, x, v, r, live, local); break;
-      case G_RMAX: fast_group<G_RMAX, MODEL, K, NS, SC, DEBUG>(A, gd, g, k0, z, x, v, r, live, local); break;
-      case G_RMIN: fast_group<G_RMIN, MODEL, K, NS, SC, DEBUG>(A, gd, g, k0, z, x, v, r, live, local); break;
+#define KT_FAST(KIND) \
+  case KIND: fast_group<KIND, MODEL, K, SC, DEBUG>(A, gd, g, k0, z, x, v, macc, mcomp, live, local); break;
+      KT_FAST(G_NONE)
+      KT_FAST(G_KO_GE)
+      KT_FAST(G_KO_GT)
+      KT_FAST(G_KO_LE)
+      KT_FAST(G_KO_LT)
+      KT_FAST(G_AVG)
+      KT_FAST(G_RMAX)
+      KT_FAST(G_RMIN)
+#undef KT_FAST
       default: {
+        flush();
         const int kend = min(4, A.n_grid - k0);
         if constexpr (MODEL == MK_BASKET) {
           // 4K normals exceed what indexed register access covers: unroll the
@@ -317,6 +361,7 @@ __global__ __launch_bounds__(kBlock) void path_kernel(const PathArgs A) {
       }
     }
   }
+  flush();
 
   if constexpr (DEBUG) {
     if (live && A.pv_out) A.pv_out[local] = pv[0][0];
