@@ -60,7 +60,11 @@ __device__ __forceinline__ float normal_from_word(uint32_t w) {
   p = fmaf(p, wc, KT_SQ2(1.50140941f));
   const bool tail = lg <= -7.21347520444482f;  // -ln(1-x^2) >= 5
   if (__builtin_amdgcn_ballot_w64(tail)) {
-    const float wt = __builtin_amdgcn_sqrtf(-0.693147180559945f * lg) - 3.0f;
+    // The empty volatile asm keeps the compiler from evaluating this branch
+    // unconditionally and selecting afterwards.
+    float lt = lg;
+    __asm__ volatile("" : "+v"(lt));
+    const float wt = __builtin_amdgcn_sqrtf(-0.693147180559945f * lt) - 3.0f;
     float q = KT_SQ2(-0.000200214257f);
     q = fmaf(q, wt, KT_SQ2(0.000100950558f));
     q = fmaf(q, wt, KT_SQ2(0.00134934322f));

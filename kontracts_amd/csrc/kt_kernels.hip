@@ -291,12 +291,27 @@ __global__ __launch_bounds__(kBlock) void path_kernel(const PathArgs A) {
   float macc[SC], mcomp[SC];
   FOR_SC macc[sc] = mcomp[sc] = 0.0f;
   int mslot = -1, mkind = 0;
+  // Slot access by compare-select chains: these run only at the boundaries of
+  // monitor runs, and unlike indexed access they leave the register file in
+  // place for the interpreter.
+  auto put = [&](int sc, uint32_t slot, float val) {
+#pragma unroll
+    for (int i = 0; i < NS; ++i)
+      if (slot == (uint32_t)i) r[sc][i] = val;
+  };
+  auto get = [&](int sc, uint32_t slot) {
+    float val = r[sc][0];
+#pragma unroll
+    for (int i = 1; i < NS; ++i)
+      if (slot == (uint32_t)i) val = r[sc][i];
+    return val;
+  };
   auto flush = [&]() {
     if (mslot >= 0) {
       const uint32_t s0i = (uint32_t)mslot & (NS - 1), s1i = (s0i + 1) & (NS - 1);
       FOR_SC {
-        r[sc][s0i] = macc[sc];
-        if (mkind == G_AVG) r[sc][s1i] = mcomp[sc];
+        put(sc, s0i, macc[sc]);
+        if (mkind == G_AVG) put(sc, s1i, mcomp[sc]);
       }
       mslot = -1;
     }
@@ -322,8 +337,8 @@ __global__ __launch_bounds__(kBlock) void path_kernel(const PathArgs A) {
       flush();
       const uint32_t s0i = (uint32_t)gd.slot & (NS - 1), s1i = (s0i + 1) & (NS - 1);
       FOR_SC {
-        macc[sc] = r[sc][s0i];
-        mcomp[sc] = gd.kind == G_AVG ? r[sc][s1i] : 0.0f;
+        macc[sc] = get(sc, s0i);
+        mcomp[sc] = gd.kind == G_AVG ? get(sc, s1i) : 0.0f;
       }
       mslot = gd.slot;
       mkind = gd.kind;
@@ -352,9 +367,17 @@ __global__ __launch_bounds__(kBlock) void path_kernel(const PathArgs A) {
               generic_point<MODEL, K, NS, NOUT, SC, DEBUG>(A, k0 + s, &z[s * M], x, v, r, pv, live, local);
         } else {
           for (int s = 0; s < kend; ++s) {
+            // select this point's normals with uniform compares: a dynamic
+            // index would move z out of registers for every group
             float zs[M];
 #pragma unroll
-            for (int c = 0; c < M; ++c) zs[c] = z[s * M + c];
+            for (int c = 0; c < M; ++c) {
+              float t = z[c];
+#pragma unroll
+              for (int q = 1; q < 4; ++q)
+                if (s == q) t = z[q * M + c];
+              zs[c] = t;
+            }
             generic_point<MODEL, K, NS, NOUT, SC, DEBUG>(A, k0 + s, zs, x, v, r, pv, live, local);
           }
         }
@@ -450,10 +473,8 @@ hipError_t go(const PathArgs& a, int n_blocks, hipStream_t s) {
 template <int MODEL, int K>
 hipError_t go_ns(int ns, int nout, bool debug, const PathArgs& a, int n_blocks, hipStream_t s) {
   if (debug) return go<MODEL, K, 32, 1, 1, 1, true>(a, n_blocks, s);
-  if constexpr (MODEL != MK_BASKET) {
-    if (ns == 16 && nout == 1) return go<MODEL, K, 16, 1, 1, 1, false>(a, n_blocks, s);
-    if (ns == 16 && nout == 8) return go<MODEL, K, 16, 8, 1, 8, false>(a, n_blocks, s);
-  }
+  if (ns == 16 && nout == 1) return go<MODEL, K, 16, 1, 1, 1, false>(a, n_blocks, s);
+  if (ns == 16 && nout == 8) return go<MODEL, K, 16, 8, 1, 8, false>(a, n_blocks, s);
   if (ns == 32 && nout == 1) return go<MODEL, K, 32, 1, 1, 1, false>(a, n_blocks, s);
   if (ns == 32 && nout == 8) return go<MODEL, K, 32, 8, 1, 8, false>(a, n_blocks, s);
   return hipErrorInvalidValue;
@@ -476,8 +497,6 @@ bool path_variant_exists(int model, int k_assets, int n_slots, int n_outputs, bo
   int ns;
   if (debug) {
     if (n_outputs != 1) return false;
-    ns = 32;
-  } else if (model == MK_BASKET) {
     ns = 32;
   } else {
     ns = n_slots <= 16 ? 16 : 32;
