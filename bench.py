@@ -152,6 +152,44 @@ def main():
         r3 = c1.price(h, n_paths=2_097_152, steps_per_year=365)
         extras["config3_heston_forward_path_steps_per_s"] = 2_097_152 * 365 / (time.perf_counter() - t)
         extras["config3_price"] = r3.price
+        fs = [K.GbmFactor(f"A{i}", 100.0, 0.05, 0.15 + 0.05 * i) for i in range(5)]
+        b5 = K.correlated_gbm(fs, [[1.0 if i == j else 0.3 for j in range(5)] for i in range(5)], 0.05)
+        avg5 = (K.spot("A0") + K.spot("A1") + K.spot("A2") + K.spot("A3") + K.spot("A4")) / 5.0
+        cb = K.scale((avg5 - 100.0).clip(0.0), K.one("USD")) @ K.at(1.0)
+        cb.price(b5, n_paths=1 << 20, grid=grid)
+        t = time.perf_counter()
+        rb = cb.price(b5, n_paths=1 << 20, grid=grid)
+        extras["basket5_path_steps_per_s"] = (1 << 20) * 252 / (time.perf_counter() - t)
+        extras["basket5_price"] = rb.price
+        ca = K.scale((K.average(K.spot("X")) - 100.0).clip(0.0), K.one("USD")) @ K.at(1.0)
+        ca.price(m1, n_paths=1 << 20, steps_per_year=252)
+        t = time.perf_counter()
+        ra = ca.price(m1, n_paths=1 << 20, steps_per_year=252)
+        extras["asian_daily_path_steps_per_s"] = (1 << 20) * 252 / (time.perf_counter() - t)
+        extras["asian_price"] = ra.price
+        # American put by LSM: s0=36, K=40, sigma=0.2, r=0.06, T=1, 50 dates,
+        # 2^20 paths stored in HBM.  Algorithmic HBM bytes: the paths kernel
+        # writes payoff + spot (8 B per path and date); each backward pass reads
+        # payoff/spot of two dates (16 B) and reads + writes the f64 value (16 B).
+        mL = K.GBM(36.0, 0.2, 0.06, asset="X")
+        put = K.scale((K.const_(40.0) - K.spot("X")).max(0.0), K.one("USD"))
+        dates = [i / 50 for i in range(1, 51)]
+        nL = 1 << 20
+        put.price_american(mL, dates, n_paths=nL, seed=42)
+        eng.set_profiling(True)
+        eng.kernel_time(reset=True)
+        t = time.perf_counter()
+        rL = put.price_american(mL, dates, n_paths=nL, seed=42)
+        wallL = time.perf_counter() - t
+        kL_ms, _, _ = eng.kernel_time(reset=True)
+        eng.set_profiling(False)
+        bytesL = nL * len(dates) * (8 + 32)
+        extras["config5_lsm_ms"] = wallL * 1e3
+        extras["config5_lsm_device_ms"] = kL_ms
+        extras["config5_lsm_path_steps_per_s"] = nL * len(dates) / wallL
+        extras["config5_lsm_algorithmic_GBps"] = bytesL / (kL_ms / 1e3) / 1e9 if kL_ms > 0 else None
+        extras["config5_price"] = rL.price
+        extras["config5_std_error"] = rL.std_error
 
     line = {
         "metric": METRIC,

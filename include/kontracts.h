@@ -223,6 +223,13 @@ kt_status kt_debug_copy(kt_engine* engine, uint64_t bytes, int32_t reps, double*
 kt_status kt_debug_paths(kt_engine* engine, const char* contract_json, const kt_model* model,
                          const kt_mc_config* cfg, double* pv_out, float* normals_out, double* grid_out,
                          uint64_t grid_cap, uint64_t* n_grid, kt_price_result* out);
+/* kt_price_american that also returns every path's value discounted to t = 0
+ * (pv_out[n_paths]) and, when coef_out != NULL, each regression date's fitted
+ * coefficients: row d (d = 0 .. n_dates-2) holds c_0 .. c_(n_basis-1) and then
+ * 1.0 if a regression was fitted (>= n_basis ITM paths), else 0.0. */
+kt_status kt_debug_american(kt_engine* engine, const char* payoff_json, const double* exercise_dates,
+                            uint64_t n_dates, const kt_model* model, const kt_mc_config* cfg, int32_t n_basis,
+                            double* pv_out, double* coef_out, kt_price_result* out);
 
 #ifdef __cplusplus
 }

@@ -137,6 +137,11 @@ _SIGNATURES = {
         [C.c_void_p, C.c_char_p, _P(C.c_double), C.c_uint64, _P(KtModel), _P(KtMcConfig), C.c_int32,
          _P(KtPriceResult)],
     ),
+    "kt_debug_american": (
+        C.c_int,
+        [C.c_void_p, C.c_char_p, _P(C.c_double), C.c_uint64, _P(KtModel), _P(KtMcConfig), C.c_int32,
+         _P(C.c_double), _P(C.c_double), _P(KtPriceResult)],
+    ),
     "kt_price_portfolio": (C.c_int, [C.c_void_p, _P(KtJob), C.c_uint64, _P(KtPriceResult)]),
     "kt_engine_set_profiling": (C.c_int, [C.c_void_p, C.c_int32]),
     "kt_engine_kernel_time": (

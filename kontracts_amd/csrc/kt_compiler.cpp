@@ -527,6 +527,23 @@ class Lowerer {
     if (c.kind != Con::Zero && contains_price_condition(c)) time_only_ = false;
   }
 
+  // LSM exercise payoff: the contract acquired at every grid point k >= 1
+  // (value_at, src/lsm.rs:238-251).  With every flow at its acquisition step,
+  // each point's list sums the flows paid there; the caller runs it with unit
+  // discount factors and reads the per-point value.
+  void add_exercise_contract(const Con& c) {
+    collect_temporals(c, 0);
+    for (int k = 1; k < n_; ++k) {
+      Acq root;
+      root.k = Acq::Static;
+      root.step = k;
+      Event ctx;
+      ctx.k = Event::Emit;
+      ctx.out = 0;
+      visit(c, root, ctx);
+    }
+  }
+
   Program finish() {
     // Spot slots for every model asset referenced anywhere.
     for (size_t a = 0; a < assets_.size(); ++a) spot_slot_.push_back(-1);
@@ -1093,6 +1110,20 @@ void classify_groups(Program& p) {
     }
     p.groups.push_back(grp);
   }
+}
+
+Program lower_exercise(const Con& contract, const std::vector<double>& grid,
+                       const std::vector<std::string>& model_assets) {
+  if (!flows_at_acquisition(contract))
+    raise(KT_ERR_UNSUPPORTED,
+          "LSM: the exercised payoff must pay at the exercise date (no when/anytime/until/or inside)");
+  if (model_assets.size() > (size_t)kMaxAssets) raise(KT_ERR_INVALID_ARGUMENT, "too many assets");
+  Lowerer L(grid, model_assets);
+  L.add_exercise_contract(contract);
+  Program p = L.finish();
+  p.n_outputs = 1;
+  p.groups.assign((p.steps.size() + 3) / 4, KGroup{G_GENERIC, 0, 0, 0.0f});
+  return p;
 }
 
 std::string Program::describe() const {

@@ -78,6 +78,11 @@ struct Program {
 Program lower(const std::vector<const Con*>& contracts, const std::vector<double>& grid,
               const std::vector<std::string>& model_assets);
 
+// LSM: the exercised payoff evaluated at every grid point k >= 1 (flows at
+// the acquisition step only; raises Unsupported otherwise).
+Program lower_exercise(const Con& contract, const std::vector<double>& grid,
+                       const std::vector<std::string>& model_assets);
+
 // Classify groups of 4 grid points (fills prog.groups).
 void classify_groups(Program& prog);
 

@@ -208,7 +208,13 @@ void* kt_engine::ensure(DevBuf& b, size_t bytes) {
   b.p = nullptr;
   b.cap = 0;
   size_t want = std::max<size_t>(bytes, 4096);
-  check_hip(hipMalloc(&b.p, want), "hipMalloc");
+  hipError_t e = hipMalloc(&b.p, want);
+  if (e == hipErrorOutOfMemory) {
+    (void)hipGetLastError();
+    b.p = nullptr;
+    raise(KT_ERR_OUT_OF_MEMORY, "device allocation of " + std::to_string(want) + " bytes failed");
+  }
+  check_hip(e, "hipMalloc");
   b.cap = want;
   return b.p;
 }
@@ -237,6 +243,19 @@ namespace kt {
 // model kind, different parameters and discount rates); returns the
 // per-output statistics.  pv_host / z_host (nullable) request the debug
 // variant (one scenario only).
+void fill_model_args(PathArgs& A, const ModelDesc& md, uint64_t seed) {
+  const int K = (int)md.names.size();
+  A.heston_full_truncation = md.scheme == KT_HESTON_FULL_TRUNCATION;
+  A.seed_lo = (uint32_t)seed;
+  A.seed_hi = (uint32_t)(seed >> 32);
+  for (int i = 0; i < K && i < kMaxAssets; ++i) A.s0[i] = (float)md.s0[i];
+  for (size_t i = 0; i < md.chol.size() && i < (size_t)kMaxAssets * kMaxAssets; ++i) A.chol[i] = (float)md.chol[i];
+  A.v0 = (float)std::max(md.v0, 0.0);
+  A.rho = (float)md.rho;
+  A.rho_perp = (float)std::sqrt(std::max(1.0 - md.rho * md.rho, 0.0));
+  A.s0_scen[0] = (float)md.s0[0];
+}
+
 std::vector<KStats> run_program_scen(kt_engine* E, const Program& prog, const std::vector<ModelDesc>& models,
                                      const std::vector<double>& rates, const std::vector<std::vector<double>>& stat_w,
                                      const std::vector<double>& grid, uint64_t n_paths, uint64_t seed,
@@ -335,14 +354,7 @@ std::vector<KStats> run_program_scen(kt_engine* E, const Program& prog, const st
   A.n_grid = (int)grid.size();
   A.n_groups = (int)groups.size();
   A.tab_stride = stride;
-  A.heston_full_truncation = md.scheme == KT_HESTON_FULL_TRUNCATION;
-  A.seed_lo = (uint32_t)seed;
-  A.seed_hi = (uint32_t)(seed >> 32);
-  for (int i = 0; i < K && i < kMaxAssets; ++i) A.s0[i] = (float)md.s0[i];
-  for (size_t i = 0; i < md.chol.size(); ++i) A.chol[i] = (float)md.chol[i];
-  A.v0 = (float)std::max(md.v0, 0.0);
-  A.rho = (float)md.rho;
-  A.rho_perp = (float)std::sqrt(std::max(1.0 - md.rho * md.rho, 0.0));
+  fill_model_args(A, md, seed);
   for (int sc = 0; sc < SC && sc < kScenarios; ++sc) {
     A.s0_scen[sc] = (float)models[sc].s0[0];
     A.df_index[sc] = df_index[sc];
@@ -592,6 +604,18 @@ kt_status kt_price_american(kt_engine* E, const char* json, const double* dates,
   if (!E || !out || !cfg) raise(KT_ERR_INVALID_ARGUMENT, "null argument");
   check_hip(hipSetDevice(E->device), "hipSetDevice");
   price_american(E, json, dates, n_dates, model, cfg, n_basis, out);
+  E->err.clear();
+  return KT_OK;
+  KT_GUARD_END(E)
+}
+
+kt_status kt_debug_american(kt_engine* E, const char* json, const double* dates, uint64_t n_dates,
+                            const kt_model* model, const kt_mc_config* cfg, int32_t n_basis, double* pv_out,
+                            double* coef_out, kt_price_result* out) {
+  KT_GUARD_BEGIN
+  if (!E || !out || !cfg || !pv_out) raise(KT_ERR_INVALID_ARGUMENT, "null argument");
+  check_hip(hipSetDevice(E->device), "hipSetDevice");
+  price_american(E, json, dates, n_dates, model, cfg, n_basis, out, pv_out, coef_out);
   E->err.clear();
   return KT_OK;
   KT_GUARD_END(E)

@@ -70,6 +70,10 @@ std::vector<KStats> run_program(kt_engine* E, const Program& prog, const ModelDe
                                 const std::vector<double>& grid, double rate, uint64_t n_paths, uint64_t seed,
                                 double* pv_host, float* z_host);
 
+// Model constants of the path kernels (spots, Cholesky factor, Heston
+// correlation split) and the Philox key.
+void fill_model_args(PathArgs& A, const ModelDesc& md, uint64_t seed);
+
 std::vector<KStats> run_program_scen(kt_engine* E, const Program& prog, const std::vector<ModelDesc>& models,
                                      const std::vector<double>& rates, const std::vector<std::vector<double>>& stat_w,
                                      const std::vector<double>& grid, uint64_t n_paths, uint64_t seed,
@@ -82,8 +86,12 @@ void greeks_bump(kt_engine* E, const char* json, const kt_model* model, const kt
                  uint64_t n_params, double* out, double* stderr_out);
 void greeks_adjoint(kt_engine* E, const char* json, const kt_model* model, const kt_mc_config* cfg, double* out,
                     double* stderr_out, uint64_t n_out);
+// pv_host[n_paths] and coef_host[(n_dates-1) x (n_basis+1)] (nullable)
+// receive each path's discounted value and each date's regression
+// coefficients followed by a fitted flag.
 void price_american(kt_engine* E, const char* json, const double* dates, uint64_t n_dates, const kt_model* model,
-                    const kt_mc_config* cfg, int n_basis, kt_price_result* out);
+                    const kt_mc_config* cfg, int n_basis, kt_price_result* out, double* pv_host = nullptr,
+                    double* coef_host = nullptr);
 void price_portfolio(kt_engine* E, const kt_job* jobs, uint64_t n_jobs, kt_price_result* out);
 
 }  // namespace kt

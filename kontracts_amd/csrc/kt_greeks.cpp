@@ -133,11 +133,6 @@ void greeks_adjoint(kt_engine*, const char*, const kt_model*, const kt_mc_config
   raise(KT_ERR_UNSUPPORTED, "pathwise Greeks: not built yet");
 }
 
-void price_american(kt_engine*, const char*, const double*, uint64_t, const kt_model*, const kt_mc_config*, int,
-                    kt_price_result*) {
-  raise(KT_ERR_UNSUPPORTED, "LSM: not built yet");
-}
-
 void price_portfolio(kt_engine*, const kt_job*, uint64_t, kt_price_result*) {
   raise(KT_ERR_UNSUPPORTED, "portfolio: not built yet");
 }

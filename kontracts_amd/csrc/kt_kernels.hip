@@ -424,7 +424,21 @@ __global__ __launch_bounds__(256) void finalize_kernel(const KStats* __restrict_
                                                        KStats* __restrict__ out) {
   const int j = blockIdx.x;
   KStats acc{0.0, 0.0, 0.0};
-  for (int b = threadIdx.x; b < n_blocks; b += 256) acc = stats_combine(acc, partials[(size_t)b * nout + j]);
+  // Thread t takes blocks t, t + 256, ... eight at a time: the eight loads
+  // issue together and are combined as a pairwise tree (fixed order).
+  for (int b0 = threadIdx.x; b0 < n_blocks; b0 += 8 * 256) {
+    KStats q[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int b = b0 + u * 256;
+      q[u] = b < n_blocks ? partials[(size_t)b * nout + j] : KStats{0.0, 0.0, 0.0};
+    }
+#pragma unroll
+    for (int w = 1; w < 8; w *= 2)
+#pragma unroll
+      for (int u = 0; u < 8; u += 2 * w) q[u] = stats_combine(q[u], q[u + w]);
+    acc = stats_combine(acc, q[0]);
+  }
   __shared__ KStats sh[256];
   sh[threadIdx.x] = acc;
   __syncthreads();
@@ -457,6 +471,293 @@ __global__ __launch_bounds__(256) void debug_copy_kernel(const uint4* __restrict
 __global__ void debug_normals_kernel(const uint32_t* __restrict__ w, uint64_t n, float* __restrict__ out) {
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) out[i] = normal_from_word(w[i]);
+}
+
+// ---------------------------------------------------------------------------
+// Longstaff-Schwartz (src/lsm.rs)
+// ---------------------------------------------------------------------------
+// Simulate each path on [0, dates...] and store, for every exercise date d,
+// the exercised payoff value_at(d) and the regression spot (asset 0) as one
+// FP32 pair, laid out [date][path] so that the backward passes stream them
+// with coalesced 8-byte loads.  The payoff program has unit discount factors.
+template <int MODEL, int K, int NS>
+__global__ __launch_bounds__(kBlock) void lsm_paths_kernel(const PathArgs A, float2* __restrict__ cols,
+                                                           uint64_t stride) {
+  constexpr int M = MODEL == MK_GBM ? 1 : (MODEL == MK_HESTON ? 2 : K);
+  const uint64_t local = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+  const bool live = local < A.n_paths;
+  const uint64_t path = A.path_base + local;
+  const uint32_t p_lo = (uint32_t)path, p_hi = (uint32_t)(path >> 32);
+  float r[1][NS];
+  float x[1][K];
+#pragma unroll
+  for (int i = 0; i < NS; ++i) r[0][i] = 0.0f;
+#pragma unroll
+  for (int i = 0; i < K; ++i) x[0][i] = 0.0f;
+  for (int i = 0; i < A.n_init; ++i) {
+    const uint64_t w = ld_u64(A.init, (size_t)i);
+    r[0][((uint32_t)w >> 8) & (NS - 1)] = __uint_as_float((uint32_t)(w >> 32));
+  }
+  float v = A.v0;
+  for (int g = 0; g < A.n_groups; ++g) {
+    float z[4 * M];
+#pragma unroll
+    for (int bk = 0; bk < M; ++bk) {
+      const uint4 q = philox4x32_10((uint32_t)(g * M + bk), p_lo, p_hi, 0u, A.seed_lo, A.seed_hi);
+      z[4 * bk + 0] = normal_from_word(q.x);
+      z[4 * bk + 1] = normal_from_word(q.y);
+      z[4 * bk + 2] = normal_from_word(q.z);
+      z[4 * bk + 3] = normal_from_word(q.w);
+    }
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      const int k = 4 * g + s;
+      if (k < A.n_grid) {
+        advance<MODEL, K, 1>(A, k, &z[s * M], x, v);
+        if (k >= 1) {
+          double pv[1][1] = {{0.0}};
+          const uint64_t st = ld_u64(A.steps, (size_t)k);
+          const int begin = (int)(uint32_t)st, count = (int)(uint32_t)(st >> 32);
+          if (count > 0) run_ops<NS, 1, K, 1>(A, k, begin, count, r, pv, x, live);
+          if (live) cols[(size_t)(k - 1) * stride + local] = make_float2((float)pv[0][0], A.s0[0] * exp2_fast(x[0][0]));
+        }
+      }
+    }
+  }
+}
+
+constexpr int kLsmMaxBasis = 6;
+constexpr int kLsmPow = 2 * kLsmMaxBasis - 1;        // sums of S^p, p < 2m-1
+constexpr int kLsmReg = kLsmPow + kLsmMaxBasis + 1;  // ... then S^j y, j < m, then the ITM count
+
+struct LsmPass {
+  const float2* cols;  // [n_dates][n] {payoff, spot}
+  double* value;       // [n] realised value at the current date (t = 0 after the last pass)
+  const double* coef;  // coefficients of date d_b: c_0..c_5, then 1 if fitted
+  double* reg;         // [kLsmReg][gridDim.x] block partial sums of date d_a
+  KStats* stats;       // [gridDim.x] final statistics
+  uint64_t n;
+  int n_dates;
+  int d_b;  // apply the exercise decision at this date (-1: start from the last date's payoff)
+  int d_a;  // accumulate the regression sums of this date (-1: none)
+  int final_pass;
+  double disc_b, disc_a, disc0;
+};
+
+// Sum of one value over the block's 256 threads in a fixed order, through
+// LDS: each thread stores its value, 8 threads per value add 32 entries each,
+// one thread adds the 8 partials.  red must hold NV x 257 doubles.
+template <int NV>
+__device__ __forceinline__ void block_sums(const double (&v)[NV], double* red, double* out) {
+  constexpr int kRow = kBlock + 1;
+#pragma unroll
+  for (int j = 0; j < NV; ++j) red[j * kRow + threadIdx.x] = v[j];
+  __syncthreads();
+  double part = 0.0;
+  const int j = threadIdx.x >> 3, seg = threadIdx.x & 7;
+  if (j < NV) {
+#pragma unroll 8
+    for (int i = 0; i < kBlock / 8; ++i) part += red[j * kRow + seg * (kBlock / 8) + i];
+  }
+  __syncthreads();
+  if (j < NV) red[j * 8 + seg] = part;
+  __syncthreads();
+  if (threadIdx.x < NV) {
+    double t = 0.0;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) t += red[threadIdx.x * 8 + q];
+    out[threadIdx.x] = t;
+  }
+}
+
+// One backward step, fused: apply the exercise decision of date d_b with its
+// fitted coefficients (src/lsm.rs:203-222: an ITM path exercises iff its
+// payoff beats the continuation estimate, the polynomial or, with no fit, the
+// realised discounted value; the carried value is always realised), then
+// accumulate date d_a's ITM normal equations (poly_fit, src/lsm.rs:259-288)
+// in FP64 per block.  The last pass discounts to t = 0 and reduces the
+// statistics.  Paths are visited grid-stride; all sums have a fixed order.
+template <int MB>
+__global__ __launch_bounds__(kBlock) void lsm_pass_kernel(const LsmPass P) {
+  constexpr int NP = 2 * MB - 1;
+  constexpr int NV = NP + MB + 1;
+  const bool regress = P.d_a >= 0;
+  double acc[NV];
+#pragma unroll
+  for (int j = 0; j < NV; ++j) acc[j] = 0.0;
+  KStats st{0.0, 0.0, 0.0};
+  bool have = false;
+  double c[MB];
+  if (P.d_b >= 0) {
+    have = ld_f64(P.coef, kLsmMaxBasis) != 0.0;
+#pragma unroll
+    for (int j = 0; j < MB; ++j) c[j] = ld_f64(P.coef, j);
+  }
+  const uint64_t step = (uint64_t)gridDim.x * kBlock;
+  const uint64_t n_round = (P.n + step - 1) / step * step;  // every lane runs the same trip count
+  for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < n_round; i += step) {
+    const bool live = i < P.n;
+    const uint64_t ii = live ? i : 0;
+    double val;
+    if (P.d_b < 0) {
+      val = (double)P.cols[(size_t)(P.n_dates - 1) * P.n + ii].x;
+    } else {
+      const double y = P.value[ii] * P.disc_b;
+      const float2 ps = P.cols[(size_t)P.d_b * P.n + ii];
+      val = y;
+      if (ps.x > 0.0f) {
+        double cont = y;
+        if (have) {
+          const double S = (double)ps.y;
+          cont = c[MB - 1];
+#pragma unroll
+          for (int j = MB - 2; j >= 0; --j) cont = cont * S + c[j];
+        }
+        val = (double)ps.x > cont ? (double)ps.x : y;
+      }
+    }
+    if (P.final_pass) {
+      const double pv = val * P.disc0;
+      if (live) P.value[ii] = pv;
+      st = stats_combine(st, wave_stats(pv, live));
+    } else if (live) {
+      P.value[ii] = val;
+    }
+    if (regress) {
+      const float2 ps = P.cols[(size_t)P.d_a * P.n + ii];
+      if (live && ps.x > 0.0f) {
+        const double S = (double)ps.y;
+        const double y = val * P.disc_a;
+        double pw = 1.0;
+#pragma unroll
+        for (int p = 0; p < NP; ++p) {
+          acc[p] += pw;
+          if (p < MB) acc[NP + p] = fma(pw, y, acc[NP + p]);
+          pw *= S;
+        }
+        acc[NV - 1] += 1.0;
+      }
+    }
+  }
+  __shared__ double red[kLsmReg * (kBlock + 1)];
+  if (P.final_pass) {
+    __shared__ KStats shs[kBlock / 64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) shs[wave] = st;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      KStats t = shs[0];
+#pragma unroll
+      for (int wv = 1; wv < kBlock / 64; ++wv) t = stats_combine(t, shs[wv]);
+      P.stats[blockIdx.x] = t;
+    }
+  }
+  if (!regress) return;
+  __shared__ double out[kLsmReg];
+  block_sums<NV>(acc, red, out);
+  __syncthreads();
+  // layout [value][block]: sums S^p at 0..NP-1, S^j y at kLsmPow.., count last
+  if (threadIdx.x < NV) {
+    const int j = threadIdx.x;
+    const int slot = j < NP ? j : (j < NP + MB ? kLsmPow + (j - NP) : kLsmReg - 1);
+    P.reg[(size_t)slot * gridDim.x + blockIdx.x] = out[j];
+  }
+}
+
+// Reduce the pass's block partials in a fixed order and solve the m x m
+// normal equations by Gauss elimination with partial pivoting (solve_linear,
+// src/numerics.rs:190-236: a pivot below 1e-12 gives zero coefficients).  No
+// regression when fewer than m paths are ITM (src/lsm.rs:195).
+template <int MB>
+__global__ __launch_bounds__(kBlock) void lsm_solve_kernel(const double* __restrict__ reg, int n_blocks,
+                                                           double* __restrict__ coef) {
+  constexpr int NP = 2 * MB - 1;
+  constexpr int NV = NP + MB + 1;
+  // n_blocks <= kLsmPassBlocks: every load is independent and issues at once.
+  double v[NV];
+#pragma unroll
+  for (int j = 0; j < NV; ++j) v[j] = 0.0;
+#pragma unroll
+  for (int q = 0; q < kLsmPassBlocks / kBlock; ++q) {
+    const int b = threadIdx.x + q * kBlock;
+    if (b < n_blocks) {
+#pragma unroll
+      for (int j = 0; j < NV; ++j) {
+        const int slot = j < NP ? j : (j < NP + MB ? kLsmPow + (j - NP) : kLsmReg - 1);
+        v[j] += reg[(size_t)slot * n_blocks + b];
+      }
+    }
+  }
+  __shared__ double red[kLsmReg * (kBlock + 1)];
+  __shared__ double tot[kLsmReg];
+  block_sums<NV>(v, red, tot);
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  double a[MB][MB], b[MB], x[MB];
+#pragma unroll
+  for (int j = 0; j < MB; ++j) x[j] = 0.0;
+  const bool have = tot[NV - 1] >= (double)MB;
+  if (have) {
+#pragma unroll
+    for (int r0 = 0; r0 < MB; ++r0) {
+#pragma unroll
+      for (int c0 = 0; c0 < MB; ++c0) a[r0][c0] = tot[r0 + c0];
+      b[r0] = tot[NP + r0];
+    }
+    bool singular = false;
+#pragma unroll
+    for (int col = 0; col < MB; ++col) {
+      if (singular) break;
+      // pivot: the first row of largest magnitude, then swap by selects so
+      // that every index stays a compile-time constant
+      int piv = col;
+      double best = fabs(a[col][col]);
+#pragma unroll
+      for (int row = col + 1; row < MB; ++row) {
+        const double m = fabs(a[row][col]);
+        if (m > best) {
+          best = m;
+          piv = row;
+        }
+      }
+      if (best < 1e-12) {
+        singular = true;
+        break;
+      }
+#pragma unroll
+      for (int row = col + 1; row < MB; ++row) {
+        const bool sw = row == piv;
+#pragma unroll
+        for (int c0 = 0; c0 < MB; ++c0) {
+          const double t = a[col][c0];
+          a[col][c0] = sw ? a[row][c0] : t;
+          a[row][c0] = sw ? t : a[row][c0];
+        }
+        const double tb = b[col];
+        b[col] = sw ? b[row] : tb;
+        b[row] = sw ? tb : b[row];
+      }
+#pragma unroll
+      for (int row = col + 1; row < MB; ++row) {
+        const double f = a[row][col] / a[col][col];
+#pragma unroll
+        for (int c0 = col; c0 < MB; ++c0) a[row][c0] -= f * a[col][c0];
+        b[row] -= f * b[col];
+      }
+    }
+    if (!singular) {
+#pragma unroll
+      for (int row = MB - 1; row >= 0; --row) {
+        double sum = b[row];
+#pragma unroll
+        for (int c0 = row + 1; c0 < MB; ++c0) sum -= a[row][c0] * x[c0];
+        x[row] = sum / a[row][row];
+      }
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < kLsmMaxBasis; ++j) coef[j] = j < MB ? x[j] : 0.0;
+  coef[kLsmMaxBasis] = have ? 1.0 : 0.0;
 }
 
 // ---------------------------------------------------------------------------
@@ -528,6 +829,59 @@ hipError_t launch_path(int model, int k, int ns, int nout, bool debug, int n_sce
       }
   }
   return hipErrorInvalidValue;
+}
+
+hipError_t launch_lsm_paths(int model, int k, int ns, const PathArgs& a, int n_blocks, float2* cols,
+                            uint64_t stride, hipStream_t s) {
+#define KT_LSM(MODEL, KK, NSV) \
+  hipLaunchKernelGGL((lsm_paths_kernel<MODEL, KK, NSV>), dim3(n_blocks), dim3(kBlock), 0, s, a, cols, stride)
+  if (model == MK_GBM) {
+    if (ns == 16) KT_LSM(MK_GBM, 1, 16);
+    else KT_LSM(MK_GBM, 1, 32);
+  } else if (model == MK_HESTON) {
+    if (ns == 16) KT_LSM(MK_HESTON, 1, 16);
+    else KT_LSM(MK_HESTON, 1, 32);
+  } else {
+    switch (k) {
+      case 2: KT_LSM(MK_BASKET, 2, 32); break;
+      case 3: KT_LSM(MK_BASKET, 3, 32); break;
+      case 4: KT_LSM(MK_BASKET, 4, 32); break;
+      case 5: KT_LSM(MK_BASKET, 5, 32); break;
+      case 6: KT_LSM(MK_BASKET, 6, 32); break;
+      case 7: KT_LSM(MK_BASKET, 7, 32); break;
+      case 8: KT_LSM(MK_BASKET, 8, 32); break;
+      default: return hipErrorInvalidValue;
+    }
+  }
+#undef KT_LSM
+  return hipGetLastError();
+}
+
+hipError_t launch_lsm_pass(const LsmPassArgs& p, int n_blocks, hipStream_t s) {
+  LsmPass P{p.cols, p.value, p.coef, p.reg, p.stats, p.n, p.n_dates, p.d_b, p.d_a, p.final_pass,
+            p.disc_b, p.disc_a, p.disc0};
+  switch (p.m) {
+    case 2: hipLaunchKernelGGL(lsm_pass_kernel<2>, dim3(n_blocks), dim3(kBlock), 0, s, P); break;
+    case 3: hipLaunchKernelGGL(lsm_pass_kernel<3>, dim3(n_blocks), dim3(kBlock), 0, s, P); break;
+    case 4: hipLaunchKernelGGL(lsm_pass_kernel<4>, dim3(n_blocks), dim3(kBlock), 0, s, P); break;
+    case 5: hipLaunchKernelGGL(lsm_pass_kernel<5>, dim3(n_blocks), dim3(kBlock), 0, s, P); break;
+    case 6: hipLaunchKernelGGL(lsm_pass_kernel<6>, dim3(n_blocks), dim3(kBlock), 0, s, P); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_lsm_solve(const double* reg, int n_blocks, int m, double* coef, hipStream_t s) {
+  if (n_blocks < 1 || n_blocks > kLsmPassBlocks) return hipErrorInvalidValue;
+  switch (m) {
+    case 2: hipLaunchKernelGGL(lsm_solve_kernel<2>, dim3(1), dim3(kBlock), 0, s, reg, n_blocks, coef); break;
+    case 3: hipLaunchKernelGGL(lsm_solve_kernel<3>, dim3(1), dim3(kBlock), 0, s, reg, n_blocks, coef); break;
+    case 4: hipLaunchKernelGGL(lsm_solve_kernel<4>, dim3(1), dim3(kBlock), 0, s, reg, n_blocks, coef); break;
+    case 5: hipLaunchKernelGGL(lsm_solve_kernel<5>, dim3(1), dim3(kBlock), 0, s, reg, n_blocks, coef); break;
+    case 6: hipLaunchKernelGGL(lsm_solve_kernel<6>, dim3(1), dim3(kBlock), 0, s, reg, n_blocks, coef); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
 }
 
 hipError_t launch_finalize(const KStats* partials, int n_blocks, int nout, int n_used, KStats* out,

@@ -60,6 +60,28 @@ bool path_variant_exists(int model, int k_assets, int n_slots, int n_outputs, bo
                          int* ns_chosen, int* nout_chosen);
 hipError_t launch_path(int model, int k_assets, int ns, int nout, bool debug, int n_scen, const PathArgs& a,
                        int n_blocks, hipStream_t s);
+// Longstaff-Schwartz kernels.
+constexpr int kLsmBasisMax = 6;
+constexpr int kLsmRegCount = (2 * kLsmBasisMax - 1) + kLsmBasisMax + 1;
+constexpr int kLsmPassBlocks = 1024;  // grid of the backward passes (grid-stride over paths)
+struct LsmPassArgs {
+  const float2* cols;
+  double* value;
+  const double* coef;
+  double* reg;
+  KStats* stats;
+  uint64_t n;
+  int n_dates, m, d_b, d_a, final_pass;
+  double disc_b, disc_a, disc0;
+};
+// Writes cols[(k-1)*stride + local path] = {payoff, spot} for k = 1..n_grid-1.
+hipError_t launch_lsm_paths(int model, int k_assets, int ns, const PathArgs& a, int n_blocks, float2* cols,
+                            uint64_t stride, hipStream_t s);
+// reg holds kLsmRegCount x n_blocks doubles; m in [2, kLsmBasisMax].
+hipError_t launch_lsm_pass(const LsmPassArgs& p, int n_blocks, hipStream_t s);
+// One block: reduce reg and write coef[0..kLsmBasisMax] (coefficients, fitted flag).
+hipError_t launch_lsm_solve(const double* reg, int n_blocks, int m, double* coef, hipStream_t s);
+
 hipError_t launch_finalize(const KStats* partials, int n_blocks, int nout, int n_used, KStats* out,
                            hipStream_t s);
 hipError_t launch_debug_philox(const uint32_t* ctr, uint32_t k0, uint32_t k1, uint64_t n, uint32_t* out,
