@@ -190,6 +190,37 @@ def main():
         extras["config5_lsm_algorithmic_GBps"] = bytesL / (kL_ms / 1e3) / 1e9 if kL_ms > 0 else None
         extras["config5_price"] = rL.price
         extras["config5_std_error"] = rL.std_error
+        # Config 3 Greeks: Heston 2^21 paths x 365 steps, d price / d {s0, v0,
+        # kappa, theta, sigma_v, rho}: pathwise (one forward + adjoint launch)
+        # vs CRN bump-and-reprice (13 pricings); then at equal standard error:
+        # paths needed scale with the variance, so the speedup per Greek is
+        # (t_bump / t_pathwise) * (se_bump / se_pathwise)^2.
+        hm = K.heston(100, 0.04, 2.0, 0.04, 0.3, -0.5, 0.05, asset="X")
+        n3 = 2_097_152
+        hb = [1.0, 1e-3, 1e-2, 1e-3, 1e-2, 1e-2]
+        c1.pathwise_greeks(hm, n_paths=n3, steps_per_year=365)
+        eng.set_profiling(True)
+        eng.kernel_time(reset=True)
+        t = time.perf_counter()
+        gA = c1.pathwise_greeks(hm, n_paths=n3, steps_per_year=365)
+        tA = time.perf_counter() - t
+        kA_ms, _, _ = eng.kernel_time(reset=True)
+        eng.set_profiling(False)
+        c1.bump_greeks(hm, bumps=hb, n_paths=n3, steps_per_year=365, std_errors=False)
+        t = time.perf_counter()
+        c1.bump_greeks(hm, bumps=hb, n_paths=n3, steps_per_year=365, std_errors=False)
+        tB = time.perf_counter() - t
+        gB = c1.bump_greeks(hm, bumps=hb, n_paths=n3, steps_per_year=365)
+        eq = {nm: (tB / tA) * (sb / sa) ** 2 for nm, sa, sb in zip(gA.names, gA.std_errors, gB.std_errors)}
+        extras["config3_pathwise_ms"] = tA * 1e3
+        extras["config3_pathwise_kernel_ms"] = kA_ms
+        extras["config3_bump13_ms"] = tB * 1e3
+        extras["config3_speedup_raw"] = tB / tA
+        extras["config3_speedup_equal_stderr"] = eq
+        extras["config3_pathwise"] = dict(zip(gA.names, gA.values))
+        extras["config3_pathwise_se"] = dict(zip(gA.names, gA.std_errors))
+        extras["config3_bump"] = dict(zip(gB.names, gB.values))
+        extras["config3_bump_se"] = dict(zip(gB.names, gB.std_errors))
 
     line = {
         "metric": METRIC,

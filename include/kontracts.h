@@ -150,17 +150,28 @@ kt_status kt_greeks_gbm(kt_engine* engine, const char* contract_json, const char
  * model's parameters.  Parameters: GBM {s0, sigma, mu}; HESTON {s0, v0, kappa,
  * theta, sigma_v, rho}.  h[i] are absolute bumps (n_params entries).
  * out[0] = price, out[1+i] = dPrice/dparam_i; stderr_out (nullable) holds the
- * standard error of each.  Reprices with separate launches at the same seed. */
+ * standard error of each (for a derivative: of the per-path differences of
+ * the two bumped present values).  Reprices with separate launches at the
+ * same seed. */
 kt_status kt_greeks_bump(kt_engine* engine, const char* contract_json, const kt_model* model,
                          const kt_mc_config* cfg, const double* h, uint64_t n_params, double* out,
                          double* stderr_out);
 
-/* Pathwise (adjoint) Greeks for Lipschitz payoffs: contracts whose conditions
- * depend on time only (European, Asian, basket; no barrier, no digital).  A
- * contract with a price-dependent condition returns KT_ERR_UNSUPPORTED: use
- * kt_greeks_bump for those.  Parameters as in kt_greeks_bump; out[0] = price,
- * out[1+i] = dPrice/dparam_i; stderr_out (nullable) as above.  No reference
- * counterpart (the reference has bump Greeks only, src/greeks.rs). */
+/* Pathwise (adjoint) Greeks for Lipschitz payoffs paid on one date (the
+ * last grid point): arithmetic functions of the spots there and of running
+ * averages / extremes of one asset's spot (European, Asian, lookback,
+ * basket).  Anything else (barriers, digitals, price-dependent conditions,
+ * flows on several dates) returns KT_ERR_UNSUPPORTED: use kt_greeks_bump.
+ * Parameters: GBM {s0, sigma, mu}; HESTON {s0, v0, kappa, theta, sigma_v,
+ * rho}; CORRELATED_GBM {s0 of each asset, sigma of each, mu of each}.
+ * out[0] = price, out[1+i] = dPrice/dparam_i (n_out >= 1 + #params);
+ * stderr_out (nullable) holds each one's Monte Carlo standard error.  Heston
+ * runs a reverse sweep that recomputes each segment of the path (16 grid
+ * points by default, kt_engine_set_option "adjoint_segment") from a
+ * checkpoint.  Heston pathwise derivatives assume the variance seldom reaches
+ * the floor (Feller: 2 kappa theta >= sigma_v^2); where it often does, sqrt(v)
+ * is not Lipschitz and bump Greeks are the unbiased choice.  No reference counterpart (the reference has bump Greeks
+ * only, src/greeks.rs). */
 kt_status kt_greeks_adjoint(kt_engine* engine, const char* contract_json, const kt_model* model,
                             const kt_mc_config* cfg, double* out, double* stderr_out, uint64_t n_out);
 
@@ -192,6 +203,12 @@ kt_status kt_price_portfolio(kt_engine* engine, const kt_job* jobs, uint64_t n_j
 kt_status kt_engine_set_profiling(kt_engine* engine, int32_t enabled);
 kt_status kt_engine_kernel_time(kt_engine* engine, double* total_ms, uint64_t* n_launches, double* path_steps,
                                 int32_t reset);
+
+/* Tuning knobs of an engine (no reference counterpart).  Names:
+ *   "adjoint_segment": Heston pathwise-Greek checkpoint interval, 16 or 32
+ *                      grid points (default 16).
+ * Unknown names or values return KT_ERR_INVALID_ARGUMENT. */
+kt_status kt_engine_set_option(kt_engine* engine, const char* name, int64_t value);
 
 /* ---- Host-only (no GPU needed) -------------------------------------------
  * compile() + Plan::time_grid_forced (src/compiler.rs:54-85).  Writes up to

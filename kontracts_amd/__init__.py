@@ -4,7 +4,10 @@ Drop-in for the path-simulation hot path of the reference's Python package
 (python/kontract/__init__.py): the same contract algebra and the same
 ``Contract.price(model, n_paths, seed, steps_per_year)``, ``Contract.greeks``
 and ``Contract.price_american`` calls, executed by hand-written gfx950 kernels
-behind the C ABI in include/kontracts.h.
+behind the C ABI in include/kontracts.h.  Beyond the reference:
+``Contract.pathwise_greeks`` / ``Contract.bump_greeks`` (derivatives with
+respect to every model parameter) and ``kontracts_amd.autograd`` (a
+torch.autograd.Function over the pathwise Greeks).
 """
 from ._native import DeviceError, KontractError
 from .dsl import (
@@ -18,6 +21,7 @@ from .dsl import (
     GbmFactor,
     Greeks,
     Model,
+    ModelGreeks,
     Observable,
     Plan,
     PriceResult,
@@ -61,7 +65,7 @@ from .dsl import (
 
 __all__ = [
     "DeviceError", "KontractError", "EUR", "GBM", "GBP", "JPY", "USD", "Condition", "Contract", "GbmFactor",
-    "Greeks", "Model", "Observable", "Plan", "PriceResult", "S", "__version__", "and_", "anytime", "at",
+    "Greeks", "Model", "ModelGreeks", "Observable", "Plan", "PriceResult", "S", "__version__", "and_", "anytime", "at",
     "average", "average_over", "bool_", "bull_call_spread", "cash_or_nothing_call", "compile", "const_",
     "correlated_gbm", "describe_program", "down_and_out_call", "european_call", "european_put", "forward", "give",
     "heston", "konst", "one", "or_", "plan", "price_batch", "price_portfolio", "running_max", "running_min",

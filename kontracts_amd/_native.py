@@ -144,6 +144,7 @@ _SIGNATURES = {
     ),
     "kt_price_portfolio": (C.c_int, [C.c_void_p, _P(KtJob), C.c_uint64, _P(KtPriceResult)]),
     "kt_engine_set_profiling": (C.c_int, [C.c_void_p, C.c_int32]),
+    "kt_engine_set_option": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int64]),
     "kt_engine_kernel_time": (
         C.c_int, [C.c_void_p, _P(C.c_double), _P(C.c_uint64), _P(C.c_double), C.c_int32]),
     "kt_plan": (
@@ -209,6 +210,10 @@ class Engine:
 
     def set_profiling(self, on: bool) -> None:
         check(lib().kt_engine_set_profiling(self.handle, 1 if on else 0), self.handle)
+
+    def set_option(self, name: str, value: int) -> None:
+        """Engine tuning knob (see kt_engine_set_option in include/kontracts.h)."""
+        check(lib().kt_engine_set_option(self.handle, name.encode(), int(value)), self.handle)
 
     def kernel_time(self, reset: bool = False):
         """(total ms, launches, path-steps) of the path kernels since the last reset."""

@@ -1,0 +1,512 @@
+// kt_adjoint.hip — pathwise Greeks on gfx950 (no reference counterpart: the
+// reference has bump-and-reprice Greeks only, src/greeks.rs).
+//
+// Supported payoffs pay on one date, the last grid point n, an arithmetic
+// function of the spots at n and of running sums / extremes of spots.  The
+// per-path sample of d(price)/d(theta) is the pathwise derivative
+//   sum over leaves L of  d(pv)/dL * dL/dtheta,
+// where the leaves are the spots at n and the accumulators.  d(pv)/dL comes
+// from a dual-number pass over point n's ops; dL/dtheta from the dynamics:
+//
+// * Heston: a reverse (adjoint) sweep.  The forward pass stores the state
+//   (log-spot, variance) every SEG points (16 by default, 32 also built); the
+//   backward pass walks the
+//   segments from last to first, recomputes each one from its checkpoint
+//   (normals regenerated from the Philox counters, kept in registers with the
+//   variance), then runs the adjoint recursion of the Euler step back through
+//   it, propagating d/dv through sqrt and the floor and accumulating
+//   d/d{kappa, theta, sigma_v, rho}.  The spot seeds of each point (running
+//   sums: every fed point; running extremes: the point that set the extreme)
+//   enter as the sweep passes them.
+// * GBM and correlated GBM: d x_{p+1} / d x_p = 1, so the adjoint sweep is a
+//   running sum and collapses into the forward pass: it carries d x/d sigma and
+//   d x/d mu per asset (two FMAs per step) and each accumulator's derivative.
+#include "kt_path_ops.h"
+
+namespace kt {
+
+namespace {
+
+constexpr float kLn2f = 0.693147180559945309f;
+constexpr float kLog2ef = 1.44269504088896341f;
+
+__device__ __forceinline__ int acc_index(const AdjArgs& A, uint32_t slot) {
+  int q = 0;
+#pragma unroll
+  for (int i = 1; i < kAdjMaxAcc; ++i)
+    if ((uint32_t)A.acc_slot[i] == slot) q = i;
+  return q;
+}
+
+// Dual-number pass over the ops of point k: every register carries its value
+// and its derivative with respect to one leaf (leaf < K: the spot of asset
+// `leaf` at this point; leaf = K + q: accumulator q).  Accumulator registers
+// keep their forward values (their update ops only set the leaf's tangent),
+// every other register is recomputed.  Returns d(pv)/d(leaf) through EMIT.
+template <int NS, int K>
+__device__ float dual_point(const AdjArgs& A, int k, int begin, int count, const float (&S)[K],
+                            const float (&r)[1][NS], int leaf) {
+  const PathArgs& P = A.p;
+  float rv[NS], rt[NS];
+#pragma unroll
+  for (int i = 0; i < NS; ++i) {
+    rv[i] = r[0][i];
+    rt[i] = 0.0f;
+  }
+  float dpv = 0.0f;
+  const float df = (float)ld_f64(P.df, (size_t)k);
+  for (int i = 0; i < count; ++i) {
+    const uint64_t word = ld_u64(P.ops, (size_t)begin + i);
+    const uint32_t wx = (uint32_t)word;
+    const uint32_t code = wx & 0xffu;
+    const uint32_t d = (wx >> 8) & (NS - 1), a = (wx >> 16) & (NS - 1), b = (wx >> 24) & (NS - 1);
+    const float imm = __uint_as_float((uint32_t)(word >> 32));
+    const float va = rv[a], vb = rv[b], ta = rt[a], tb = rt[b];
+    float v = 0.0f, t = 0.0f;
+    bool wr = true;
+    switch (code) {
+      case OP_CONST: v = imm; break;
+      case OP_SPOT: {
+        const uint32_t asset = (wx >> 16) & 0xffu;
+        const uint32_t ai = asset < (uint32_t)K ? asset : 0u;
+        v = S[0];
+#pragma unroll
+        for (int c = 1; c < K; ++c)
+          if (ai == (uint32_t)c) v = S[c];
+        t = (uint32_t)leaf == ai ? 1.0f : 0.0f;
+        break;
+      }
+      case OP_NEG: v = -va; t = -ta; break;
+      case OP_ADD: v = va + vb; t = ta + tb; break;
+      case OP_SUB: v = va - vb; t = ta - tb; break;
+      case OP_MUL: v = va * vb; t = fmaf(ta, vb, va * tb); break;
+      case OP_DIV: v = va / vb; t = (ta - v * tb) / vb; break;
+      case OP_MAX: v = va >= vb ? va : vb; t = va >= vb ? ta : tb; break;
+      case OP_MIN: v = va <= vb ? va : vb; t = va <= vb ? ta : tb; break;
+      case OP_ADDI: v = va + imm; t = ta; break;
+      case OP_SUBI: v = va - imm; t = ta; break;
+      case OP_RSUBI: v = imm - va; t = -ta; break;
+      case OP_MULI: v = va * imm; t = ta * imm; break;
+      case OP_DIVI: v = va / imm; t = ta / imm; break;
+      case OP_RDIVI: v = imm / va; t = -v * ta / va; break;
+      case OP_MAXI: v = va >= imm ? va : imm; t = va >= imm ? ta : 0.0f; break;
+      case OP_MINI: v = va <= imm ? va : imm; t = va <= imm ? ta : 0.0f; break;
+      case OP_AVG:
+      case OP_RMAX:
+      case OP_RMIN: {
+        const int q = acc_index(A, d);
+        if (code == OP_AVG) rt[(d + 1) & (NS - 1)] = 0.0f;  // compensation term: no derivative
+        v = rv[d];
+        t = leaf == K + q ? 1.0f : 0.0f;
+        break;
+      }
+      case OP_EMIT:
+        dpv = fmaf(imm * fmaf(ta, vb, va * tb), df, dpv);
+        wr = false;
+        break;
+      default: wr = false; break;
+    }
+    if (wr) {
+      rv[d] = v;
+      rt[d] = t;
+    }
+  }
+  return dpv;
+}
+
+// Per-block reduction of NOUT per-path samples into partials[block][NOUT].
+template <int NOUT>
+__device__ __forceinline__ void block_stats_out(const double (&vals)[NOUT], bool live, KStats* partials) {
+  __shared__ KStats sh[kBlock / 64][NOUT];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int j = 0; j < NOUT; ++j) {
+    const KStats s = wave_stats(vals[j], live);
+    if (lane == 0) sh[wave][j] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x < NOUT) {
+    KStats t = sh[0][threadIdx.x];
+#pragma unroll
+    for (int wv = 1; wv < kBlock / 64; ++wv) t = stats_combine(t, sh[wv][threadIdx.x]);
+    partials[(size_t)blockIdx.x * NOUT + threadIdx.x] = t;
+  }
+}
+
+// Forward part shared by both kernels: one grid point's program ops, with
+// the point at which each running extreme was last set.
+template <int NS, int K>
+__device__ __forceinline__ void run_point_ops(const AdjArgs& A, int k, float (&r)[1][NS], double (&pv)[1][1],
+                                              const float (&x)[1][K], bool live, uint32_t fm,
+                                              int (&argstep)[kAdjMaxAcc]) {
+  const uint64_t st = ld_u64(A.p.steps, (size_t)k);
+  const int begin = (int)(uint32_t)st, count = (int)(uint32_t)(st >> 32);
+  if (count == 0) return;
+  float prev[kAdjMaxAcc];
+#pragma unroll
+  for (int q = 0; q < kAdjMaxAcc; ++q) prev[q] = (fm >> q) & 1u ? r[0][A.acc_slot[q] & (NS - 1)] : 0.0f;
+  run_ops<NS, 1, K, 1>(A.p, k, begin, count, r, pv, x, live);
+#pragma unroll
+  for (int q = 0; q < kAdjMaxAcc; ++q)
+    if (((fm >> q) & 1u) && A.acc_kind[q] != OP_AVG && r[0][A.acc_slot[q] & (NS - 1)] != prev[q]) argstep[q] = k;
+}
+
+template <int NS>
+__device__ __forceinline__ void init_registers(const PathArgs& P, float (&r)[1][NS]) {
+#pragma unroll
+  for (int i = 0; i < NS; ++i) r[0][i] = 0.0f;
+  for (int i = 0; i < P.n_init; ++i) {
+    const uint64_t w = ld_u64(P.init, (size_t)i);
+    r[0][((uint32_t)w >> 8) & (NS - 1)] = __uint_as_float((uint32_t)(w >> 32));
+  }
+}
+
+}  // namespace
+
+// GBM (K = 1) and correlated GBM (K >= 2).  Outputs: price, d/ds0_a, d/dsigma_a, d/dmu_a.
+template <int K, int NS>
+__global__ __launch_bounds__(kBlock) void adjoint_gbm_kernel(const AdjArgs A) {
+  constexpr int MODEL = K == 1 ? MK_GBM : MK_BASKET;
+  constexpr int M = K;
+  constexpr int NP = 3 * K, NOUT = NP + 1;
+  const PathArgs& P = A.p;
+  const uint64_t local = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+  const bool live = local < P.n_paths;
+  const uint64_t path = P.path_base + local;
+  const uint32_t p_lo = (uint32_t)path, p_hi = (uint32_t)(path >> 32);
+  const int n = P.n_grid - 1;
+  float r[1][NS];
+  init_registers<NS>(P, r);
+  float x[1][K], js[K], jm[K];
+#pragma unroll
+  for (int a = 0; a < K; ++a) x[0][a] = js[a] = jm[a] = 0.0f;
+  float W[kAdjMaxAcc][3];
+  int argstep[kAdjMaxAcc];
+#pragma unroll
+  for (int q = 0; q < kAdjMaxAcc; ++q) {
+    W[q][0] = W[q][1] = W[q][2] = 0.0f;
+    argstep[q] = 0;
+  }
+  double pv[1][1] = {{0.0}};
+  float v = 0.0f;
+  for (int g = 0; g < P.n_groups; ++g) {
+    float z[4 * M];
+#pragma unroll
+    for (int bk = 0; bk < M; ++bk) {
+      const uint4 q = philox4x32_10((uint32_t)(g * M + bk), p_lo, p_hi, 0u, P.seed_lo, P.seed_hi);
+      z[4 * bk + 0] = normal_from_word(q.x);
+      z[4 * bk + 1] = normal_from_word(q.y);
+      z[4 * bk + 2] = normal_from_word(q.z);
+      z[4 * bk + 3] = normal_from_word(q.w);
+    }
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      const int k = 4 * g + s;
+      if (k <= n) {
+        const float* zk = &z[s * M];
+        // d x_a / d sigma_a += (-sigma_a dt + sqrt(dt) w_a) log2(e); d x_a / d mu_a += dt log2(e)
+        const float dt = ld_f32(A.adj_tab, 2 * (size_t)k), sq = ld_f32(A.adj_tab, 2 * (size_t)k + 1);
+#pragma unroll
+        for (int a = 0; a < K; ++a) {
+          float w = 0.0f;
+#pragma unroll
+          for (int c = 0; c <= a; ++c) w = fmaf(K == 1 ? 1.0f : P.chol[a * K + c], zk[c], w);
+          js[a] = fmaf(fmaf(-A.sigma[a], dt, sq * w), kLog2ef, js[a]);
+          jm[a] = fmaf(dt, kLog2ef, jm[a]);
+        }
+        advance<MODEL, K, 1>(P, k, zk, x, v);
+        const uint32_t fm = ld_u32(A.feed, (size_t)k);
+        int before[kAdjMaxAcc];
+#pragma unroll
+        for (int q = 0; q < kAdjMaxAcc; ++q) before[q] = argstep[q];
+        run_point_ops<NS, K>(A, k, r, pv, x, live, fm, argstep);
+        if (fm) {
+#pragma unroll
+          for (int q = 0; q < kAdjMaxAcc; ++q) {
+            if ((fm >> q) & 1u) {
+              const int aq = A.acc_asset[q];
+              float xa = x[0][0], ja = js[0], jma = jm[0], s0a = P.s0[0];
+#pragma unroll
+              for (int c = 1; c < K; ++c)
+                if (aq == c) {
+                  xa = x[0][c];
+                  ja = js[c];
+                  jma = jm[c];
+                  s0a = P.s0[c];
+                }
+              const float S = s0a * exp2_fast(xa);
+              const float d0 = S * kLn2f * ja, d1 = S * kLn2f * jma, d2 = S / s0a;
+              if (A.acc_kind[q] == OP_AVG) {
+                W[q][0] += d0;
+                W[q][1] += d1;
+                W[q][2] += d2;
+              } else if (argstep[q] != before[q] || k == 0) {
+                W[q][0] = d0;
+                W[q][1] = d1;
+                W[q][2] = d2;
+              }
+            }
+          }
+        }
+      }
+    }
+  }
+  // payoff derivatives at the flow date
+  float S[K];
+#pragma unroll
+  for (int a = 0; a < K; ++a) S[a] = P.s0[a] * exp2_fast(x[0][a]);
+  float gr[NP];
+#pragma unroll
+  for (int i = 0; i < NP; ++i) gr[i] = 0.0f;
+  const uint64_t st = ld_u64(P.steps, (size_t)n);
+  const int begin = (int)(uint32_t)st, count = (int)(uint32_t)(st >> 32);
+  if (count > 0) {
+#pragma unroll
+    for (int a = 0; a < K; ++a) {
+      const float ds = dual_point<NS, K>(A, n, begin, count, S, r, a);
+      gr[a] += ds * S[a] / P.s0[a];
+      gr[K + a] += ds * S[a] * kLn2f * js[a];
+      gr[2 * K + a] += ds * S[a] * kLn2f * jm[a];
+    }
+    for (int q = 0; q < A.n_acc; ++q) {
+      const float ab = dual_point<NS, K>(A, n, begin, count, S, r, K + q);
+      float w0 = W[0][0], w1 = W[0][1], w2 = W[0][2];
+#pragma unroll
+      for (int c = 1; c < kAdjMaxAcc; ++c)
+        if (q == c) {
+          w0 = W[c][0];
+          w1 = W[c][1];
+          w2 = W[c][2];
+        }
+      const int aq = A.acc_asset[q];
+#pragma unroll
+      for (int a = 0; a < K; ++a)
+        if (aq == a) {
+          gr[a] += ab * w2;
+          gr[K + a] += ab * w0;
+          gr[2 * K + a] += ab * w1;
+        }
+    }
+  }
+  double vals[NOUT];
+  vals[0] = pv[0][0];
+#pragma unroll
+  for (int i = 0; i < NP; ++i) vals[1 + i] = (double)gr[i];
+  block_stats_out<NOUT>(vals, live, P.partials);
+}
+
+// One Heston Euler step of the variance only (the same expression as
+// advance<MK_HESTON>, so the recomputed variance is bit-identical).
+__device__ __forceinline__ float heston_v_step(const PathArgs& P, int k, float z1, float z2, float v) {
+  const float t3 = ld_f32(P.tab, (size_t)kHestonStride * k + 3);
+  const float t4 = ld_f32(P.tab, (size_t)kHestonStride * k + 4);
+  const float t5 = ld_f32(P.tab, (size_t)kHestonStride * k + 5);
+  const float vp = P.heston_full_truncation ? fmaxf(v, 0.0f) : v;
+  const float sv = __builtin_amdgcn_sqrtf(vp);
+  const float wv = fmaf(P.rho, z1, P.rho_perp * z2);
+  const float vn = fmaf(t3, vp, v) + fmaf(t5 * sv, wv, t4);
+  return P.heston_full_truncation ? vn : fmaxf(vn, 0.0f);
+}
+
+// Heston: forward with a checkpoint every SEG points, then the adjoint sweep
+// with per-segment recompute.  Outputs: price, d/d{s0, v0, kappa, theta,
+// sigma_v, rho}.  FEEDS: the payoff has running averages / extremes, so the
+// sweep needs the log-spot at every point (recomputed with the variance and
+// walked back by inverting the log step); without them only the last point
+// seeds d/dx, which then stays constant, and the sweep carries the variance
+// alone.
+template <int NS, int SEG, bool FEEDS>
+__global__ __launch_bounds__(kBlock) void adjoint_heston_kernel(const AdjArgs A) {
+  constexpr int NOUT = 7;
+  const PathArgs& P = A.p;
+  const uint64_t local = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+  const bool live = local < P.n_paths;
+  const uint64_t path = P.path_base + local;
+  const uint32_t p_lo = (uint32_t)path, p_hi = (uint32_t)(path >> 32);
+  const int n = P.n_grid - 1;
+  const size_t cs = A.ck_stride;
+  float r[1][NS];
+  init_registers<NS>(P, r);
+  float x[1][1] = {{0.0f}};
+  float v = P.v0;
+  int argstep[kAdjMaxAcc];
+#pragma unroll
+  for (int q = 0; q < kAdjMaxAcc; ++q) argstep[q] = 0;
+  double pv[1][1] = {{0.0}};
+  for (int g = 0; g < P.n_groups; ++g) {
+    float z[8];
+#pragma unroll
+    for (int bk = 0; bk < 2; ++bk) {
+      const uint4 q = philox4x32_10((uint32_t)(2 * g + bk), p_lo, p_hi, 0u, P.seed_lo, P.seed_hi);
+      z[4 * bk + 0] = normal_from_word(q.x);
+      z[4 * bk + 1] = normal_from_word(q.y);
+      z[4 * bk + 2] = normal_from_word(q.z);
+      z[4 * bk + 3] = normal_from_word(q.w);
+    }
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      const int k = 4 * g + s;
+      if (k <= n) {
+        if ((k & (SEG - 1)) == 0 && live) {
+          const size_t c = (size_t)(k / SEG);
+          if constexpr (FEEDS) A.ck[(2 * c) * cs + path] = x[0][0];
+          A.ck[(2 * c + 1) * cs + path] = v;
+        }
+        advance<MK_HESTON, 1, 1>(P, k, &z[2 * s], x, v);
+        run_point_ops<NS, 1>(A, k, r, pv, x, live, FEEDS ? ld_u32(A.feed, (size_t)k) : 0u, argstep);
+      }
+    }
+  }
+  // d(pv)/d(spot at n) and d(pv)/d(accumulator q)
+  const float s0 = P.s0[0];
+  const float S_n = s0 * exp2_fast(x[0][0]);
+  float ds_n = 0.0f;
+  float abar[kAdjMaxAcc];
+#pragma unroll
+  for (int q = 0; q < kAdjMaxAcc; ++q) abar[q] = 0.0f;
+  {
+    const float S[1] = {S_n};
+    const uint64_t st = ld_u64(P.steps, (size_t)n);
+    const int begin = (int)(uint32_t)st, count = (int)(uint32_t)(st >> 32);
+    if (count > 0) {
+      ds_n = dual_point<NS, 1>(A, n, begin, count, S, r, 0);
+      if constexpr (FEEDS) {
+#pragma unroll
+        for (int q = 0; q < kAdjMaxAcc; ++q)
+          if (q < A.n_acc) abar[q] = dual_point<NS, 1>(A, n, begin, count, S, r, 1 + q);
+      }
+    }
+  }
+  const bool ft = P.heston_full_truncation != 0;
+  const float kappa = A.kappa, theta = A.theta, rho = P.rho, rho_perp = P.rho_perp;
+  const float rho_ratio = rho_perp > 0.0f ? rho / rho_perp : 0.0f;
+  // without feeds the only spot seed is at n: d/dx is constant along the path
+  float xb = FEEDS ? 0.0f : ds_n * S_n * kLn2f;
+  float g_s0 = FEEDS ? 0.0f : ds_n * S_n / s0;
+  float vbar = 0.0f, g_k = 0.0f, g_th = 0.0f, g_sv = 0.0f, g_rho = 0.0f;
+  const int n_seg = n / SEG + 1;
+  for (int c = n_seg - 1; c >= 0; --c) {
+    // recompute segment c (points SEG*c .. SEG*c + SEG - 1) from its checkpoint
+    float xs[1][1] = {{FEEDS && live ? A.ck[(2 * (size_t)c) * cs + path] : 0.0f}};
+    float vs = live ? A.ck[(2 * (size_t)c + 1) * cs + path] : 0.0f;
+    float vb[SEG], z1s[SEG], z2s[SEG];
+#pragma unroll
+    for (int gq = 0; gq < SEG / 4; ++gq) {
+      const int g = (SEG / 4) * c + gq;
+      float z[8];
+#pragma unroll
+      for (int bk = 0; bk < 2; ++bk) {
+        const uint4 q = philox4x32_10((uint32_t)(2 * g + bk), p_lo, p_hi, 0u, P.seed_lo, P.seed_hi);
+        z[4 * bk + 0] = normal_from_word(q.x);
+        z[4 * bk + 1] = normal_from_word(q.y);
+        z[4 * bk + 2] = normal_from_word(q.z);
+        z[4 * bk + 3] = normal_from_word(q.w);
+      }
+#pragma unroll
+      for (int s = 0; s < 4; ++s) {
+        const int i = 4 * gq + s;
+        vb[i] = vs;
+        z1s[i] = z[2 * s];
+        z2s[i] = z[2 * s + 1];
+        if (4 * g + s <= n) {
+          if constexpr (FEEDS) advance<MK_HESTON, 1, 1>(P, 4 * g + s, &z[2 * s], xs, vs);
+          else vs = heston_v_step(P, 4 * g + s, z[2 * s], z[2 * s + 1], vs);
+        }
+      }
+    }
+    // reverse sweep through the segment; xs, vs hold the state after its last point
+    float vnext = vs;
+#pragma unroll
+    for (int i = SEG - 1; i >= 0; --i) {
+      const int p = SEG * c + i;
+      if (p <= n) {
+        if constexpr (FEEDS) {
+          float seed = p == n ? ds_n : 0.0f;
+          const uint32_t fm = ld_u32(A.feed, (size_t)p);
+          if (fm) {
+#pragma unroll
+            for (int q = 0; q < kAdjMaxAcc; ++q)
+              if ((fm >> q) & 1u) seed += (A.acc_kind[q] == OP_AVG || argstep[q] == p) ? abar[q] : 0.0f;
+          }
+          const float S = s0 * exp2_fast(xs[0][0]);
+          xb = fmaf(seed * S, kLn2f, xb);
+          g_s0 = fmaf(seed, S / s0, g_s0);
+        }
+        if (p >= 1) {
+          const float t1 = ld_f32(P.tab, (size_t)kHestonStride * p + 1);
+          const float t2 = ld_f32(P.tab, (size_t)kHestonStride * p + 2);
+          const float t3 = ld_f32(P.tab, (size_t)kHestonStride * p + 3);
+          const float t5 = ld_f32(P.tab, (size_t)kHestonStride * p + 5);
+          const float dt = ld_f32(A.adj_tab, 2 * (size_t)p), sq = ld_f32(A.adj_tab, 2 * (size_t)p + 1);
+          const float vprev = vb[i], z1 = z1s[i], z2 = z2s[i];
+          const float vp = ft ? fmaxf(vprev, 0.0f) : vprev;
+          const float sv = __builtin_amdgcn_sqrtf(vp);
+          const float w = fmaf(rho, z1, rho_perp * z2);
+          // v_new = floor(v + e0 vp + d1 + d2 sv w); x_new = x + c0 + c1 vp + c2 sv z1
+          const float vnb = (ft || vnext > 0.0f) ? vbar : 0.0f;
+          float vpb = fmaf(xb, t1, vnb * t3);
+          const float svb = fmaf(xb * t2, z1, vnb * t5 * w);
+          g_k = fmaf(vnb, dt * (theta - vp), g_k);
+          g_th = fmaf(vnb, kappa * dt, g_th);
+          g_sv = fmaf(vnb, sq * sv * w, g_sv);
+          g_rho = fmaf(vnb * t5 * sv, fmaf(-rho_ratio, z2, z1), g_rho);
+          vpb = sv > 0.0f ? fmaf(svb, 0.5f / sv, vpb) : vpb;
+          vbar = vnb + ((!ft || vprev > 0.0f) ? vpb : 0.0f);
+          if constexpr (FEEDS) {
+            const float t0 = ld_f32(P.tab, (size_t)kHestonStride * p + 0);
+            xs[0][0] = xs[0][0] - (fmaf(t1, vp, t0) + t2 * sv * z1);
+          }
+          vnext = vprev;
+        }
+      }
+    }
+  }
+  double vals[NOUT] = {pv[0][0], (double)g_s0, (double)vbar, (double)g_k, (double)g_th, (double)g_sv, (double)g_rho};
+  block_stats_out<NOUT>(vals, live, P.partials);
+}
+
+int adjoint_outputs(int model, int k_assets) { return model == MK_HESTON ? 7 : 1 + 3 * k_assets; }
+
+bool adjoint_variant_exists(int model, int k_assets, int n_slots, int* ns_chosen) {
+  if (n_slots > 32) return false;
+  if (model == MK_BASKET && (k_assets < 2 || k_assets > kMaxAssets)) return false;
+  *ns_chosen = (n_slots <= 16 && model != MK_BASKET) ? 16 : 32;
+  return true;
+}
+
+hipError_t launch_adjoint(int model, int k, int ns, const AdjArgs& a, int n_blocks, hipStream_t s) {
+#define KT_ADJ(KERN) hipLaunchKernelGGL((KERN), dim3(n_blocks), dim3(kBlock), 0, s, a)
+  if (model == MK_GBM) {
+    if (ns == 16) KT_ADJ((adjoint_gbm_kernel<1, 16>));
+    else KT_ADJ((adjoint_gbm_kernel<1, 32>));
+  } else if (model == MK_HESTON) {
+    const bool feeds = a.n_acc > 0;
+    if (a.seg != 16 && a.seg != 32) return hipErrorInvalidValue;
+    const int v = (ns == 16 ? 0 : 4) + (a.seg == 16 ? 0 : 2) + (feeds ? 1 : 0);
+    switch (v) {
+      case 0: KT_ADJ((adjoint_heston_kernel<16, 16, false>)); break;
+      case 1: KT_ADJ((adjoint_heston_kernel<16, 16, true>)); break;
+      case 2: KT_ADJ((adjoint_heston_kernel<16, 32, false>)); break;
+      case 3: KT_ADJ((adjoint_heston_kernel<16, 32, true>)); break;
+      case 4: KT_ADJ((adjoint_heston_kernel<32, 16, false>)); break;
+      case 5: KT_ADJ((adjoint_heston_kernel<32, 16, true>)); break;
+      case 6: KT_ADJ((adjoint_heston_kernel<32, 32, false>)); break;
+      default: KT_ADJ((adjoint_heston_kernel<32, 32, true>)); break;
+    }
+  } else {
+    switch (k) {
+      case 2: KT_ADJ((adjoint_gbm_kernel<2, 32>)); break;
+      case 3: KT_ADJ((adjoint_gbm_kernel<3, 32>)); break;
+      case 4: KT_ADJ((adjoint_gbm_kernel<4, 32>)); break;
+      case 5: KT_ADJ((adjoint_gbm_kernel<5, 32>)); break;
+      case 6: KT_ADJ((adjoint_gbm_kernel<6, 32>)); break;
+      case 7: KT_ADJ((adjoint_gbm_kernel<7, 32>)); break;
+      case 8: KT_ADJ((adjoint_gbm_kernel<8, 32>)); break;
+      default: return hipErrorInvalidValue;
+    }
+  }
+#undef KT_ADJ
+  return hipGetLastError();
+}
+
+}  // namespace kt

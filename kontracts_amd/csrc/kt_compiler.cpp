@@ -1126,6 +1126,86 @@ Program lower_exercise(const Con& contract, const std::vector<double>& grid,
   return p;
 }
 
+AdjPlan classify_adjoint(const Program& prog) {
+  AdjPlan plan;
+  const size_t n_pts = prog.steps.size();
+  if (n_pts == 0) raise(KT_ERR_UNSUPPORTED, "pathwise Greeks: empty grid");
+  const size_t last = n_pts - 1;
+  plan.feed.assign(n_pts, 0u);
+  auto unsupported = [](const std::string& why) {
+    raise(KT_ERR_UNSUPPORTED, "pathwise Greeks need a Lipschitz payoff paid on one date: " + why +
+                                  " (use bump Greeks)");
+  };
+  std::vector<bool> is_init(kMaxSlots, false);
+  for (const KOp& op : prog.init) is_init[op.d] = true;
+  is_init[kSlotZero] = is_init[kSlotOne] = true;
+  std::vector<int> acc_of(kMaxSlots, -1);  // slot -> accumulator (AVG also owns slot d + 1)
+  auto reads = [](const KOp& op, int* r) {
+    switch (op.code) {
+      case OP_NOP: case OP_CONST: case OP_SPOT: return 0;
+      case OP_NEG: case OP_ADDI: case OP_SUBI: case OP_RSUBI: case OP_MULI: case OP_DIVI: case OP_RDIVI:
+      case OP_MAXI: case OP_MINI: case OP_AVG: case OP_RMAX: case OP_RMIN:
+        r[0] = op.a;
+        return 1;
+      default:
+        r[0] = op.a;
+        r[1] = op.b;
+        return 2;
+    }
+  };
+  for (size_t k = 0; k < n_pts; ++k) {
+    const KStep& st = prog.steps[k];
+    std::vector<int> spot_of(kMaxSlots, -1);  // slot -> asset, if last written by SPOT at this point
+    std::vector<bool> written(kMaxSlots, false);
+    for (int32_t i = 0; i < st.count; ++i) {
+      const KOp& op = prog.ops[(size_t)st.begin + i];
+      int rd[2];
+      const int nr = reads(op, rd);
+      switch (op.code) {
+        case OP_NOP: case OP_CONST: case OP_SPOT: case OP_NEG: case OP_ADD: case OP_SUB: case OP_MUL:
+        case OP_DIV: case OP_MAX: case OP_MIN: case OP_ADDI: case OP_SUBI: case OP_RSUBI: case OP_MULI:
+        case OP_DIVI: case OP_RDIVI: case OP_MAXI: case OP_MINI:
+          for (int j = 0; j < nr; ++j) {
+            if (acc_of[rd[j]] >= 0 && k != last) unsupported("a running average/extreme read before the payment date");
+            if (k == last && !written[rd[j]] && !is_init[rd[j]] && acc_of[rd[j]] < 0)
+              unsupported("a value carried between dates");
+          }
+          break;
+        case OP_AVG: case OP_RMAX: case OP_RMIN: {
+          if (spot_of[op.a] < 0) unsupported("a running average/extreme of something other than a spot");
+          int q = acc_of[op.d];
+          if (q < 0) {
+            if (plan.n_acc == 4) unsupported("more than 4 running averages/extremes");
+            q = plan.n_acc++;
+            plan.slot[q] = op.d;
+            plan.kind[q] = op.code;
+            plan.asset[q] = spot_of[op.a];
+            acc_of[op.d] = q;
+            if (op.code == OP_AVG) acc_of[(op.d + 1) % kMaxSlots] = q;
+          } else if (plan.kind[q] != op.code || plan.asset[q] != spot_of[op.a] || plan.slot[q] != op.d) {
+            unsupported("a register shared by different running averages/extremes");
+          }
+          plan.feed[k] |= 1u << q;
+          break;
+        }
+        case OP_EMIT:
+          if (k != last) unsupported("flows on more than one date");
+          if (op.b != kSlotOne) unsupported("a conditional flow");
+          if (!written[rd[0]] && !is_init[rd[0]] && acc_of[rd[0]] < 0) unsupported("a value carried between dates");
+          break;
+        default:
+          unsupported("a price-dependent condition, digital or knock-out");
+      }
+      if (op.code != OP_EMIT && op.code != OP_NOP) {
+        written[op.d] = true;
+        spot_of[op.d] = op.code == OP_SPOT ? (int)op.a : -1;
+        if (op.code == OP_AVG) written[(op.d + 1) % kMaxSlots] = true;
+      }
+    }
+  }
+  return plan;
+}
+
 std::string Program::describe() const {
   std::ostringstream os;
   os << "slots " << n_slots << ", outputs " << n_outputs << ", ops " << ops.size() << ", max/step "

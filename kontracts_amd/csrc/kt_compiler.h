@@ -86,6 +86,19 @@ Program lower_exercise(const Con& contract, const std::vector<double>& grid,
 // Classify groups of 4 grid points (fills prog.groups).
 void classify_groups(Program& prog);
 
+// Pathwise-Greek plan of a program: every flow on the last grid point and
+// paid unconditionally, only arithmetic ops, and each running sum / extreme
+// fed directly by one asset's spot and read only on the last point.  Raises
+// Unsupported (with the reason) otherwise: such contracts need bump Greeks.
+struct AdjPlan {
+  int n_acc = 0;
+  int slot[4] = {0, 0, 0, 0};
+  int kind[4] = {0, 0, 0, 0};
+  int asset[4] = {0, 0, 0, 0};
+  std::vector<uint32_t> feed;  // per grid point: bit q set if accumulator q is fed there
+};
+AdjPlan classify_adjoint(const Program& prog);
+
 // For LSM: true iff every flow of the contract happens at its acquisition
 // step (no When/Anytime/Until/Or anywhere in the tree).
 bool flows_at_acquisition(const Con& c);

@@ -259,7 +259,7 @@ void fill_model_args(PathArgs& A, const ModelDesc& md, uint64_t seed) {
 std::vector<KStats> run_program_scen(kt_engine* E, const Program& prog, const std::vector<ModelDesc>& models,
                                      const std::vector<double>& rates, const std::vector<std::vector<double>>& stat_w,
                                      const std::vector<double>& grid, uint64_t n_paths, uint64_t seed,
-                                     double* pv_host, float* z_host) {
+                                     double* pv_host, float* z_host, double* pv_keep) {
   const int SC = (int)models.size();
   const ModelDesc& md = models[0];
   int stride = 0;
@@ -311,7 +311,7 @@ std::vector<KStats> run_program_scen(kt_engine* E, const Program& prog, const st
 
   int kind = model_kernel_kind(md);
   int K = (int)md.names.size();
-  bool debug = pv_host || z_host;
+  bool debug = pv_host || z_host || pv_keep;
   int ns = 0, nout = 0;
   if (!path_variant_exists(kind, K, prog.n_slots, prog.n_outputs, debug, SC, &ns, &nout))
     raise(KT_ERR_UNSUPPORTED, "no path kernel variant for this program shape");
@@ -339,7 +339,8 @@ std::vector<KStats> run_program_scen(kt_engine* E, const Program& prog, const st
   double* pv_dev = nullptr;
   float* z_dev = nullptr;
   int M = model_normals(md);
-  if (pv_host) pv_dev = (double*)E->ensure(E->pv, std::max<uint64_t>(n_paths, 1) * sizeof(double));
+  if (pv_keep) pv_dev = pv_keep;
+  else if (pv_host) pv_dev = (double*)E->ensure(E->pv, std::max<uint64_t>(n_paths, 1) * sizeof(double));
   if (z_host) z_dev = (float*)E->ensure(E->z, std::max<uint64_t>(n_paths * grid.size() * M, 1) * sizeof(float));
 
   PathArgs A{};
@@ -380,7 +381,7 @@ std::vector<KStats> run_program_scen(kt_engine* E, const Program& prog, const st
   check_hip(hipMemcpyAsync(out.data(), stats, n_used * sizeof(KStats), hipMemcpyDeviceToHost, E->stream),
             "copy stats");
   check_hip(hipMemcpyAsync(&err_bits, err, sizeof(unsigned), hipMemcpyDeviceToHost, E->stream), "copy err");
-  if (pv_host)
+  if (pv_host && pv_dev)
     check_hip(hipMemcpyAsync(pv_host, pv_dev, n_paths * sizeof(double), hipMemcpyDeviceToHost, E->stream), "copy pv");
   if (z_host)
     check_hip(hipMemcpyAsync(z_host, z_dev, n_paths * grid.size() * M * sizeof(float), hipMemcpyDeviceToHost,
@@ -396,8 +397,8 @@ std::vector<KStats> run_program_scen(kt_engine* E, const Program& prog, const st
 
 std::vector<KStats> run_program(kt_engine* E, const Program& prog, const ModelDesc& md,
                                 const std::vector<double>& grid, double rate, uint64_t n_paths, uint64_t seed,
-                                double* pv_host, float* z_host) {
-  return run_program_scen(E, prog, {md}, {rate}, {}, grid, n_paths, seed, pv_host, z_host);
+                                double* pv_host, float* z_host, double* pv_keep) {
+  return run_program_scen(E, prog, {md}, {rate}, {}, grid, n_paths, seed, pv_host, z_host, pv_keep);
 }
 
 // Shared front half of price_gbm: parse, plan, grid, validation.
@@ -616,6 +617,21 @@ kt_status kt_debug_american(kt_engine* E, const char* json, const double* dates,
   if (!E || !out || !cfg || !pv_out) raise(KT_ERR_INVALID_ARGUMENT, "null argument");
   check_hip(hipSetDevice(E->device), "hipSetDevice");
   price_american(E, json, dates, n_dates, model, cfg, n_basis, out, pv_out, coef_out);
+  E->err.clear();
+  return KT_OK;
+  KT_GUARD_END(E)
+}
+
+kt_status kt_engine_set_option(kt_engine* E, const char* name, int64_t value) {
+  KT_GUARD_BEGIN
+  if (!E || !name) raise(KT_ERR_INVALID_ARGUMENT, "null argument");
+  const std::string key(name);
+  if (key == "adjoint_segment") {
+    if (value != 16 && value != 32) raise(KT_ERR_INVALID_ARGUMENT, "adjoint_segment must be 16 or 32");
+    E->adj_seg = (int)value;
+  } else {
+    raise(KT_ERR_INVALID_ARGUMENT, "unknown engine option: " + key);
+  }
   E->err.clear();
   return KT_OK;
   KT_GUARD_END(E)

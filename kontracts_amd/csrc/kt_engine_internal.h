@@ -27,6 +27,7 @@ struct kt_engine {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   double prof_ms = 0.0, prof_path_steps = 0.0;
   uint64_t prof_launches = 0;
+  int adj_seg = kt::kAdjSeg;  // Heston adjoint checkpoint interval (kt_engine_set_option)
   // Bracket a group of launches with events when profiling is enabled.
   void prof_begin();
   void prof_end(double path_steps);
@@ -66,9 +67,11 @@ int lowest_error_code(unsigned bits);
 const char* error_text(int code);
 
 PreparedContract prepare(const char* json, const ModelDesc& md, const kt_mc_config* cfg);
+// pv_keep (device, n_paths doubles, nullable): receives each path's pv and
+// stays on the device.
 std::vector<KStats> run_program(kt_engine* E, const Program& prog, const ModelDesc& md,
                                 const std::vector<double>& grid, double rate, uint64_t n_paths, uint64_t seed,
-                                double* pv_host, float* z_host);
+                                double* pv_host, float* z_host, double* pv_keep = nullptr);
 
 // Model constants of the path kernels (spots, Cholesky factor, Heston
 // correlation split) and the Philox key.
@@ -77,7 +80,7 @@ void fill_model_args(PathArgs& A, const ModelDesc& md, uint64_t seed);
 std::vector<KStats> run_program_scen(kt_engine* E, const Program& prog, const std::vector<ModelDesc>& models,
                                      const std::vector<double>& rates, const std::vector<std::vector<double>>& stat_w,
                                      const std::vector<double>& grid, uint64_t n_paths, uint64_t seed,
-                                     double* pv_host, float* z_host);
+                                     double* pv_host, float* z_host, double* pv_keep = nullptr);
 
 // Implemented in kt_greeks.cpp / kt_lsm.cpp / kt_portfolio.cpp.
 void greeks_gbm_crn(kt_engine* E, const char* json, const char* asset, double s0, double sigma,

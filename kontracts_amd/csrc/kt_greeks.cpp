@@ -1,24 +1,13 @@
 // kt_greeks.cpp — Greeks by CRN bump-and-reprice (src/greeks.rs:60-98).
+#include <algorithm>
 #include <cmath>
 #include <limits>
+#include <string>
 
 #include "kt_engine_internal.h"
 
 namespace kt {
 
-namespace {
-
-double price_once(kt_engine* E, const Con& con, const ModelDesc& md, const std::vector<double>& grid, double rate,
-                  const kt_mc_config* cfg, double* se) {
-  validate_model(md);
-  Program prog = lower({&con}, grid, md.names);
-  auto st = run_program(E, prog, md, grid, rate, cfg->n_paths, cfg->seed, nullptr, nullptr);
-  kt_price_result r = result_from_stats(st[0]);
-  if (se) *se = r.std_error;
-  return r.price;
-}
-
-}  // namespace
 
 // greeks_gbm: drift = rate, seven repricings at the same seed.  The seven
 // scenarios run in one launch that shares every normal (common random
@@ -91,6 +80,8 @@ void greeks_gbm_crn(kt_engine* E, const char* json, const char* asset, double s0
 }
 
 // Central differences over the model parameters with common random numbers.
+// With stderr_out, each bumped pair keeps its per-path present values on the
+// device and the standard error of the per-path differences is reduced there.
 void greeks_bump(kt_engine* E, const char* json, const kt_model* model, const kt_mc_config* cfg, const double* h,
                  uint64_t n_params, double* out, double* stderr_out) {
   ModelDesc base = model_from_abi(model);
@@ -98,8 +89,13 @@ void greeks_bump(kt_engine* E, const char* json, const kt_model* model, const kt
   uint64_t want = base.kind == KT_MODEL_HESTON ? 6 : (base.kind == KT_MODEL_GBM ? 3 : 0);
   if (want == 0) raise(KT_ERR_UNSUPPORTED, "bump Greeks are defined for GBM and Heston models");
   if (n_params != want) raise(KT_ERR_INVALID_ARGUMENT, "wrong number of bump sizes for this model");
-  for (uint64_t i = 0; i <= n_params; ++i) out[i] = 0.0;
-  if (cfg->n_paths == 0) return;
+  if (!h) raise(KT_ERR_INVALID_ARGUMENT, "null bump sizes");
+  for (uint64_t i = 0; i <= n_params; ++i) {
+    out[i] = 0.0;
+    if (stderr_out) stderr_out[i] = 0.0;
+  }
+  const uint64_t N = cfg->n_paths;
+  if (N == 0) return;
   auto bumped = [&](int which, double delta) {
     ModelDesc m = base;
     if (base.kind == KT_MODEL_GBM) {
@@ -114,23 +110,156 @@ void greeks_bump(kt_engine* E, const char* json, const kt_model* model, const kt
       if (which == 4) m.sigma_v += delta;
       if (which == 5) m.rho += delta;
     }
+    validate_model(m);
     return m;
   };
-  double se0 = 0.0;
-  out[0] = price_once(E, *pc.con, base, pc.grid, cfg->rate, cfg, &se0);
-  for (uint64_t j = 0; j < n_params; ++j) {
-    double up = price_once(E, *pc.con, bumped((int)j, h[j]), pc.grid, cfg->rate, cfg, nullptr);
-    double dn = price_once(E, *pc.con, bumped((int)j, -h[j]), pc.grid, cfg->rate, cfg, nullptr);
-    out[1 + j] = (up - dn) / (2.0 * h[j]);
-  }
+  Program prog = lower({pc.con.get()}, pc.grid, base.names);
+  auto st0 = run_program(E, prog, base, pc.grid, cfg->rate, N, cfg->seed, nullptr, nullptr);
+  kt_price_result r0 = result_from_stats(st0[0]);
+  out[0] = r0.price;
+  if (stderr_out) stderr_out[0] = r0.std_error;
+  double* up_dev = nullptr;
+  double* dn_dev = nullptr;
+  KStats* diff_partials = nullptr;
+  KStats* diff_stats = nullptr;
   if (stderr_out) {
-    stderr_out[0] = se0;
-    for (uint64_t j = 0; j < n_params; ++j) stderr_out[1 + j] = std::numeric_limits<double>::quiet_NaN();
+    up_dev = (double*)E->ensure(E->pv, N * sizeof(double));
+    dn_dev = (double*)E->ensure(E->z, N * sizeof(double));
+    diff_partials = (KStats*)E->ensure(E->scratch, ((N + 255) / 256) * sizeof(KStats));
+    diff_stats = (KStats*)E->ensure(E->stats, kAdjMaxOut * sizeof(KStats));
+  }
+  for (uint64_t j = 0; j < n_params; ++j) {
+    auto su = run_program(E, prog, bumped((int)j, h[j]), pc.grid, cfg->rate, N, cfg->seed, nullptr, nullptr, up_dev);
+    auto sd = run_program(E, prog, bumped((int)j, -h[j]), pc.grid, cfg->rate, N, cfg->seed, nullptr, nullptr, dn_dev);
+    out[1 + j] = (result_from_stats(su[0]).price - result_from_stats(sd[0]).price) / (2.0 * h[j]);
+    if (stderr_out) {
+      const int nb = (int)((N + 255) / 256);
+      check_hip(launch_diff_stats(up_dev, dn_dev, 1.0 / (2.0 * h[j]), N, diff_partials, E->stream), "diff stats");
+      check_hip(launch_finalize(diff_partials, nb, 1, 1, diff_stats, E->stream), "finalize launch");
+      KStats ds{};
+      check_hip(hipMemcpyAsync(&ds, diff_stats, sizeof(KStats), hipMemcpyDeviceToHost, E->stream), "copy stats");
+      check_hip(hipStreamSynchronize(E->stream), "stream sync");
+      stderr_out[1 + j] = result_from_stats(ds).std_error;
+    }
   }
 }
 
-void greeks_adjoint(kt_engine*, const char*, const kt_model*, const kt_mc_config*, double*, double*, uint64_t) {
-  raise(KT_ERR_UNSUPPORTED, "pathwise Greeks: not built yet");
+// Pathwise Greeks (kt_adjoint.hip): one launch; per-path samples of the
+// price and of each parameter derivative, reduced in FP64 like prices.
+void greeks_adjoint(kt_engine* E, const char* json, const kt_model* model, const kt_mc_config* cfg, double* out,
+                    double* stderr_out, uint64_t n_out) {
+  ModelDesc md = model_from_abi(model);
+  PreparedContract pc = prepare(json, md, cfg);
+  const int kind = model_kernel_kind(md);
+  const int K = (int)md.names.size();
+  const int need = adjoint_outputs(kind, K);
+  if (n_out < (uint64_t)need)
+    raise(KT_ERR_INVALID_ARGUMENT, "output arrays need " + std::to_string(need) + " entries for this model");
+  const std::vector<double>& grid = pc.grid;
+  Program prog = lower({pc.con.get()}, grid, md.names);
+  bool flags = false;
+  for (const KOp& op : prog.ops) flags = flags || op.code == OP_FLAGERR;
+  if (flags) {
+    // surface the path error the pricer raises (unknown asset, `or` reached, ...)
+    run_program(E, prog, md, grid, cfg->rate, cfg->n_paths, cfg->seed, nullptr, nullptr);
+  }
+  AdjPlan plan = classify_adjoint(prog);
+  int ns = 0;
+  if (!adjoint_variant_exists(kind, K, prog.n_slots, &ns))
+    raise(KT_ERR_UNSUPPORTED, "pathwise Greeks: program needs more registers than the kernel holds");
+  for (int j = 0; j < need; ++j) {
+    out[j] = 0.0;
+    if (stderr_out) stderr_out[j] = 0.0;
+  }
+  const uint64_t N = cfg->n_paths;
+  if (N == 0) return;
+
+  int stride = 0;
+  std::vector<float> tab = model_table(md, grid, &stride);
+  std::vector<double> df(grid.size());
+  for (size_t k = 0; k < grid.size(); ++k) df[k] = std::exp(-cfg->rate * grid[k]);
+  std::vector<float> adj(2 * grid.size(), 0.0f);
+  for (size_t k = 1; k < grid.size(); ++k) {
+    const double dt = grid[k] - grid[k - 1];
+    if (dt > 0.0) {
+      adj[2 * k] = (float)dt;
+      adj[2 * k + 1] = (float)std::sqrt(dt);
+    }
+  }
+  std::vector<KOp> ops = prog.ops;
+  if (ops.empty()) ops.push_back(KOp{OP_NOP, 0, 0, 0, 0.0f});
+  std::vector<float> thr(1, 0.0f);
+  auto dev = E->upload({{ops.data(), ops.size() * sizeof(KOp)},
+                        {prog.steps.data(), prog.steps.size() * sizeof(KStep)},
+                        {prog.groups.data(), prog.groups.size() * sizeof(KGroup)},
+                        {prog.init.data(), prog.init.size() * sizeof(KOp)},
+                        {df.data(), df.size() * sizeof(double)},
+                        {tab.data(), tab.size() * sizeof(float)},
+                        {thr.data(), thr.size() * sizeof(float)},
+                        {adj.data(), adj.size() * sizeof(float)},
+                        {plan.feed.data(), plan.feed.size() * sizeof(uint32_t)}});
+
+  const uint64_t total_blocks = (N + kBlock - 1) / kBlock;
+  if (total_blocks > (1ull << 31)) raise(KT_ERR_INVALID_ARGUMENT, "too many paths");
+  KStats* partials = (KStats*)E->ensure(E->partials, total_blocks * need * sizeof(KStats));
+  KStats* stats = (KStats*)E->ensure(E->stats, kAdjMaxOut * sizeof(KStats));
+  float* ck = nullptr;
+  const int seg = E->adj_seg;
+  const size_t n_ck = (grid.size() - 1) / seg + 1;
+  if (kind == MK_HESTON) ck = (float*)E->ensure(E->aux, n_ck * 2 * N * sizeof(float));
+
+  AdjArgs A{};
+  PathArgs& P = A.p;
+  P.ops = (const KOp*)dev[0];
+  P.steps = (const KStep*)dev[1];
+  P.groups = (const KGroup*)dev[2];
+  P.init = (const KOp*)dev[3];
+  P.df = (const double*)dev[4];
+  P.tab = (const float*)dev[5];
+  P.thr = (const float*)dev[6];
+  P.n_init = (int)prog.init.size();
+  P.n_grid = (int)grid.size();
+  P.n_groups = (int)((grid.size() + 3) / 4);
+  P.tab_stride = stride;
+  fill_model_args(P, md, cfg->seed);
+  A.adj_tab = (const float*)dev[7];
+  A.feed = (const uint32_t*)dev[8];
+  A.ck = ck;
+  A.ck_stride = N;
+  A.n_acc = plan.n_acc;
+  for (int q = 0; q < 4; ++q) {
+    A.acc_slot[q] = plan.slot[q];
+    A.acc_kind[q] = plan.kind[q];
+    A.acc_asset[q] = plan.asset[q];
+  }
+  for (int a = 0; a < K && a < kMaxAssets; ++a) {
+    A.sigma[a] = (float)md.sigma[a];
+    A.mu[a] = (float)md.mu[a];
+  }
+  A.kappa = (float)md.kappa;
+  A.theta = (float)md.theta;
+  A.sigma_v = (float)md.sigma_v;
+  A.seg = seg;
+
+  E->prof_begin();
+  const uint64_t kMaxBlocksPerLaunch = 1ull << 20;
+  for (uint64_t b0 = 0; b0 < total_blocks; b0 += kMaxBlocksPerLaunch) {
+    const uint64_t nb = std::min(kMaxBlocksPerLaunch, total_blocks - b0);
+    P.path_base = b0 * kBlock;
+    P.n_paths = std::min<uint64_t>(nb * kBlock, N - b0 * kBlock);
+    P.partials = partials + b0 * need;
+    check_hip(launch_adjoint(kind, K, ns, A, (int)nb, E->stream), "adjoint kernel launch");
+  }
+  E->prof_end((double)N * (double)(grid.size() - 1));
+  check_hip(launch_finalize(partials, (int)total_blocks, need, need, stats, E->stream), "finalize launch");
+  std::vector<KStats> st(need);
+  check_hip(hipMemcpyAsync(st.data(), stats, need * sizeof(KStats), hipMemcpyDeviceToHost, E->stream), "copy stats");
+  check_hip(hipStreamSynchronize(E->stream), "stream sync");
+  for (int j = 0; j < need; ++j) {
+    kt_price_result r = result_from_stats(st[j]);
+    out[j] = r.price;
+    if (stderr_out) stderr_out[j] = r.std_error;
+  }
 }
 
 void price_portfolio(kt_engine*, const kt_job*, uint64_t, kt_price_result*) {

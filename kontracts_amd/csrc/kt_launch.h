@@ -60,6 +60,35 @@ bool path_variant_exists(int model, int k_assets, int n_slots, int n_outputs, bo
                          int* ns_chosen, int* nout_chosen);
 hipError_t launch_path(int model, int k_assets, int ns, int nout, bool debug, int n_scen, const PathArgs& a,
                        int n_blocks, hipStream_t s);
+// Pathwise Greeks.  Supported programs: one flow date (the last grid point),
+// arithmetic ops, and running sums / extremes fed directly by a spot (see
+// classify_adjoint).  Outputs per path: price, then d/d params:
+//   GBM {s0, sigma, mu}; Heston {s0, v0, kappa, theta, sigma_v, rho};
+//   basket {s0_0..s0_{K-1}, sigma_0.., mu_0..}.
+// Heston checkpoint interval (grid points).  16 by default: at 32 the
+// recomputed segment (variance and two normals per point in registers) pushes
+// the kernel past 128 VGPRs and halves its occupancy; 32 is also built.
+constexpr int kAdjSeg = 16;
+constexpr int kAdjMaxAcc = 4;
+constexpr int kAdjMaxOut = 1 + 3 * kMaxAssets;
+struct AdjArgs {
+  PathArgs p;               // program, model tables, RNG (n_grid points, last = flow date)
+  const float* adj_tab;     // [n_grid][2] {dt, sqrt(dt)} (zero at point 0)
+  const uint32_t* feed;     // [n_grid] bit q: accumulator q is fed at this point
+  float* ck;                // Heston: [n_ck][2][ck_stride] {log2(S/s0), v} before point seg*c
+  uint64_t ck_stride;       // >= total paths (checkpoints are indexed by global path)
+  int n_acc;
+  int acc_slot[kAdjMaxAcc];   // register of the accumulator
+  int acc_kind[kAdjMaxAcc];   // OP_AVG, OP_RMAX or OP_RMIN
+  int acc_asset[kAdjMaxAcc];  // asset whose spot feeds it
+  float sigma[kMaxAssets], mu[kMaxAssets];
+  float kappa, theta, sigma_v;
+  int seg;  // Heston checkpoint interval: 16 or 32 points
+};
+int adjoint_outputs(int model, int k_assets);
+bool adjoint_variant_exists(int model, int k_assets, int n_slots, int* ns_chosen);
+hipError_t launch_adjoint(int model, int k_assets, int ns, const AdjArgs& a, int n_blocks, hipStream_t s);
+
 // Longstaff-Schwartz kernels.
 constexpr int kLsmBasisMax = 6;
 constexpr int kLsmRegCount = (2 * kLsmBasisMax - 1) + kLsmBasisMax + 1;
@@ -82,6 +111,9 @@ hipError_t launch_lsm_pass(const LsmPassArgs& p, int n_blocks, hipStream_t s);
 // One block: reduce reg and write coef[0..kLsmBasisMax] (coefficients, fitted flag).
 hipError_t launch_lsm_solve(const double* reg, int n_blocks, int m, double* coef, hipStream_t s);
 
+// partials[block] = statistics of (a[i] - b[i]) * scale over the block's paths.
+hipError_t launch_diff_stats(const double* a, const double* b, double scale, uint64_t n, KStats* partials,
+                             hipStream_t s);
 hipError_t launch_finalize(const KStats* partials, int n_blocks, int nout, int n_used, KStats* out,
                            hipStream_t s);
 hipError_t launch_debug_philox(const uint32_t* ctr, uint32_t k0, uint32_t k1, uint64_t n, uint32_t* out,

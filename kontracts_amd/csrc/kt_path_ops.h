@@ -1,0 +1,169 @@
+// kt_path_ops.h — device building blocks shared by the gfx950 kernels:
+// constant-address-space table loads, the op-program interpreter and the
+// per-grid-point model advance.  Included by each kernel translation unit.
+#pragma once
+#include "kt_device.h"
+#include "kt_launch.h"
+
+#define KT_CONST __attribute__((address_space(4)))
+
+namespace kt {
+
+template <class T>
+__device__ __forceinline__ const KT_CONST T* cst(const T* p) {
+  return (const KT_CONST T*)(p);
+}
+
+// Scalar-typed loads through the constant address space (wave-uniform index).
+__device__ __forceinline__ uint32_t ld_u32(const void* p, size_t i) { return cst((const uint32_t*)p)[i]; }
+__device__ __forceinline__ uint64_t ld_u64(const void* p, size_t i) { return cst((const uint64_t*)p)[i]; }
+__device__ __forceinline__ float ld_f32(const float* p, size_t i) { return cst(p)[i]; }
+__device__ __forceinline__ double ld_f64(const double* p, size_t i) { return cst(p)[i]; }
+
+// Wave-uniform selection of a per-asset kernel argument.
+template <int K>
+__device__ __forceinline__ float pick_asset(const float (&arr)[kMaxAssets], uint32_t i) {
+  float v = arr[0];
+#pragma unroll
+  for (int c = 1; c < K; ++c)
+    if (i == (uint32_t)c) v = arr[c];
+  return v;
+}
+
+template <int K, int SC>
+__device__ __forceinline__ float scen_s0(const PathArgs& A, int sc, uint32_t asset) {
+  if constexpr (SC > 1) return A.s0_scen[sc];
+  return pick_asset<K>(A.s0, asset);
+}
+
+#define FOR_SC _Pragma("unroll") for (int sc = 0; sc < SC; ++sc)
+
+// In the greeks_gbm scenario layout (base, S+h, S-h, sigma+-h, r+-h) the two
+// spot-bumped scenarios differ from the base only by s0: they share its
+// log-spot path log2(S/s0).  xsrc(sc) is the scenario whose log-spot sc reads.
+template <int SC>
+__device__ constexpr int xsrc(int sc) {
+  return (SC == kScenarios && (sc == 1 || sc == 2)) ? 0 : sc;
+}
+
+template <int NS, int NOUT, int K, int SC>
+__device__ __forceinline__ void run_ops(const PathArgs& A, int k, int begin, int count, float (&r)[SC][NS],
+                                        double (&pv)[SC][NOUT], const float (&x)[SC][K], bool live) {
+  for (int i = 0; i < count; ++i) {
+    const uint64_t word = ld_u64(A.ops, (size_t)begin + i);
+    const uint2 w = make_uint2((uint32_t)word, (uint32_t)(word >> 32));
+    const uint32_t code = w.x & 0xffu;
+    const uint32_t d = (w.x >> 8) & (NS - 1);
+    const uint32_t a = (w.x >> 16) & (NS - 1);
+    const uint32_t b = (w.x >> 24) & (NS - 1);
+    const float imm = __uint_as_float(w.y);
+    switch (code) {
+      case OP_CONST: FOR_SC r[sc][d] = imm; break;
+      case OP_SPOT: {
+        const uint32_t asset = (w.x >> 16) & 0xffu;
+        const uint32_t ai = asset < (uint32_t)K ? asset : 0u;
+        FOR_SC r[sc][d] = scen_s0<K, SC>(A, sc, ai) * exp2_fast(x[xsrc<SC>(sc)][ai]);
+        break;
+      }
+      case OP_NEG: FOR_SC r[sc][d] = -r[sc][a]; break;
+      case OP_ADD: FOR_SC r[sc][d] = r[sc][a] + r[sc][b]; break;
+      case OP_SUB: FOR_SC r[sc][d] = r[sc][a] - r[sc][b]; break;
+      case OP_MUL: FOR_SC r[sc][d] = r[sc][a] * r[sc][b]; break;
+      case OP_DIV: FOR_SC r[sc][d] = r[sc][a] / r[sc][b]; break;
+      case OP_MAX: FOR_SC r[sc][d] = fmaxf(r[sc][a], r[sc][b]); break;
+      case OP_MIN: FOR_SC r[sc][d] = fminf(r[sc][a], r[sc][b]); break;
+      case OP_ADDI: FOR_SC r[sc][d] = r[sc][a] + imm; break;
+      case OP_SUBI: FOR_SC r[sc][d] = r[sc][a] - imm; break;
+      case OP_RSUBI: FOR_SC r[sc][d] = imm - r[sc][a]; break;
+      case OP_MULI: FOR_SC r[sc][d] = r[sc][a] * imm; break;
+      case OP_DIVI: FOR_SC r[sc][d] = r[sc][a] / imm; break;
+      case OP_RDIVI: FOR_SC r[sc][d] = imm / r[sc][a]; break;
+      case OP_MAXI: FOR_SC r[sc][d] = fmaxf(r[sc][a], imm); break;
+      case OP_MINI: FOR_SC r[sc][d] = fminf(r[sc][a], imm); break;
+      case OP_GE: FOR_SC r[sc][d] = r[sc][a] >= r[sc][b] ? 1.0f : 0.0f; break;
+      case OP_GT: FOR_SC r[sc][d] = r[sc][a] > r[sc][b] ? 1.0f : 0.0f; break;
+      case OP_LE: FOR_SC r[sc][d] = r[sc][a] <= r[sc][b] ? 1.0f : 0.0f; break;
+      case OP_LT: FOR_SC r[sc][d] = r[sc][a] < r[sc][b] ? 1.0f : 0.0f; break;
+      case OP_GEI: FOR_SC r[sc][d] = r[sc][a] >= imm ? 1.0f : 0.0f; break;
+      case OP_GTI: FOR_SC r[sc][d] = r[sc][a] > imm ? 1.0f : 0.0f; break;
+      case OP_LEI: FOR_SC r[sc][d] = r[sc][a] <= imm ? 1.0f : 0.0f; break;
+      case OP_LTI: FOR_SC r[sc][d] = r[sc][a] < imm ? 1.0f : 0.0f; break;
+      case OP_AND: FOR_SC r[sc][d] = r[sc][a] * r[sc][b]; break;
+      case OP_OR: FOR_SC r[sc][d] = fmaxf(r[sc][a], r[sc][b]); break;
+      case OP_NOT: FOR_SC r[sc][d] = 1.0f - r[sc][a]; break;
+      case OP_WHEN: {
+        const uint32_t dn = (d + 1) & (NS - 1);
+        FOR_SC {
+          const float n = r[sc][a] * (1.0f - r[sc][d]) * r[sc][b];
+          r[sc][d] = r[sc][d] + n;
+          r[sc][dn] = n;
+        }
+        break;
+      }
+      case OP_UNTIL: FOR_SC r[sc][d] = fmaxf(r[sc][d], r[sc][a] * r[sc][b]); break;
+      case OP_EMIT: {
+        const uint32_t o = ((w.x >> 8) & 0xffu) & (NOUT - 1);
+        FOR_SC {
+          const double df = ld_f64(A.df, (size_t)A.df_index[sc] * A.n_grid + k);
+          pv[sc][o] += (double)(imm * (r[sc][a] * r[sc][b])) * df;
+        }
+        break;
+      }
+      case OP_RMAX: FOR_SC r[sc][d] = fmaxf(r[sc][d], r[sc][a]); break;
+      case OP_RMIN: FOR_SC r[sc][d] = fminf(r[sc][d], r[sc][a]); break;
+      case OP_AVG: {
+        const uint32_t dc = (d + 1) & (NS - 1);
+        FOR_SC {
+          const float y = r[sc][a] - r[sc][dc];
+          const float t = r[sc][d] + y;
+          r[sc][dc] = (t - r[sc][d]) - y;
+          r[sc][d] = t;
+        }
+        break;
+      }
+      case OP_FLAGERR: {
+        bool hit = false;
+        FOR_SC hit = hit || r[sc][a] != 0.0f;
+        if (live && hit) atomicOr(A.err, 1u << ((uint32_t)imm & 31u));
+        break;
+      }
+      default: break;
+    }
+  }
+}
+
+// Advance every scenario of one path to grid point k with that point's
+// normals zk[0..M).
+template <int MODEL, int K, int SC>
+__device__ __forceinline__ void advance(const PathArgs& A, int k, const float* zk, float (&x)[SC][K], float& v) {
+  if constexpr (MODEL == MK_GBM) {
+    FOR_SC {
+      if (xsrc<SC>(sc) == sc) {
+        const float a = ld_f32(A.tab, ((size_t)k * SC + sc) * 2), b = ld_f32(A.tab, ((size_t)k * SC + sc) * 2 + 1);
+        x[sc][0] = fmaf(b, zk[0], x[sc][0] + a);
+      }
+    }
+  } else if constexpr (MODEL == MK_HESTON) {
+    float t[6];
+#pragma unroll
+    for (int i = 0; i < 6; ++i) t[i] = ld_f32(A.tab, (size_t)kHestonStride * k + i);
+    const float z1 = zk[0], z2 = zk[1];
+    const float vp = A.heston_full_truncation ? fmaxf(v, 0.0f) : v;
+    const float sv = __builtin_amdgcn_sqrtf(vp);
+    x[0][0] = x[0][0] + fmaf(t[1], vp, t[0]) + t[2] * sv * z1;
+    const float wv = fmaf(A.rho, z1, A.rho_perp * z2);
+    const float vn = fmaf(t[3], vp, v) + fmaf(t[5] * sv, wv, t[4]);
+    v = A.heston_full_truncation ? vn : fmaxf(vn, 0.0f);
+  } else {
+#pragma unroll
+    for (int i = 0; i < K; ++i) {
+      float wi = 0.0f;
+#pragma unroll
+      for (int c = 0; c <= i; ++c) wi = fmaf(A.chol[i * K + c], zk[c], wi);
+      const float a = ld_f32(A.tab, ((size_t)k * K + i) * 2), b = ld_f32(A.tab, ((size_t)k * K + i) * 2 + 1);
+      x[0][i] = fmaf(b, wi, x[0][i] + a);
+    }
+  }
+}
+
+}  // namespace kt

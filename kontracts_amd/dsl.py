@@ -243,6 +243,43 @@ class Greeks:
                 f"vega={self.vega:.6f}, rho={self.rho:.6f})")
 
 
+class ModelGreeks:
+    """Price and its derivatives with respect to the model's parameters, each
+    with its Monte Carlo standard error.  ``names`` lists the parameters:
+    GBM (s0, sigma, mu), Heston (s0, v0, kappa, theta, sigma_v, rho),
+    correlated GBM (s0, sigma, mu of each asset)."""
+
+    __slots__ = ("price", "price_std_error", "names", "values", "std_errors", "method")
+
+    def __init__(self, names, out, se, method):
+        self.price, self.price_std_error = out[0], se[0]
+        self.names = list(names)
+        self.values = list(out[1:1 + len(names)])
+        self.std_errors = list(se[1:1 + len(names)])
+        self.method = method
+
+    def __getitem__(self, name: str) -> float:
+        return self.values[self.names.index(name)]
+
+    def as_dict(self) -> dict:
+        return dict(zip(self.names, self.values))
+
+    def __repr__(self):
+        body = ", ".join(f"{n}={v:.6f}" for n, v in zip(self.names, self.values))
+        return f"ModelGreeks({self.method}: price={self.price:.6f}, {body})"
+
+
+def _param_names(model) -> list:
+    if isinstance(model, GBM):
+        return ["s0", "sigma", "mu"]
+    if isinstance(model, Model) and model.kind == N.KT_MODEL_HESTON:
+        return ["s0", "v0", "kappa", "theta", "sigma_v", "rho"]
+    if isinstance(model, Model) and model.kind == N.KT_MODEL_CORRELATED_GBM:
+        fs = [f.name for f in model.params["factors"]]
+        return [f"s0[{n}]" for n in fs] + [f"sigma[{n}]" for n in fs] + [f"mu[{n}]" for n in fs]
+    raise TypeError("expected a GBM or Model")
+
+
 class Contract:
     """Peyton-Jones combinators (src/ast.rs:187-206)."""
 
@@ -325,6 +362,43 @@ class Contract:
         N.check(N.lib().kt_greeks_gbm(eng.handle, self.to_json().encode(), model.asset.encode(), model.s0,
                                       model.sigma, C.byref(cfg.s), C.byref(b), C.byref(out)), eng.handle)
         return Greeks(out)
+
+    def pathwise_greeks(self, model, n_paths: int = 200_000, seed: int = 42, steps_per_year: int = 50,
+                        grid: Sequence[float] | None = None, device: int | None = None) -> ModelGreeks:
+        """Pathwise (adjoint) derivatives of the price with respect to every
+        model parameter, from one GPU launch.  Lipschitz payoffs paid on one
+        date only (European, Asian, lookback, basket); others raise
+        KontractError(Unsupported): use ``bump_greeks`` for them."""
+        names = _param_names(model)
+        m = _model_struct(model)
+        cfg = _cfg(n_paths, seed, steps_per_year, model.rate, grid)
+        out = (C.c_double * (1 + len(names)))()
+        se = (C.c_double * (1 + len(names)))()
+        eng = N.engine(device)
+        N.check(N.lib().kt_greeks_adjoint(eng.handle, self.to_json().encode(), C.byref(m.s), C.byref(cfg.s), out,
+                                          se, len(out)), eng.handle)
+        return ModelGreeks(names, list(out), list(se), "pathwise")
+
+    def bump_greeks(self, model, bumps: Sequence[float] | None = None, n_paths: int = 200_000, seed: int = 42,
+                    steps_per_year: int = 50, grid: Sequence[float] | None = None, device: int | None = None,
+                    std_errors: bool = True) -> ModelGreeks:
+        """Central finite differences over the model parameters with common
+        random numbers (the src/greeks.rs pattern), GBM or Heston.  Default
+        bumps: 1% of spot, then 0.01 (GBM sigma), 1e-4 (mu); Heston 1e-3 (v0),
+        1e-2 (kappa), 1e-3 (theta), 1e-2 (sigma_v), 1e-2 (rho)."""
+        names = _param_names(model)
+        if bumps is None:
+            s0 = model.s0 if isinstance(model, GBM) else model.params.get("spot", 1.0)
+            bumps = [0.01 * s0, 0.01, 1e-4] if isinstance(model, GBM) else [0.01 * s0, 1e-3, 1e-2, 1e-3, 1e-2, 1e-2]
+        m = _model_struct(model)
+        cfg = _cfg(n_paths, seed, steps_per_year, model.rate, grid)
+        h = _dbl(bumps)
+        out = (C.c_double * (1 + len(names)))()
+        se = (C.c_double * (1 + len(names)))()
+        eng = N.engine(device)
+        N.check(N.lib().kt_greeks_bump(eng.handle, self.to_json().encode(), C.byref(m.s), C.byref(cfg.s), h,
+                                       len(bumps), out, se if std_errors else None), eng.handle)
+        return ModelGreeks(names, list(out), list(se) if std_errors else [math.nan] * len(out), "bump")
 
     def price_american(self, model, exercise_dates: Sequence[float], n_paths: int = 100_000, seed: int = 42,
                        n_basis: int = 3, device: int | None = None) -> PriceResult:
