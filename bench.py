@@ -221,6 +221,40 @@ def main():
         extras["config3_pathwise_se"] = dict(zip(gA.names, gA.std_errors))
         extras["config3_bump"] = dict(zip(gB.names, gB.values))
         extras["config3_bump_se"] = dict(zip(gB.names, gB.std_errors))
+        # Config 4: the 1,000-contract book in one persistent launch vs one
+        # launch per contract (the reference's loop of price_gbm calls).
+        from kontracts_amd import book as B
+        from kontracts_amd import tune as T
+        jobs = B.book(1000)
+        tuned = T.autotune_portfolio(B.book(200), repeats=1)
+        extras["config4_autotune"] = {"best": list(tuned["best"]),
+                                      "device_ms": {f"{k[0]}x{k[1]}": v for k, v in tuned["device_ms"].items()}}
+        K.price_portfolio(jobs)
+        t = time.perf_counter()
+        rp = K.price_portfolio(jobs)
+        tp = time.perf_counter() - t
+        t = time.perf_counter()
+        rs = [job[0].price(job[1], n_paths=job[2], seed=job[3], steps_per_year=job[4], grid=job[5]) for job in jobs]
+        ts = time.perf_counter() - t
+        extras["config4_path_steps"] = B.path_steps(jobs)
+        extras["config4_portfolio_s"] = tp
+        extras["config4_per_contract_s"] = ts
+        extras["config4_speedup"] = ts / tp
+        extras["config4_portfolio_path_steps_per_s"] = B.path_steps(jobs) / tp
+        extras["config4_identical_to_per_contract"] = all(a.price == b.price and a.std_error == b.std_error
+                                                          for a, b in zip(rp, rs))
+        # launch-bound regime: a book of the same kinds at 2^14 paths and 3-month maturities (63 steps)
+        small = B.book(1000, paths_exp=(14, 14), maturity=(0.25, 0.25))
+        K.price_portfolio(small)
+        t = time.perf_counter()
+        K.price_portfolio(small)
+        tps = time.perf_counter() - t
+        t = time.perf_counter()
+        for job in small:
+            job[0].price(job[1], n_paths=job[2], seed=job[3], steps_per_year=job[4], grid=job[5])
+        tss = time.perf_counter() - t
+        extras["config4_small_book_speedup"] = tss / tps
+        extras["config4_small_book_portfolio_ms"] = tps * 1e3
 
     line = {
         "metric": METRIC,

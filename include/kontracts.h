@@ -207,6 +207,11 @@ kt_status kt_engine_kernel_time(kt_engine* engine, double* total_ms, uint64_t* n
 /* Tuning knobs of an engine (no reference counterpart).  Names:
  *   "adjoint_segment": Heston pathwise-Greek checkpoint interval, 16 or 32
  *                      grid points (default 16).
+ *   "portfolio_blocks_per_cu": resident blocks per compute unit of the
+ *                      persistent portfolio grid (0 = the kernel's occupancy
+ *                      limit, the default).
+ *   "portfolio_chunk": work items taken per queue fetch (default 1).
+ * Results never depend on these: each job's reduction order is fixed.
  * Unknown names or values return KT_ERR_INVALID_ARGUMENT. */
 kt_status kt_engine_set_option(kt_engine* engine, const char* name, int64_t value);
 

@@ -31,6 +31,7 @@ from .dsl import (
     anytime,
     at,
     average,
+    average_at,
     average_over,
     bool_,
     bull_call_spread,
@@ -66,7 +67,7 @@ from .dsl import (
 __all__ = [
     "DeviceError", "KontractError", "EUR", "GBM", "GBP", "JPY", "USD", "Condition", "Contract", "GbmFactor",
     "Greeks", "Model", "ModelGreeks", "Observable", "Plan", "PriceResult", "S", "__version__", "and_", "anytime", "at",
-    "average", "average_over", "bool_", "bull_call_spread", "cash_or_nothing_call", "compile", "const_",
+    "average", "average_at", "average_over", "bool_", "bull_call_spread", "cash_or_nothing_call", "compile", "const_",
     "correlated_gbm", "describe_program", "down_and_out_call", "european_call", "european_put", "forward", "give",
     "heston", "konst", "one", "or_", "plan", "price_batch", "price_portfolio", "running_max", "running_min",
     "scale", "spot", "straddle", "until", "up_and_out_call", "when", "zero", "zero_coupon_bond",

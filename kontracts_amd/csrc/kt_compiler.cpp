@@ -130,6 +130,36 @@ class Reader {
       }
       expect('}');
       if (!o->a) bad("missing field `obs`");
+    } else if (tag == "AverageAt") {
+      o->kind = Obs::AverageAt;
+      expect('{');
+      bool first = true, seen_dates = false;
+      while (!peek('}')) {
+        if (!first) expect(',');
+        first = false;
+        ws();
+        std::string key = str();
+        expect(':');
+        if (key == "obs") {
+          o->a = observable();
+        } else if (key == "dates") {
+          seen_dates = true;
+          expect('[');
+          bool f2 = true;
+          while (!peek(']')) {
+            if (!f2) expect(',');
+            f2 = false;
+            ws();
+            o->dates.push_back(number());
+          }
+          expect(']');
+        } else {
+          skip_value();
+        }
+      }
+      expect('}');
+      if (!o->a) bad("missing field `obs`");
+      if (!seen_dates) bad("missing field `dates`");
     } else {
       bad("unknown variant `" + tag + "`, expected an Observable");
     }
@@ -352,6 +382,10 @@ void plan_obs(const Obs& o, PlanAcc& acc) {
       acc.fine = true;
       plan_obs(*o.a, acc);
       return;
+    case Obs::AverageAt:  // the fixings join the grid as fixed dates
+      acc.dates.insert(acc.dates.end(), o.dates.begin(), o.dates.end());
+      plan_obs(*o.a, acc);
+      return;
     default:
       plan_obs(*o.a, acc);
       plan_obs(*o.b, acc);
@@ -572,6 +606,7 @@ class Lowerer {
       if (T.last_read < 0) continue;
       for (int k = 0; k <= T.last_read && k < n_; ++k) {
         if (T.node->kind == Obs::Average && !in_window(*T.node, k)) continue;
+        if (T.node->kind == Obs::AverageAt && !is_fixing(*T.node, k)) continue;
         Group g{upd[oi][k], k, temp_base_, {}, spot_used[k]};
         int v = comp_obs(*T.node->a, g);
         for (int code : g.pending)
@@ -669,10 +704,11 @@ class Lowerer {
 
   // --- temporal nodes -------------------------------------------------------
   void collect_obs(const Obs& o, int depth) {
-    bool temporal = o.kind == Obs::Average || o.kind == Obs::RunningMax || o.kind == Obs::RunningMin;
+    bool temporal = o.kind == Obs::Average || o.kind == Obs::RunningMax || o.kind == Obs::RunningMin ||
+                    o.kind == Obs::AverageAt;
     if (temporal && !temporal_index_.count(&o)) {
       Temporal T{&o, depth};
-      T.slot = alloc_persistent(o.kind == Obs::Average ? 2 : 1);
+      T.slot = alloc_persistent(o.kind == Obs::Average || o.kind == Obs::AverageAt ? 2 : 1);
       temporal_index_[&o] = (int)temporals_.size();
       temporals_.push_back(T);
     }
@@ -690,6 +726,13 @@ class Lowerer {
     if (c.cond) collect_cond(*c.cond);
     if (c.a) collect_temporals(*c.a, 0);
     if (c.b) collect_temporals(*c.b, 0);
+  }
+
+  // AverageAt: grid point i is one of the fixing dates.
+  bool is_fixing(const Obs& o, int i) const {
+    for (double d : o.dates)
+      if (std::fabs(grid_[i] - d) <= 1e-12) return true;
+    return false;
   }
 
   bool in_window(const Obs& o, int i) const {
@@ -850,7 +893,8 @@ class Lowerer {
       case Obs::Min: return comp_binary(o, g);
       case Obs::RunningMax:
       case Obs::RunningMin:
-      case Obs::Average: return comp_temporal(o, g);
+      case Obs::Average:
+      case Obs::AverageAt: return comp_temporal(o, g);
     }
     return kSlotZero;
   }
@@ -887,6 +931,38 @@ class Lowerer {
     // Errors of the inner expression are only known once its update ops are
     // compiled; check the inner expression for unknown assets right away.
     if (has_unknown_asset(*o.a)) g.pending.push_back(KT_ERR_UNKNOWN_ASSET);
+    if (o.kind == Obs::AverageAt) {
+      // mean over the fixings on or before now; each must be a grid point
+      if (o.dates.empty()) {
+        g.pending.push_back(KT_ERR_MALFORMED_CONTRACT);
+        return kSlotZero;
+      }
+      for (size_t i = 1; i < o.dates.size(); ++i)
+        if (!(o.dates[i] > o.dates[i - 1])) {
+          g.pending.push_back(KT_ERR_MALFORMED_CONTRACT);
+          return kSlotZero;
+        }
+      const double t_now = grid_[g.k];
+      int count = 0;
+      for (double d : o.dates) {
+        if (d > t_now + 1e-12) break;
+        bool on_grid = false;
+        for (int i = 0; i < n_ && !on_grid; ++i) on_grid = std::fabs(grid_[i] - d) <= 1e-12;
+        if (!on_grid) {
+          g.pending.push_back(KT_ERR_TIME_OUT_OF_RANGE);
+          return kSlotZero;
+        }
+        ++count;
+      }
+      if (count == 0) {
+        g.pending.push_back(KT_ERR_MALFORMED_CONTRACT);
+        return kSlotZero;
+      }
+      int d = temp(g);
+      op(g, OP_SUB, d, T.slot, T.slot + 1);
+      op(g, OP_MULI, d, d, 0, (float)(1.0 / (double)count));
+      return d;
+    }
     if (o.kind != Obs::Average) return T.slot;
     // Average over the window (src/observable.rs:103-133).
     double t_now = grid_[g.k];

@@ -19,12 +19,15 @@ struct Error {
 
 // ---- AST (kontract src/ast.rs:34-206) -------------------------------------
 struct Obs {
-  enum Kind { Const, Spot, Neg, Add, Sub, Mul, Div, Max, Min, Average, RunningMax, RunningMin } kind;
+  // AverageAt is an extension (no reference counterpart): the arithmetic mean
+  // of obs over fixing dates, {"AverageAt":{"obs":...,"dates":[t1,...]}}.
+  enum Kind { Const, Spot, Neg, Add, Sub, Mul, Div, Max, Min, Average, RunningMax, RunningMin, AverageAt } kind;
   double value = 0.0;    // Const
   std::string name;      // Spot
   std::unique_ptr<Obs> a, b;
   bool has_from = false, has_to = false;  // Average window
   double from = 0.0, to = 0.0;
+  std::vector<double> dates;  // AverageAt fixings
 };
 
 struct Cond {

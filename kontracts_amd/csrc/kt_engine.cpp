@@ -163,7 +163,8 @@ int lowest_error_code(unsigned bits) {
 const char* error_text(int code) {
   switch (code) {
     case KT_ERR_UNKNOWN_ASSET: return "unknown asset referenced by the contract";
-    case KT_ERR_MALFORMED_CONTRACT: return "malformed contract: empty or invalid average window";
+    case KT_ERR_TIME_OUT_OF_RANGE: return "a fixing date of AverageAt is not a grid point";
+    case KT_ERR_MALFORMED_CONTRACT: return "malformed contract: empty or invalid average window or fixing set";
     case KT_ERR_UNSUPPORTED:
       return "unsupported: `or` reached on a path, or an average window extending past its evaluation step";
     default: return "error raised on a path";
@@ -177,7 +178,9 @@ const char* error_text(int code) {
 // ---------------------------------------------------------------------------
 kt_engine::~kt_engine() {
   (void)hipSetDevice(device);
-  for (DevBuf* b : {&arena, &partials, &stats, &errw, &pv, &z, &scratch, &paths, &aux})
+  if (pf_exec) (void)hipGraphExecDestroy(pf_exec);
+  if (pf_graph) (void)hipGraphDestroy(pf_graph);
+  for (DevBuf* b : {&arena, &partials, &stats, &errw, &pv, &z, &scratch, &paths, &aux, &book})
     if (b->p) (void)hipFree(b->p);
   if (ev0) (void)hipEventDestroy(ev0);
   if (ev1) (void)hipEventDestroy(ev1);
@@ -256,6 +259,28 @@ void fill_model_args(PathArgs& A, const ModelDesc& md, uint64_t seed) {
   A.s0_scen[0] = (float)md.s0[0];
 }
 
+// Knock-out thresholds log2(H/s0) per group and scenario; groups whose spot
+// or level is not positive are interpreted instead.
+std::vector<KGroup> knockout_tables(const Program& prog, const std::vector<ModelDesc>& models,
+                                    std::vector<float>* thr) {
+  const size_t SC = models.size();
+  const ModelDesc& md = models[0];
+  std::vector<KGroup> groups = prog.groups;
+  thr->assign(std::max<size_t>(groups.size() * SC, 1), 0.0f);
+  for (size_t g = 0; g < groups.size(); ++g) {
+    KGroup& gd = groups[g];
+    if (gd.kind < G_KO_GE || gd.kind > G_KO_LT) continue;
+    bool ok = gd.imm > 0.0f && gd.asset >= 0 && gd.asset < (int)md.s0.size();
+    for (size_t sc = 0; ok && sc < SC; ++sc) {
+      double s0 = models[sc].s0[gd.asset];
+      ok = s0 > 0.0;
+      if (ok) (*thr)[g * SC + sc] = (float)std::log2((double)gd.imm / s0);
+    }
+    if (!ok) gd.kind = G_GENERIC;
+  }
+  return groups;
+}
+
 std::vector<KStats> run_program_scen(kt_engine* E, const Program& prog, const std::vector<ModelDesc>& models,
                                      const std::vector<double>& rates, const std::vector<std::vector<double>>& stat_w,
                                      const std::vector<double>& grid, uint64_t n_paths, uint64_t seed,
@@ -293,21 +318,8 @@ std::vector<KStats> run_program_scen(kt_engine* E, const Program& prog, const st
   std::vector<double> df(distinct.size() * grid.size());
   for (size_t t = 0; t < distinct.size(); ++t)
     for (size_t k = 0; k < grid.size(); ++k) df[t * grid.size() + k] = std::exp(-distinct[t] * grid[k]);
-  // knock-out thresholds log2(H/s0) per group and scenario; groups whose
-  // spot or level is not positive are interpreted instead
-  std::vector<KGroup> groups = prog.groups;
-  std::vector<float> thr(std::max<size_t>(groups.size() * SC, 1), 0.0f);
-  for (size_t g = 0; g < groups.size(); ++g) {
-    KGroup& gd = groups[g];
-    if (gd.kind < G_KO_GE || gd.kind > G_KO_LT) continue;
-    bool ok = gd.imm > 0.0f && gd.asset >= 0 && gd.asset < (int)md.s0.size();
-    for (int sc = 0; ok && sc < SC; ++sc) {
-      double s0 = models[sc].s0[gd.asset];
-      ok = s0 > 0.0;
-      if (ok) thr[g * SC + sc] = (float)std::log2((double)gd.imm / s0);
-    }
-    if (!ok) gd.kind = G_GENERIC;
-  }
+  std::vector<float> thr;
+  std::vector<KGroup> groups = knockout_tables(prog, models, &thr);
 
   int kind = model_kernel_kind(md);
   int K = (int)md.names.size();
@@ -629,6 +641,12 @@ kt_status kt_engine_set_option(kt_engine* E, const char* name, int64_t value) {
   if (key == "adjoint_segment") {
     if (value != 16 && value != 32) raise(KT_ERR_INVALID_ARGUMENT, "adjoint_segment must be 16 or 32");
     E->adj_seg = (int)value;
+  } else if (key == "portfolio_blocks_per_cu") {
+    if (value < 0 || value > 64) raise(KT_ERR_INVALID_ARGUMENT, "portfolio_blocks_per_cu must be in [0, 64]");
+    E->pf_blocks_per_cu = (int)value;
+  } else if (key == "portfolio_chunk") {
+    if (value < 1 || value > 1024) raise(KT_ERR_INVALID_ARGUMENT, "portfolio_chunk must be in [1, 1024]");
+    E->pf_chunk = (int)value;
   } else {
     raise(KT_ERR_INVALID_ARGUMENT, "unknown engine option: " + key);
   }

@@ -21,7 +21,13 @@ struct kt_engine {
   int device = 0;
   hipStream_t stream = nullptr;
   std::string err;
-  DevBuf arena, partials, stats, errw, pv, z, scratch, paths, aux;
+  DevBuf arena, partials, stats, errw, pv, z, scratch, paths, aux, book;
+  // persistent portfolio launch: tile choice and the captured launch graph
+  int pf_blocks_per_cu = 0;  // 0: the kernel's occupancy limit
+  int pf_chunk = 1;
+  hipGraph_t pf_graph = nullptr;
+  hipGraphExec_t pf_exec = nullptr;
+  std::vector<uint64_t> pf_key;
   std::vector<char> host_stage;
   bool profiling = false;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -72,6 +78,9 @@ PreparedContract prepare(const char* json, const ModelDesc& md, const kt_mc_conf
 std::vector<KStats> run_program(kt_engine* E, const Program& prog, const ModelDesc& md,
                                 const std::vector<double>& grid, double rate, uint64_t n_paths, uint64_t seed,
                                 double* pv_host, float* z_host, double* pv_keep = nullptr);
+
+std::vector<KGroup> knockout_tables(const Program& prog, const std::vector<ModelDesc>& models,
+                                    std::vector<float>* thr);
 
 // Model constants of the path kernels (spots, Cholesky factor, Heston
 // correlation split) and the Philox key.

@@ -262,8 +262,4 @@ void greeks_adjoint(kt_engine* E, const char* json, const kt_model* model, const
   }
 }
 
-void price_portfolio(kt_engine*, const kt_job*, uint64_t, kt_price_result*) {
-  raise(KT_ERR_UNSUPPORTED, "portfolio: not built yet");
-}
-
 }  // namespace kt

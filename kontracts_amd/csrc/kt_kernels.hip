@@ -19,12 +19,14 @@
 // model parameters share every normal of a path (common random numbers); each
 // scenario has its own log-spot, register file and present value, and the
 // kernel reduces SC-weighted combinations of them (finite-difference Greeks).
+#include <algorithm>
+
 #include "kt_path_ops.h"
 
 namespace kt {
 
-template <int MODEL, int K, int SC, bool DEBUG>
-__device__ __forceinline__ void store_normals(const PathArgs& A, int k, const float* zk, bool live, uint64_t local) {
+template <int MODEL, int K, int SC, bool DEBUG, class PA>
+__device__ __forceinline__ void store_normals(const PA& A, int k, const float* zk, bool live, uint64_t local) {
   constexpr int M = MODEL == MK_GBM ? 1 : (MODEL == MK_HESTON ? 2 : K);
   if constexpr (DEBUG) {
     if (live && A.z_out) {
@@ -35,8 +37,8 @@ __device__ __forceinline__ void store_normals(const PathArgs& A, int k, const fl
 }
 
 // One interpreted grid point.
-template <int MODEL, int K, int NS, int NOUT, int SC, bool DEBUG>
-__device__ __forceinline__ void generic_point(const PathArgs& A, int k, const float* zk, float (&x)[SC][K],
+template <int MODEL, int K, int NS, int NOUT, int SC, bool DEBUG, class PA>
+__device__ __forceinline__ void generic_point(const PA& A, int k, const float* zk, float (&x)[SC][K],
                                               float& v, float (&r)[SC][NS], double (&pv)[SC][NOUT], bool live,
                                               uint64_t local) {
   advance<MODEL, K, SC>(A, k, zk, x, v);
@@ -53,8 +55,8 @@ __device__ __forceinline__ void generic_point(const PathArgs& A, int k, const fl
 // follows.  Knock-out groups keep the running max (min) of log-spot over the
 // group and compare it once with the threshold log2(H/s0): some point
 // satisfies S >= H exactly when the group's maximum does.
-template <int KIND, int MODEL, int K, int SC, bool DEBUG>
-__device__ __forceinline__ void fast_group(const PathArgs& A, const KGroup gd, int g, int k0, const float* z,
+template <int KIND, int MODEL, int K, int SC, bool DEBUG, class PA>
+__device__ __forceinline__ void fast_group(const PA& A, const KGroup gd, int g, int k0, const float* z,
                                            float (&x)[SC][K], float& v, float (&acc)[SC], float (&comp)[SC],
                                            bool live, uint64_t local) {
   constexpr int M = MODEL == MK_GBM ? 1 : (MODEL == MK_HESTON ? 2 : K);
@@ -103,15 +105,19 @@ __device__ __forceinline__ void fast_group(const PathArgs& A, const KGroup gd, i
   }
 }
 
-template <int MODEL, int K, int NS, int NOUT, int SC, int NSTAT, bool DEBUG>
-__global__ __launch_bounds__(kBlock) void path_kernel(const PathArgs A) {
+// One block of 256 paths: paths blk*256 .. blk*256+255 of the launch (global
+// path index path_base + local), statistics into partials[blk].  Shared by
+// the per-contract path kernel and the persistent portfolio kernel.
+// r: the op program's register file, owned by the caller so that callers
+// with several variants keep one register file for all of them.
+template <int MODEL, int K, int NS, int NOUT, int SC, int NSTAT, bool DEBUG, class PA>
+__device__ __forceinline__ void path_block(const PA& A, uint32_t blk, float (&r)[SC][NS]) {
   constexpr int M = MODEL == MK_GBM ? 1 : (MODEL == MK_HESTON ? 2 : K);
-  const uint64_t local = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+  const uint64_t local = (uint64_t)blk * kBlock + threadIdx.x;
   const bool live = local < A.n_paths;
   const uint64_t path = A.path_base + local;
   const uint32_t p_lo = (uint32_t)path, p_hi = (uint32_t)(path >> 32);
 
-  float r[SC][NS];
   double pv[SC][NOUT];
   float x[SC][K];
   FOR_SC {
@@ -255,14 +261,19 @@ __global__ __launch_bounds__(kBlock) void path_kernel(const PathArgs A) {
     KStats acc = sh[threadIdx.x][0];
 #pragma unroll
     for (int wv = 1; wv < kBlock / 64; ++wv) acc = stats_combine(acc, sh[threadIdx.x][wv]);
-    A.partials[(size_t)blockIdx.x * NSTAT + threadIdx.x] = acc;
+    A.partials[(size_t)blk * NSTAT + threadIdx.x] = acc;
   }
 }
 
-// Deterministic combine of per-block statistics: block j reduces output j.
-__global__ __launch_bounds__(256) void finalize_kernel(const KStats* __restrict__ partials, int n_blocks, int nout,
-                                                       KStats* __restrict__ out) {
-  const int j = blockIdx.x;
+template <int MODEL, int K, int NS, int NOUT, int SC, int NSTAT, bool DEBUG>
+__global__ __launch_bounds__(kBlock) void path_kernel(const PathArgs A) {
+  float r[SC][NS];
+  path_block<MODEL, K, NS, NOUT, SC, NSTAT, DEBUG>(A, blockIdx.x, r);
+}
+
+// Deterministic combine of per-block statistics of output j (whole block).
+__device__ __forceinline__ void finalize_body(const KStats* __restrict__ partials, int n_blocks, int nout, int j,
+                                              KStats* __restrict__ out) {
   KStats acc{0.0, 0.0, 0.0};
   // Thread t takes blocks t, t + 256, ... eight at a time: the eight loads
   // issue together and are combined as a pairwise tree (fixed order).
@@ -286,7 +297,63 @@ __global__ __launch_bounds__(256) void finalize_kernel(const KStats* __restrict_
     if (threadIdx.x < s) sh[threadIdx.x] = stats_combine(sh[threadIdx.x], sh[threadIdx.x + s]);
     __syncthreads();
   }
-  if (threadIdx.x == 0) out[j] = sh[0];
+  if (threadIdx.x == 0) *out = sh[0];
+}
+
+// Block j reduces output j.
+__global__ __launch_bounds__(256) void finalize_kernel(const KStats* __restrict__ partials, int n_blocks, int nout,
+                                                       KStats* __restrict__ out) {
+  finalize_body(partials, n_blocks, nout, blockIdx.x, out + blockIdx.x);
+}
+
+// ---------------------------------------------------------------------------
+// Persistent portfolio kernel: a grid sized to the machine takes (job, block)
+// work items from a queue ordered longest-first and runs each with the same
+// path_block code as the per-contract launch, so every job's statistics are
+// identical to pricing it alone.
+// ---------------------------------------------------------------------------
+// Clears the queue head and the jobs' error words (a kernel, not a memset, so
+// that the launch sequence can be captured in a graph and replayed).
+__global__ void portfolio_reset_kernel(unsigned* counter, unsigned* err, int n_jobs) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) *counter = 0u;
+  for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n_jobs; j += gridDim.x * blockDim.x) err[j] = 0u;
+}
+
+template <int NS>
+__global__ __launch_bounds__(kBlock) void portfolio_kernel(const PortfolioArgs Q) {
+  __shared__ uint32_t s_first;
+  float r[1][NS];
+  const KT_CONST PortfolioJob* jobs = (const KT_CONST PortfolioJob*)Q.jobs;
+  const KT_CONST uint32_t* items = (const KT_CONST uint32_t*)Q.items;
+  for (;;) {
+    __syncthreads();  // every lane has read the previous s_first
+    if (threadIdx.x == 0) s_first = atomicAdd(Q.counter, (unsigned)Q.chunk);
+    __syncthreads();
+    const uint32_t first = __builtin_amdgcn_readfirstlane(s_first);
+    if (first >= Q.n_items) return;
+    const uint32_t last = min(first + (uint32_t)Q.chunk, Q.n_items);
+    for (uint32_t it = first; it < last; ++it) {
+      const uint32_t j = items[2 * (size_t)it], blk = items[2 * (size_t)it + 1];
+      const KT_CONST PathArgs& A = jobs[j].a;
+      switch (jobs[j].variant) {
+        case PV_GBM: path_block<MK_GBM, 1, NS, 1, 1, 1, false>(A, blk, r); break;
+        case PV_HESTON: path_block<MK_HESTON, 1, NS, 1, 1, 1, false>(A, blk, r); break;
+        case PV_BASKET2: path_block<MK_BASKET, 2, NS, 1, 1, 1, false>(A, blk, r); break;
+        case PV_BASKET3: path_block<MK_BASKET, 3, NS, 1, 1, 1, false>(A, blk, r); break;
+        case PV_BASKET4: path_block<MK_BASKET, 4, NS, 1, 1, 1, false>(A, blk, r); break;
+        default: path_block<MK_BASKET, 5, NS, 1, 1, 1, false>(A, blk, r); break;
+      }
+    }
+  }
+}
+
+// Block j finalizes job j's partials (the per-contract finalize, unchanged).
+__global__ __launch_bounds__(256) void portfolio_finalize_kernel(const PortfolioJob* __restrict__ jobs,
+                                                                 const int* __restrict__ n_blocks,
+                                                                 KStats* __restrict__ out) {
+  const int j = blockIdx.x;
+  const KStats* partials = jobs[j].a.partials;
+  finalize_body(partials, n_blocks[j], 1, 0, out + j);
 }
 
 __global__ __launch_bounds__(256) void diff_stats_kernel(const double* __restrict__ a, const double* __restrict__ b,
@@ -745,6 +812,21 @@ hipError_t launch_diff_stats(const double* a, const double* b, double scale, uin
                              hipStream_t s) {
   if (n == 0) return hipSuccess;
   hipLaunchKernelGGL(diff_stats_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a, b, scale, n, partials);
+  return hipGetLastError();
+}
+
+int portfolio_occupancy_blocks_per_cu() {
+  int n = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, portfolio_kernel<16>, kBlock, 0) != hipSuccess) return 4;
+  return n > 0 ? n : 1;
+}
+
+hipError_t launch_portfolio(const PortfolioArgs& q, const int* n_blocks, int n_jobs, int grid, unsigned* err,
+                            KStats* out, hipStream_t s) {
+  hipLaunchKernelGGL(portfolio_reset_kernel, dim3(std::max(1, std::min(64, (n_jobs + 255) / 256))), dim3(256), 0, s,
+                     q.counter, err, n_jobs);
+  hipLaunchKernelGGL(portfolio_kernel<16>, dim3(grid), dim3(kBlock), 0, s, q);
+  hipLaunchKernelGGL(portfolio_finalize_kernel, dim3(n_jobs), dim3(256), 0, s, q.jobs, n_blocks, out);
   return hipGetLastError();
 }
 

@@ -111,6 +111,29 @@ hipError_t launch_lsm_pass(const LsmPassArgs& p, int n_blocks, hipStream_t s);
 // One block: reduce reg and write coef[0..kLsmBasisMax] (coefficients, fitted flag).
 hipError_t launch_lsm_solve(const double* reg, int n_blocks, int m, double* coef, hipStream_t s);
 
+// Persistent portfolio launch.  Each job is one contract priced as kt_price
+// would (single output, 16-register program); variant picks the path code.
+enum PortfolioVariant { PV_GBM = 0, PV_HESTON = 1, PV_BASKET2 = 2, PV_BASKET3 = 3, PV_BASKET4 = 4, PV_BASKET5 = 5 };
+constexpr int kPortfolioMaxAssets = 5;
+struct PortfolioJob {
+  PathArgs a;  // partials: this job's own [n_blocks] range
+  int variant;
+  int pad;
+};
+struct PortfolioArgs {
+  const PortfolioJob* jobs;
+  const uint32_t* items;  // [n_items][2] {job, block within the job}
+  uint32_t n_items;
+  int chunk;              // items taken per queue fetch
+  unsigned* counter;      // queue head (reset by the launch)
+};
+// Reset the queue and the jobs' error words err[n_jobs], run the persistent
+// grid, finalize each job into out[job].
+hipError_t launch_portfolio(const PortfolioArgs& q, const int* n_blocks, int n_jobs, int grid, unsigned* err,
+                            KStats* out, hipStream_t s);
+// Resident 256-lane blocks of the persistent kernel per compute unit.
+int portfolio_occupancy_blocks_per_cu();
+
 // partials[block] = statistics of (a[i] - b[i]) * scale over the block's paths.
 hipError_t launch_diff_stats(const double* a, const double* b, double scale, uint64_t n, KStats* partials,
                              hipStream_t s);

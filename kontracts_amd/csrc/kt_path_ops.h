@@ -21,8 +21,11 @@ __device__ __forceinline__ float ld_f32(const float* p, size_t i) { return cst(p
 __device__ __forceinline__ double ld_f64(const double* p, size_t i) { return cst(p)[i]; }
 
 // Wave-uniform selection of a per-asset kernel argument.
-template <int K>
-__device__ __forceinline__ float pick_asset(const float (&arr)[kMaxAssets], uint32_t i) {
+// PA (below) is PathArgs, or PathArgs in the constant address space when a
+// persistent kernel reads a job's arguments in place: every field access is
+// then a scalar load.
+template <int K, class ARR>
+__device__ __forceinline__ float pick_asset(const ARR& arr, uint32_t i) {
   float v = arr[0];
 #pragma unroll
   for (int c = 1; c < K; ++c)
@@ -30,8 +33,8 @@ __device__ __forceinline__ float pick_asset(const float (&arr)[kMaxAssets], uint
   return v;
 }
 
-template <int K, int SC>
-__device__ __forceinline__ float scen_s0(const PathArgs& A, int sc, uint32_t asset) {
+template <int K, int SC, class PA>
+__device__ __forceinline__ float scen_s0(const PA& A, int sc, uint32_t asset) {
   if constexpr (SC > 1) return A.s0_scen[sc];
   return pick_asset<K>(A.s0, asset);
 }
@@ -46,8 +49,8 @@ __device__ constexpr int xsrc(int sc) {
   return (SC == kScenarios && (sc == 1 || sc == 2)) ? 0 : sc;
 }
 
-template <int NS, int NOUT, int K, int SC>
-__device__ __forceinline__ void run_ops(const PathArgs& A, int k, int begin, int count, float (&r)[SC][NS],
+template <int NS, int NOUT, int K, int SC, class PA>
+__device__ __forceinline__ void run_ops(const PA& A, int k, int begin, int count, float (&r)[SC][NS],
                                         double (&pv)[SC][NOUT], const float (&x)[SC][K], bool live) {
   for (int i = 0; i < count; ++i) {
     const uint64_t word = ld_u64(A.ops, (size_t)begin + i);
@@ -134,8 +137,8 @@ __device__ __forceinline__ void run_ops(const PathArgs& A, int k, int begin, int
 
 // Advance every scenario of one path to grid point k with that point's
 // normals zk[0..M).
-template <int MODEL, int K, int SC>
-__device__ __forceinline__ void advance(const PathArgs& A, int k, const float* zk, float (&x)[SC][K], float& v) {
+template <int MODEL, int K, int SC, class PA>
+__device__ __forceinline__ void advance(const PA& A, int k, const float* zk, float (&x)[SC][K], float& v) {
   if constexpr (MODEL == MK_GBM) {
     FOR_SC {
       if (xsrc<SC>(sc) == sc) {

@@ -458,6 +458,14 @@ def average_over(obs: Observable, from_year: float, to_year: float) -> Observabl
                                    "to_year": float(to_year)}})
 
 
+def average_at(obs: Observable, dates: Sequence[float]) -> Observable:
+    """Arithmetic mean of ``obs`` over fixing dates (extension, no reference
+    counterpart): at evaluation time t, the mean over the fixings on or before
+    t.  The fixings join the pricing grid as fixed dates; a monthly-fixing
+    Asian on a daily grid is ``average_at(S, [i / 12 for i in 1..12])``."""
+    return Observable({"AverageAt": {"obs": _obs(obs).node, "dates": [float(d) for d in dates]}})
+
+
 def running_max(obs: Observable) -> Observable:
     return Observable({"RunningMax": _obs(obs).node})
 
@@ -656,22 +664,31 @@ def price_batch(contracts: Iterable[Contract], model, n_paths: int = 100_000, se
     return [PriceResult(r) for r in out]
 
 
-def price_portfolio(jobs: Sequence[tuple], device: int | None = None) -> list[PriceResult]:
-    """Heterogeneous portfolio: jobs are (contract, model, n_paths, seed,
-    steps_per_year) tuples, priced in one persistent launch."""
-    jobs = list(jobs)
-    if not jobs:
-        return []
+def _jobs_array(jobs):
     keep = []
     arr = (N.KtJob * len(jobs))()
-    for i, (con, model, n_paths, seed, spy) in enumerate(jobs):
+    for i, job in enumerate(jobs):
+        con, model, n_paths, seed, spy = job[:5]
+        grid = job[5] if len(job) > 5 else None
         m = _model_struct(model)
-        cfg = _cfg(n_paths, seed, spy, model.rate, None)
+        cfg = _cfg(n_paths, seed, spy, model.rate, grid)
         js = con.to_json().encode()
         keep += [m, cfg, js]
         arr[i].contract_json = js
         arr[i].model = m.s
         arr[i].cfg = cfg.s
+    return arr, keep
+
+
+def price_portfolio(jobs: Sequence[tuple], device: int | None = None) -> list[PriceResult]:
+    """Heterogeneous portfolio: jobs are (contract, model, n_paths, seed,
+    steps_per_year[, grid]) tuples, priced in one persistent launch; each
+    result equals ``contract.price(model, n_paths, seed, steps_per_year,
+    grid=grid)``."""
+    jobs = list(jobs)
+    if not jobs:
+        return []
+    arr, keep = _jobs_array(jobs)
     out = (N.KtPriceResult * len(jobs))()
     eng = N.engine(device)
     N.check(N.lib().kt_price_portfolio(eng.handle, arr, len(jobs), out), eng.handle)

@@ -151,13 +151,14 @@ static inline uint32_t philox_word(uint64_t seed, uint64_t path, uint64_t j, uin
 // tagged representation.
 // ---------------------------------------------------------------------------
 struct Obs {
-  enum K { Const, Spot, Neg, Add, Sub, Mul, Div, Max, Min, Average, RunningMax, RunningMin } k;
+  enum K { Const, Spot, Neg, Add, Sub, Mul, Div, Max, Min, Average, RunningMax, RunningMin, AverageAt } k;
   double c = 0.0;
   std::string name;
   int asset = -1;  // resolved index into the path's assets; -1 = unknown
   std::unique_ptr<Obs> a, b;
   bool has_from = false, has_to = false;
   double from = 0.0, to = 0.0;
+  std::vector<double> dates;  // AverageAt fixing dates
 };
 struct Cond {
   enum K { Bool, At, Ge, Gt, Le, Lt, And, Or, Not } k;
@@ -368,6 +369,24 @@ static std::unique_ptr<Obs> to_obs(const JVal& v) {
         fail(E_SERDE, "json: unknown Average field");
     }
     if (!seen_obs) fail(E_SERDE, "json: Average missing obs");
+  } else if (tag == "AverageAt") {
+    // Extension (no reference counterpart): arithmetic mean of obs over fixing
+    // dates, {"AverageAt":{"obs":...,"dates":[t1,...]}}.
+    if (body.t != JVal::Obj) fail(E_SERDE, "json: AverageAt expects object");
+    o->k = Obs::AverageAt;
+    bool seen_obs = false, seen_dates = false;
+    for (auto& kv : body.obj) {
+      if (kv.first == "obs") {
+        o->a = to_obs(kv.second);
+        seen_obs = true;
+      } else if (kv.first == "dates") {
+        if (kv.second.t != JVal::Arr) fail(E_SERDE, "json: AverageAt dates expects array");
+        for (auto& d : kv.second.arr) o->dates.push_back(num(d));
+        seen_dates = true;
+      } else
+        fail(E_SERDE, "json: unknown AverageAt field");
+    }
+    if (!seen_obs || !seen_dates) fail(E_SERDE, "json: AverageAt needs obs and dates");
   } else if (tag == "RunningMax") {
     o->k = Obs::RunningMax;
     o->a = to_obs(body);
@@ -493,6 +512,10 @@ static void walk_obs(const Obs& o, Acc& acc) {
     case Obs::RunningMax:
     case Obs::RunningMin:
       acc.fine = true;
+      walk_obs(*o.a, acc);
+      break;
+    case Obs::AverageAt:  // the fixings join the grid as fixed dates
+      acc.dates.insert(acc.dates.end(), o.dates.begin(), o.dates.end());
       walk_obs(*o.a, acc);
       break;
     default:
@@ -655,6 +678,31 @@ static double eval_obs(const Obs& o, const PathView& p, size_t t) {
         }
       }
       if (cnt == 0) fail(E_MALFORMED, "average window empty");
+      return sum / (double)cnt;
+    }
+    case Obs::AverageAt: {
+      // mean of obs at the fixings on or before now; each fixing must be a
+      // grid point; fixings strictly increasing
+      if (t >= p.n) fail(E_STEP_OUT_OF_RANGE, "step");
+      if (o.dates.empty()) fail(E_MALFORMED, "AverageAt: no fixing dates");
+      for (size_t i = 1; i < o.dates.size(); ++i)
+        if (!(o.dates[i] > o.dates[i - 1])) fail(E_MALFORMED, "AverageAt: fixing dates must increase");
+      double t_now = p.times[t];
+      double sum = 0.0;
+      size_t cnt = 0;
+      for (double d : o.dates) {
+        if (d > t_now + 1e-12) break;
+        size_t hit = p.n;
+        for (size_t i = 0; i < p.n; ++i)
+          if (std::fabs(p.times[i] - d) <= 1e-12) {
+            hit = i;
+            break;
+          }
+        if (hit == p.n) fail(E_TIME_OUT_OF_RANGE, "AverageAt: fixing date not on the grid");
+        sum += eval_obs(*o.a, p, hit);
+        ++cnt;
+      }
+      if (cnt == 0) fail(E_MALFORMED, "AverageAt: no fixing on or before the evaluation date");
       return sum / (double)cnt;
     }
     case Obs::RunningMax: {

@@ -132,3 +132,43 @@ def test_program_lowering_shapes():
     with pytest.raises(K.KontractError) as e:
         K.describe_program(K.european_call("X", 100, 1.0), K.heston(100, 0.04, 2, 0.04, 0.3, -1.5, 0.05, "X"))
     assert e.value.kind == "MalformedContract"
+
+
+def test_average_at_plan_and_lowering():
+    """AverageAt fixings join the grid; the accumulator is fed at the fixing
+    points only and read with the static count of past fixings."""
+    fix = [0.25, 0.5, 0.75, 1.0]
+    c = K.scale((K.average_at(K.spot("X"), fix) - 100.0).clip(0.0), K.one("USD")) @ K.at(1.0)
+    p = K.plan(c)
+    for d in fix:
+        assert d in p.fixed_dates
+    assert not p.needs_fine_grid
+    text = K.describe_program(c, K.GBM(100.0, 0.2, 0.05, asset="X"), grid=[i / 8 for i in range(9)])
+    fed = [ln for ln in text.splitlines() if "avg(" in ln]
+    # fed at grid points 2, 4, 6, 8 (t = 0.25, 0.5, 0.75, 1.0) only
+    assert any(ln.startswith("steps 2..2") for ln in fed) and not any(ln.startswith("steps 1..1") for ln in fed)
+    assert "muli(5,5,0,0.25)" in text
+    # round trip through the engine's reader
+    assert K.Contract.from_json(c.to_json()) == c
+
+
+def test_portfolio_book_generator():
+    """The batched-portfolio book: deterministic, 250 of each kind, daily grids
+    of 63..756 steps, 2^16..2^20 paths, single-asset models split GBM/Heston,
+    and every program small enough for the persistent kernel."""
+    from kontracts_amd import book as B
+    jobs = B.book(1000)
+    again = B.book(1000)
+    assert [j[0].to_json() for j in jobs] == [j[0].to_json() for j in again]
+    assert [j[0].to_json() for j in B.book(40)] == [j[0].to_json() for j in jobs[:40]]
+    steps = [len(j[5]) - 1 for j in jobs]
+    assert 63 <= min(steps) and max(steps) <= 756
+    assert {j[2] for j in jobs} <= {1 << e for e in range(16, 21)}
+    kinds = [B.KINDS[i % 4] for i in range(1000)]
+    assert all(kinds.count(k) == 250 for k in B.KINDS)
+    n_heston = sum(1 for j in jobs if isinstance(j[1], K.Model) and j[1].label == "Heston")
+    assert 300 < n_heston < 450
+    assert 1.4e11 < B.path_steps(jobs) < 1.9e11
+    for j in jobs[:8]:
+        first = K.describe_program(j[0], j[1], grid=j[5]).splitlines()[0]
+        assert int(first.split()[1].rstrip(",")) <= 16

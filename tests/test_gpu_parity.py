@@ -133,6 +133,9 @@ PARITY_CASES = {
     "basket5": (BASKET_PAYOFF, BASKET5, 50, None, 100.0, 0),
     "average_window": (K.scale((K.average_over(K.spot("X"), 0.25, 0.75) - K.spot("X")).clip(0.0), K.one("USD"))
                        @ K.at(1.0), GBM_X, 20, None, 100.0, 0),
+    # AverageAt extension: monthly fixings on a daily grid, under Heston
+    "heston_asian_monthly": (K.scale((K.average_at(K.spot("X"), [i / 12 for i in range(1, 13)]) - 100.0).clip(0.0),
+                                     K.one("USD")) @ K.at(1.0), HESTON_X, 252, None, 100.0, 0),
 }
 
 

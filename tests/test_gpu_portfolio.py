@@ -1,0 +1,102 @@
+"""Persistent portfolio launch (kt_price_portfolio) against per-contract
+pricing.  The reference prices a book as a loop of price_gbm calls
+(src/pricer.rs:97-115); every job of one persistent launch must give exactly
+what pricing that job alone gives (same paths, same per-block statistics,
+same reduction order), whatever the tile configuration, and errors surface as
+the loop would raise them.
+"""
+import pytest
+
+import kontracts_amd as K
+from kontracts_amd import _native as N
+from kontracts_amd import book as B
+
+pytestmark = pytest.mark.gpu
+
+
+def small_book(n=36, paths=(4096, 9000, 16384, 300)):
+    jobs = []
+    for i, job in enumerate(B.book(n)):
+        con, model, _, seed, spy, grid = job
+        jobs.append((con, model, paths[i % len(paths)], seed, spy, grid))
+    return jobs
+
+
+def edge_jobs():
+    g = K.GBM(100.0, 0.25, 0.03, asset="X")
+    h = K.heston(100.0, 0.04, 1.5, 0.05, 0.4, -0.6, 0.03, asset="X")
+    f3 = [K.GbmFactor(n, 100.0, 0.03, s) for n, s in zip("ABC", (0.2, 0.3, 0.25))]
+    b3 = K.correlated_gbm(f3, [[1.0, 0.2, 0.1], [0.2, 1.0, 0.3], [0.1, 0.3, 1.0]], 0.03)
+    f7 = [K.GbmFactor(f"S{i}", 100.0, 0.03, 0.2) for i in range(7)]
+    b7 = K.correlated_gbm(f7, [[1.0 if i == j else 0.25 for j in range(7)] for i in range(7)], 0.03)
+    b7_pay = K.scale((sum((K.spot(f"S{i}") for i in range(1, 7)), K.spot("S0")) / 7.0 - 100.0).clip(0.0),
+                     K.one("USD")) @ K.at(1.0)
+    many = K.european_call("X", 90.0, 1.0)
+    for k in range(95, 160, 5):  # a program that needs more than 16 registers
+        many = many + K.when(K.spot("X") >= float(k), K.one("USD")) + K.european_call("X", float(k), 1.0)
+    return [
+        (K.european_call("X", 100.0, 1.0), g, 0, 1, 50),                        # no paths
+        (K.european_call("X", 100.0, 1.0), g, 1, 2, 50),                        # one path
+        (K.scale(K.spot("A") - K.spot("C"), K.one("USD")) @ K.at(0.5), b3, 5000, 3, 12),
+        (b7_pay, b7, 3000, 4, 12),                                              # 7 assets: per-contract path
+        (many, g, 2000, 5, 50),                                                 # > 16 registers: per-contract
+        (K.up_and_out_call("X", 100.0, 130.0, 1.0), h, 7000, 6, 100),
+        (K.down_and_out_call("X", 100.0, 80.0, 1.0), g, 6000, 7, 252),
+        (K.anytime(K.spot("X") >= 115.0, K.one("USD")) @ K.at(0.1), g, 5000, 8, 50),
+    ]
+
+
+def price_each(jobs):
+    out = []
+    for job in jobs:
+        con, model, n, seed, spy = job[:5]
+        grid = job[5] if len(job) > 5 else None
+        out.append(con.price(model, n_paths=n, seed=seed, steps_per_year=spy, grid=grid))
+    return out
+
+
+def assert_identical(got, want):
+    for i, (a, b) in enumerate(zip(got, want)):
+        assert a.n_paths == b.n_paths, i
+        assert a.price == b.price, (i, a.price, b.price)
+        assert a.std_error == b.std_error, (i, a.std_error, b.std_error)
+
+
+def test_portfolio_equals_per_contract():
+    jobs = small_book() + edge_jobs()
+    assert_identical(K.price_portfolio(jobs), price_each(jobs))
+
+
+def test_portfolio_tile_configs_and_graph_replay():
+    eng = N.engine(0)
+    jobs = small_book(24)
+    want = price_each(jobs)
+    try:
+        for bpc, chunk in ((0, 1), (2, 4), (8, 1), (1, 64)):
+            eng.set_option("portfolio_blocks_per_cu", bpc)
+            eng.set_option("portfolio_chunk", chunk)
+            assert_identical(K.price_portfolio(jobs), want)
+            assert_identical(K.price_portfolio(jobs), want)  # replay of the captured graph
+    finally:
+        eng.set_option("portfolio_blocks_per_cu", 0)
+        eng.set_option("portfolio_chunk", 1)
+    other = small_book(10, paths=(2048,))  # another shape: a new graph
+    assert_identical(K.price_portfolio(other), price_each(other))
+    for bad in (("portfolio_chunk", 0), ("portfolio_blocks_per_cu", -1)):
+        with pytest.raises(N.KontractError):
+            eng.set_option(*bad)
+
+
+def test_portfolio_errors_like_a_loop():
+    g = K.GBM(100.0, 0.2, 0.05, asset="X")
+    ok = (K.european_call("X", 100.0, 1.0), g, 1000, 1, 50)
+    unknown = (K.european_call("Y", 100.0, 1.0), g, 1000, 1, 50)
+    bad_model = (K.european_call("X", 100.0, 1.0), K.heston(100, 0.04, 2, 0.04, 0.3, -1.5, 0.05, asset="X"),
+                 1000, 1, 50)
+    with pytest.raises(N.KontractError) as e:
+        K.price_portfolio([ok, unknown, ok])
+    assert e.value.kind == "UnknownAsset"
+    with pytest.raises(N.KontractError) as e:
+        K.price_portfolio([ok, bad_model, unknown])
+    assert e.value.kind == "MalformedContract"
+    assert K.price_portfolio([]) == []

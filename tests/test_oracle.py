@@ -216,6 +216,29 @@ def test_temporal_known_answers():
     assert e.value.code == 6
 
 
+def test_average_at_extension_known_answers():
+    """AverageAt (extension, no reference counterpart): mean over the fixings
+    on or before the evaluation step.  Pinned by hand-computed means, by its
+    reduction to the reference Average when every grid point is a fixing, and
+    by its error cases."""
+    t5 = [0.0, 0.25, 0.5, 0.75, 1.0]
+    sp = {"S": [1.0, 2.0, 4.0, 8.0, 16.0]}
+    avg_at = lambda dates: K.average_at(K.spot("S"), dates)
+    assert ev(avg_at([0.25, 0.75]), 4, t5, sp) == (2.0 + 8.0) / 2
+    assert ev(avg_at([0.25, 0.75]), 2, t5, sp) == 2.0  # only the fixing at 0.25 is past
+    assert ev(avg_at([1.0]), 4, t5, sp) == 16.0
+    for t in range(5):  # every grid point up to now is a fixing: the reference Average
+        assert ev(avg_at(t5), t, t5, sp) == ev(K.average(K.spot("S")), t, t5, sp)
+    cases = [([0.25, 0.75], 0, 6),      # no fixing on or before t = 0
+             ([], 4, 6),                # no fixings at all
+             ([0.75, 0.25], 4, 6),      # not increasing
+             ([0.3], 4, 2)]             # fixing date not on the grid: TimeOutOfRange
+    for dates, t, code in cases:
+        with pytest.raises(O.OracleError) as e:
+            ev(avg_at(dates), t, t5, sp)
+        assert e.value.code == code, dates
+
+
 def test_cashflow_semantics_on_hand_built_path():
     times = [0.0, 0.5, 1.0, 1.5]
     spots = {"X": [100.0, 120.0, 90.0, 130.0]}
