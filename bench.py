@@ -125,7 +125,10 @@ def main():
             "kernel": "path_kernel<GBM, 7 CRN scenarios>", "avg_launch_ms": round(avg_launch_s * 1e3, 4),
             "launches": k_launches, "lane_ops_per_path_step": lane_ops,
             "path_steps_per_launch": path_steps_per_launch,
-            "peak_basis": "FP32 vector 157.3 TFLOP/s = 78.6e12 lane-ops/s"}
+            "peak_basis": "FP32 vector 157.3 TFLOP/s = 78.6e12 FMA lane-ops/s (packed); no MFMA: no step is a "
+                          "dense contraction, the work is Philox integer ops, inverse normals and FMAs",
+            "frac_of_nonpacked_issue": round(achieved / R.PEAK_NONPACKED_LANE_OPS * 1e12, 4),
+            "nonpacked_issue_peak": round(R.PEAK_NONPACKED_LANE_OPS / 1e12, 3)}
 
     cpu = None
     if not args.no_cpu:

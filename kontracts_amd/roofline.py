@@ -25,6 +25,9 @@ instruction count (rocprof SQ_INSTS_VALU) is reported beside them.
 """
 
 PEAK_LANE_OPS = 256 * 128 * 2.4e9  # 78.6e12
+# Issue ceiling without packed FP32 (256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz):
+# the rate one VALU instruction per lane per cycle reaches.
+PEAK_NONPACKED_LANE_OPS = 256 * 64 * 2.4e9  # 39.3e12
 PEAK_FP32_TFLOPS = 157.3
 NORMAL = 30
 
