@@ -81,8 +81,11 @@ typedef struct kt_model {
 
 /* McConfig (src/pricer.rs:30-54).  grid/n_grid, when set, replace the compiled
  * time grid, which is the reference's price_on_paths route on an explicit grid
- * (src/pricer.rs:230-241).  antithetic/control_variate must be 0 (the variance
- * reduction branch, src/pricer.rs:117-228, is not on this engine's path). */
+ * (src/pricer.rs:230-241).  antithetic/control_variate are
+ * VarianceReductionConfig (src/variance_reduction.rs:35-43); kt_price,
+ * kt_greeks_gbm and kt_price_portfolio apply them as price_gbm does,
+ * kt_price_batch ignores them as price_batch_gbm does, and the other entry
+ * points return KT_ERR_UNSUPPORTED when they are set. */
 typedef struct kt_mc_config {
   uint64_t n_paths;
   uint64_t seed;
@@ -129,7 +132,16 @@ const char* kt_last_error_global(void);
 /* price_gbm (src/pricer.rs:97-115): compile, build the grid
  * (time_grid_forced with the model's fine-grid requirement), simulate, evaluate
  * cashflows per path and summarize.  Replaces PyContract::price
- * (src/bindings.rs:350-379) underneath the Python API. */
+ * (src/bindings.rs:350-379) underneath the Python API.
+ * With variance reduction (src/pricer.rs:117-228):
+ *  - antithetic, GBM: n_paths / 2 pairs (Z, -Z); the result summarizes the
+ *    pair means (n_paths of the result = pairs).  Other models ignore it.
+ *  - control_variate: beta = 1 on an at-the-money call
+ *    when(at(T), scale(max(S - s0, 0), one(USD))) on the model's first asset,
+ *    T = last grid date, corrected by its Black-Scholes value at
+ *    (s0, sigma) of a GBM, (100, 0.2) for the other models.
+ *  - both (GBM): price = mean_pairs(target) - (mean_pairs(control) - BS),
+ *    sample_std of the target's pair means, std_error over 2 x pairs. */
 kt_status kt_price(kt_engine* engine, const char* contract_json, const kt_model* model,
                    const kt_mc_config* cfg, kt_price_result* out);
 
@@ -141,7 +153,9 @@ kt_status kt_price_batch(kt_engine* engine, const char* const* contract_jsons, u
 
 /* greeks_gbm (src/greeks.rs:60-98): CRN bump-and-reprice of a one-asset GBM with
  * drift = cfg->rate; the seven repricings share one simulation of the
- * normals on the GPU.  bumps == NULL means BumpSizes::default(). */
+ * normals on the GPU.  bumps == NULL means BumpSizes::default().  With
+ * variance reduction each repricing is a variance-reduced kt_price at the same
+ * seed; std_error then holds the price's only (the others are NaN). */
 kt_status kt_greeks_gbm(kt_engine* engine, const char* contract_json, const char* asset, double s0,
                         double sigma, const kt_mc_config* cfg, const kt_bumps* bumps, kt_greeks* out);
 
@@ -222,6 +236,11 @@ kt_status kt_engine_set_option(kt_engine* engine, const char* name, int64_t valu
 kt_status kt_plan(const char* contract_json, uint64_t steps_per_year, int32_t force_fine, double* grid,
                   uint64_t* n_grid, double* dates, uint64_t* n_dates, uint64_t cap, int32_t* needs_fine_grid,
                   double* horizon, char* assets, uint64_t assets_cap);
+
+/* black_scholes_call (src/numerics.rs:63-71) with the reference's
+ * Abramowitz-Stegun 7.1.26 normal CDF (src/numerics.rs:22-41): the value the
+ * control variate corrects with.  Host only. */
+double kt_black_scholes_call(double s, double k, double t, double r, double sigma);
 
 /* Text listing of the per-step op program the engine would run for a
  * contract on a grid (diagnostics; host only). */

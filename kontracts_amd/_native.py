@@ -153,6 +153,7 @@ _SIGNATURES = {
          C.c_uint64, _P(C.c_int32), _P(C.c_double), C.c_char_p, C.c_uint64],
     ),
     "kt_describe_program": (C.c_int, [C.c_char_p, _P(KtModel), _P(KtMcConfig), C.c_char_p, C.c_uint64]),
+    "kt_black_scholes_call": (C.c_double, [C.c_double, C.c_double, C.c_double, C.c_double, C.c_double]),
     "kt_debug_philox": (
         C.c_int, [C.c_void_p, _P(C.c_uint32), _P(C.c_uint32), C.c_uint64, _P(C.c_uint32)]),
     "kt_debug_normals": (C.c_int, [C.c_void_p, _P(C.c_uint32), C.c_uint64, _P(C.c_float)]),

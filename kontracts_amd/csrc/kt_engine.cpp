@@ -284,7 +284,7 @@ std::vector<KGroup> knockout_tables(const Program& prog, const std::vector<Model
 std::vector<KStats> run_program_scen(kt_engine* E, const Program& prog, const std::vector<ModelDesc>& models,
                                      const std::vector<double>& rates, const std::vector<std::vector<double>>& stat_w,
                                      const std::vector<double>& grid, uint64_t n_paths, uint64_t seed,
-                                     double* pv_host, float* z_host, double* pv_keep) {
+                                     double* pv_host, float* z_host, double* pv_keep, bool anti, int combine) {
   const int SC = (int)models.size();
   const ModelDesc& md = models[0];
   int stride = 0;
@@ -325,8 +325,11 @@ std::vector<KStats> run_program_scen(kt_engine* E, const Program& prog, const st
   int K = (int)md.names.size();
   bool debug = pv_host || z_host || pv_keep;
   int ns = 0, nout = 0;
-  if (!path_variant_exists(kind, K, prog.n_slots, prog.n_outputs, debug, SC, &ns, &nout))
+  if (!path_variant_exists(kind, K, prog.n_slots, prog.n_outputs, debug, SC, &ns, &nout, anti))
     raise(KT_ERR_UNSUPPORTED, "no path kernel variant for this program shape");
+  if (combine < 0 || combine > std::min(prog.n_outputs, kScenarios) || (combine > 0 && (SC != 1 || nout < 2)) ||
+      (combine > 0 && (stat_w.empty() || (int)stat_w[0].size() < combine)))
+    raise(KT_ERR_INVALID_ARGUMENT, "bad output combination");
   if (prog.n_slots > ns) raise(KT_ERR_UNSUPPORTED, "program needs more registers than the kernel variant holds");
   const int nstat = SC == 1 ? nout : kScenarioStats;
   const int n_used = SC == 1 ? prog.n_outputs : kScenarioStats;
@@ -372,8 +375,13 @@ std::vector<KStats> run_program_scen(kt_engine* E, const Program& prog, const st
     A.s0_scen[sc] = (float)models[sc].s0[0];
     A.df_index[sc] = df_index[sc];
   }
-  for (size_t j = 0; j < stat_w.size() && j < (size_t)kScenarioStats; ++j)
-    for (int sc = 0; sc < SC && sc < kScenarios; ++sc) A.stat_w[j][sc] = stat_w[j][sc];
+  if (combine > 0) {
+    for (int o = 0; o < combine; ++o) A.stat_w[0][o] = stat_w[0][o];
+    A.combine = combine;
+  } else {
+    for (size_t j = 0; j < stat_w.size() && j < (size_t)kScenarioStats; ++j)
+      for (int sc = 0; sc < SC && sc < kScenarios; ++sc) A.stat_w[j][sc] = stat_w[j][sc];
+  }
   A.err = err;
 
   E->prof_begin();
@@ -384,9 +392,9 @@ std::vector<KStats> run_program_scen(kt_engine* E, const Program& prog, const st
     A.partials = partials + b0 * nstat;
     A.pv_out = pv_dev ? pv_dev + b0 * kBlock : nullptr;
     A.z_out = z_dev ? z_dev + b0 * kBlock * grid.size() * M : nullptr;
-    check_hip(launch_path(kind, K, ns, nout, debug, SC, A, (int)nb, E->stream), "path kernel launch");
+    check_hip(launch_path(kind, K, ns, nout, debug, SC, A, (int)nb, E->stream, anti), "path kernel launch");
   }
-  E->prof_end((double)n_paths * (double)(grid.size() > 0 ? grid.size() - 1 : 0) * SC);
+  E->prof_end((double)n_paths * (double)(grid.size() > 0 ? grid.size() - 1 : 0) * SC * (anti ? 2 : 1));
   check_hip(launch_finalize(partials, (int)total_blocks, nstat, n_used, stats, E->stream), "finalize launch");
   std::vector<KStats> out(n_used);
   unsigned err_bits = 0;
@@ -414,10 +422,10 @@ std::vector<KStats> run_program(kt_engine* E, const Program& prog, const ModelDe
 }
 
 // Shared front half of price_gbm: parse, plan, grid, validation.
-PreparedContract prepare(const char* json, const ModelDesc& md, const kt_mc_config* cfg) {
+PreparedContract prepare(const char* json, const ModelDesc& md, const kt_mc_config* cfg, bool allow_vr) {
   if (!cfg) raise(KT_ERR_INVALID_ARGUMENT, "null config");
-  if (cfg->antithetic || cfg->control_variate)
-    raise(KT_ERR_UNSUPPORTED, "variance reduction is not on the GPU engine's path");
+  if (!allow_vr && (cfg->antithetic || cfg->control_variate))
+    raise(KT_ERR_UNSUPPORTED, "variance reduction is not available for this entry point");
   PreparedContract pc;
   pc.con = parse_contract(json);
   Plan plan = compile_plan(*pc.con);
@@ -490,8 +498,14 @@ kt_status kt_price(kt_engine* E, const char* json, const kt_model* model, const 
   if (!E || !out) raise(KT_ERR_INVALID_ARGUMENT, "null engine or output");
   check_hip(hipSetDevice(E->device), "hipSetDevice");
   ModelDesc md = model_from_abi(model);
-  PreparedContract pc = prepare(json, md, cfg);
+  PreparedContract pc = prepare(json, md, cfg, true);
   *out = kt_price_result{};
+  if (cfg->antithetic || cfg->control_variate) {
+    *out = price_vr(E, *pc.con, md, pc.grid, cfg->rate, cfg->n_paths, cfg->seed, cfg->antithetic != 0,
+                    cfg->control_variate != 0);
+    E->err.clear();
+    return KT_OK;
+  }
   if (cfg->n_paths == 0) return KT_OK;
   Program prog = lower({pc.con.get()}, pc.grid, md.names);
   auto st = run_program(E, prog, md, pc.grid, cfg->rate, cfg->n_paths, cfg->seed, nullptr, nullptr);
@@ -527,8 +541,8 @@ kt_status kt_price_batch(kt_engine* E, const char* const* jsons, uint64_t n_cont
   if (!E) raise(KT_ERR_INVALID_ARGUMENT, "null engine");
   if (n_contracts == 0) return KT_OK;
   if (!jsons || !out || !cfg) raise(KT_ERR_INVALID_ARGUMENT, "null argument");
-  if (cfg->antithetic || cfg->control_variate)
-    raise(KT_ERR_UNSUPPORTED, "variance reduction is not on the GPU engine's path");
+  // Like price_batch_gbm, the batch simulates plain paths whatever the
+  // variance-reduction flags say (src/pricer.rs:250-294 never reads them).
   check_hip(hipSetDevice(E->device), "hipSetDevice");
   ModelDesc md = model_from_abi(model);
   // Merged plan (src/pricer.rs:260-278).
@@ -735,6 +749,10 @@ kt_status kt_engine_kernel_time(kt_engine* E, double* total_ms, uint64_t* n_laun
     E->prof_path_steps = 0.0;
   }
   return KT_OK;
+}
+
+double kt_black_scholes_call(double s, double k, double t, double r, double sigma) {
+  return bs_call_reference(s, k, t, r, sigma);
 }
 
 kt_status kt_plan(const char* json, uint64_t spy, int32_t force_fine, double* grid, uint64_t* n_grid, double* dates,

@@ -72,7 +72,9 @@ int set_error(kt_engine* E, int code, const std::string& msg);
 int lowest_error_code(unsigned bits);
 const char* error_text(int code);
 
-PreparedContract prepare(const char* json, const ModelDesc& md, const kt_mc_config* cfg);
+// allow_vr: the caller handles cfg's variance-reduction flags; otherwise
+// they are rejected.
+PreparedContract prepare(const char* json, const ModelDesc& md, const kt_mc_config* cfg, bool allow_vr = false);
 // pv_keep (device, n_paths doubles, nullable): receives each path's pv and
 // stays on the device.
 std::vector<KStats> run_program(kt_engine* E, const Program& prog, const ModelDesc& md,
@@ -89,7 +91,8 @@ void fill_model_args(PathArgs& A, const ModelDesc& md, uint64_t seed);
 std::vector<KStats> run_program_scen(kt_engine* E, const Program& prog, const std::vector<ModelDesc>& models,
                                      const std::vector<double>& rates, const std::vector<std::vector<double>>& stat_w,
                                      const std::vector<double>& grid, uint64_t n_paths, uint64_t seed,
-                                     double* pv_host, float* z_host, double* pv_keep = nullptr);
+                                     double* pv_host, float* z_host, double* pv_keep = nullptr, bool anti = false,
+                                     int combine = 0);
 
 // Implemented in kt_greeks.cpp / kt_lsm.cpp / kt_portfolio.cpp.
 void greeks_gbm_crn(kt_engine* E, const char* json, const char* asset, double s0, double sigma,
@@ -105,5 +108,11 @@ void price_american(kt_engine* E, const char* json, const double* dates, uint64_
                     const kt_mc_config* cfg, int n_basis, kt_price_result* out, double* pv_host = nullptr,
                     double* coef_host = nullptr);
 void price_portfolio(kt_engine* E, const kt_job* jobs, uint64_t n_jobs, kt_price_result* out);
+// kt_vr.cpp: price_gbm with variance reduction (antithetic pairs and/or the
+// at-the-money call control variate).
+kt_price_result price_vr(kt_engine* E, const Con& con, const ModelDesc& md, const std::vector<double>& grid,
+                         double rate, uint64_t n_paths, uint64_t seed, bool antithetic, bool control_variate);
+// The reference's Black-Scholes call (Abramowitz-Stegun normal CDF).
+double bs_call_reference(double s, double k, double t, double r, double sigma);
 
 }  // namespace kt

@@ -16,8 +16,6 @@ namespace kt {
 // the seven prices, and whose spread gives each Greek's standard error.
 void greeks_gbm_crn(kt_engine* E, const char* json, const char* asset, double s0, double sigma,
                     const kt_mc_config* cfg, const kt_bumps& h, kt_greeks* out) {
-  if (cfg->antithetic || cfg->control_variate)
-    raise(KT_ERR_UNSUPPORTED, "variance reduction is not on the GPU engine's path");
   auto con = parse_contract(json);
   Plan plan = compile_plan(*con);
   std::vector<double> grid = cfg->grid && cfg->n_grid ? std::vector<double>(cfg->grid, cfg->grid + cfg->n_grid)
@@ -40,6 +38,27 @@ void greeks_gbm_crn(kt_engine* E, const char* json, const char* asset, double s0
   for (const ModelDesc& m : scen) validate_model(m);
   validate_grid(grid);
   *out = kt_greeks{};
+  const double nan = std::numeric_limits<double>::quiet_NaN();
+  if (cfg->antithetic || cfg->control_variate) {
+    // Each repricing is a price_gbm with the caller's variance reduction
+    // (src/greeks.rs:68-76): seven variance-reduced prices at the same seed.
+    double p[7];
+    double se0 = 0.0;
+    for (int i = 0; i < 7; ++i) {
+      kt_price_result r = price_vr(E, *con, scen[i], grid, rates[i], cfg->n_paths, cfg->seed, cfg->antithetic != 0,
+                                   cfg->control_variate != 0);
+      p[i] = r.price;
+      if (i == 0) se0 = r.std_error;
+    }
+    out->price = p[0];
+    out->delta = (p[1] - p[2]) / (2.0 * h.spot);
+    out->gamma = (p[1] - 2.0 * p[0] + p[2]) / (h.spot * h.spot);
+    out->vega = (p[3] - p[4]) / (2.0 * h.vol);
+    out->rho = (p[5] - p[6]) / (2.0 * h.rate);
+    out->std_error[0] = se0;
+    for (int i = 1; i < 5; ++i) out->std_error[i] = nan;
+    return;
+  }
   if (cfg->n_paths == 0) return;
   Program prog = lower({con.get()}, grid, {asset});
   int ns = 0, nout = 0;
@@ -74,7 +93,6 @@ void greeks_gbm_crn(kt_engine* E, const char* json, const char* asset, double s0
   out->gamma = (p[1] - 2.0 * p[0] + p[2]) / (h.spot * h.spot);
   out->vega = (p[3] - p[4]) / (2.0 * h.vol);
   out->rho = (p[5] - p[6]) / (2.0 * h.rate);
-  const double nan = std::numeric_limits<double>::quiet_NaN();
   out->std_error[0] = se0;
   for (int i = 1; i < 5; ++i) out->std_error[i] = nan;
 }

@@ -110,8 +110,14 @@ __device__ __forceinline__ void fast_group(const PA& A, const KGroup gd, int g, 
 // the per-contract path kernel and the persistent portfolio kernel.
 // r: the op program's register file, owned by the caller so that callers
 // with several variants keep one register file for all of them.
-template <int MODEL, int K, int NS, int NOUT, int SC, int NSTAT, bool DEBUG, class PA>
+// ANTI: each lane prices an antithetic pair (src/variance_reduction.rs:76-130):
+// the path on the stream's normals, then the path on their negations, and
+// keeps 0.5 * (pv_base + pv_anti) (src/variance_reduction.rs:196-201).
+// Complementing every bit of a Philox word maps the uniform u of its cell to
+// 1 - u, so normal_from_word(~w) == -normal_from_word(w) exactly.
+template <int MODEL, int K, int NS, int NOUT, int SC, int NSTAT, bool DEBUG, bool ANTI = false, class PA>
 __device__ __forceinline__ void path_block(const PA& A, uint32_t blk, float (&r)[SC][NS]) {
+  static_assert(!ANTI || (SC == 1 && !DEBUG), "antithetic pairs: one scenario, no debug output");
   constexpr int M = MODEL == MK_GBM ? 1 : (MODEL == MK_HESTON ? 2 : K);
   const uint64_t local = (uint64_t)blk * kBlock + threadIdx.x;
   const bool live = local < A.n_paths;
@@ -119,7 +125,11 @@ __device__ __forceinline__ void path_block(const PA& A, uint32_t blk, float (&r)
   const uint32_t p_lo = (uint32_t)path, p_hi = (uint32_t)(path >> 32);
 
   double pv[SC][NOUT];
+  double pair[NOUT];  // ANTI: the base path's pv while the antithetic one runs
   float x[SC][K];
+#pragma unroll 1
+  for (int rep = 0; rep < (ANTI ? 2 : 1); ++rep) {
+  const uint32_t flip = (ANTI && rep) ? 0xffffffffu : 0u;
   FOR_SC {
 #pragma unroll
     for (int i = 0; i < NS; ++i) r[sc][i] = 0.0f;
@@ -173,10 +183,10 @@ __device__ __forceinline__ void path_block(const PA& A, uint32_t blk, float (&r)
 #pragma unroll
     for (int bk = 0; bk < M; ++bk) {
       const uint4 q = philox4x32_10((uint32_t)(g * M + bk), p_lo, p_hi, 0u, A.seed_lo, A.seed_hi);
-      z[4 * bk + 0] = normal_from_word(q.x);
-      z[4 * bk + 1] = normal_from_word(q.y);
-      z[4 * bk + 2] = normal_from_word(q.z);
-      z[4 * bk + 3] = normal_from_word(q.w);
+      z[4 * bk + 0] = normal_from_word(q.x ^ flip);
+      z[4 * bk + 1] = normal_from_word(q.y ^ flip);
+      z[4 * bk + 2] = normal_from_word(q.z ^ flip);
+      z[4 * bk + 3] = normal_from_word(q.w ^ flip);
     }
     const int k0 = 4 * g;
     if (gd.kind != G_GENERIC && gd.kind != G_NONE && gd.slot != mslot) {
@@ -231,6 +241,15 @@ __device__ __forceinline__ void path_block(const PA& A, uint32_t blk, float (&r)
     }
   }
   flush();
+  if constexpr (ANTI) {
+#pragma unroll
+    for (int j = 0; j < NOUT; ++j) pair[j] = rep ? 0.5 * (pair[j] + pv[0][j]) : pv[0][j];
+  }
+  }  // rep
+  if constexpr (ANTI) {
+#pragma unroll
+    for (int j = 0; j < NOUT; ++j) pv[0][j] = pair[j];
+  }
 
   if constexpr (DEBUG) {
     if (live && A.pv_out) A.pv_out[local] = pv[0][0];
@@ -246,6 +265,18 @@ __device__ __forceinline__ void path_block(const PA& A, uint32_t blk, float (&r)
       double t = 0.0;
       FOR_SC t += A.stat_w[j][sc] * pv[sc][0];
       y[j] = t;
+    }
+  }
+  // One scenario, several contracts: output 0 may be the combination
+  // sum_{o < combine} stat_w[0][o] pv_o of the first outputs (a control
+  // variate's per-path pv_target - pv_control, src/variance_reduction.rs:226-231).
+  if constexpr (SC == 1 && NOUT > 1) {
+    if (A.combine > 0) {
+      double t = 0.0;
+#pragma unroll
+      for (int o = 0; o < (NOUT < kScenarios ? NOUT : kScenarios); ++o)
+        if (o < A.combine) t += A.stat_w[0][o] * pv[0][o];
+      y[0] = t;
     }
   }
   // Block statistics per output: wave shifted sums, then Chan across waves.
@@ -265,10 +296,10 @@ __device__ __forceinline__ void path_block(const PA& A, uint32_t blk, float (&r)
   }
 }
 
-template <int MODEL, int K, int NS, int NOUT, int SC, int NSTAT, bool DEBUG>
+template <int MODEL, int K, int NS, int NOUT, int SC, int NSTAT, bool DEBUG, bool ANTI>
 __global__ __launch_bounds__(kBlock) void path_kernel(const PathArgs A) {
   float r[SC][NS];
-  path_block<MODEL, K, NS, NOUT, SC, NSTAT, DEBUG>(A, blockIdx.x, r);
+  path_block<MODEL, K, NS, NOUT, SC, NSTAT, DEBUG, ANTI>(A, blockIdx.x, r);
 }
 
 // Deterministic combine of per-block statistics of output j (whole block).
@@ -689,14 +720,25 @@ __global__ __launch_bounds__(kBlock) void lsm_solve_kernel(const double* __restr
 // ---------------------------------------------------------------------------
 namespace {
 
-template <int MODEL, int K, int NS, int NOUT, int SC, int NSTAT, bool DEBUG>
+template <int MODEL, int K, int NS, int NOUT, int SC, int NSTAT, bool DEBUG, bool ANTI = false>
 hipError_t go(const PathArgs& a, int n_blocks, hipStream_t s) {
-  hipLaunchKernelGGL((path_kernel<MODEL, K, NS, NOUT, SC, NSTAT, DEBUG>), dim3(n_blocks), dim3(kBlock), 0, s, a);
+  hipLaunchKernelGGL((path_kernel<MODEL, K, NS, NOUT, SC, NSTAT, DEBUG, ANTI>), dim3(n_blocks), dim3(kBlock), 0, s,
+                     a);
   return hipGetLastError();
 }
 
 template <int MODEL, int K>
-hipError_t go_ns(int ns, int nout, bool debug, const PathArgs& a, int n_blocks, hipStream_t s) {
+hipError_t go_ns(int ns, int nout, bool debug, bool anti, const PathArgs& a, int n_blocks, hipStream_t s) {
+  if constexpr (MODEL == MK_GBM) {
+    if (anti && !debug) {
+      if (ns == 16 && nout == 1) return go<MODEL, K, 16, 1, 1, 1, false, true>(a, n_blocks, s);
+      if (ns == 16 && nout == 8) return go<MODEL, K, 16, 8, 1, 8, false, true>(a, n_blocks, s);
+      if (ns == 32 && nout == 1) return go<MODEL, K, 32, 1, 1, 1, false, true>(a, n_blocks, s);
+      if (ns == 32 && nout == 8) return go<MODEL, K, 32, 8, 1, 8, false, true>(a, n_blocks, s);
+      return hipErrorInvalidValue;
+    }
+  }
+  if (anti) return hipErrorInvalidValue;
   if (debug) return go<MODEL, K, 32, 1, 1, 1, true>(a, n_blocks, s);
   if (ns == 16 && nout == 1) return go<MODEL, K, 16, 1, 1, 1, false>(a, n_blocks, s);
   if (ns == 16 && nout == 8) return go<MODEL, K, 16, 8, 1, 8, false>(a, n_blocks, s);
@@ -708,8 +750,9 @@ hipError_t go_ns(int ns, int nout, bool debug, const PathArgs& a, int n_blocks, 
 }  // namespace
 
 bool path_variant_exists(int model, int k_assets, int n_slots, int n_outputs, bool debug, int n_scen,
-                         int* ns_chosen, int* nout_chosen) {
+                         int* ns_chosen, int* nout_chosen, bool anti) {
   if (model == MK_BASKET ? (k_assets < 2 || k_assets > kMaxAssets) : k_assets != 1) return false;
+  if (anti && (model != MK_GBM || n_scen != 1 || debug)) return false;
   if (n_scen != 1) {
     // CRN scenario kernel: one GBM contract, kScenarios scenarios, 16 registers.
     if (model != MK_GBM || n_scen != kScenarios || n_outputs != 1 || n_slots > 16 || debug) return false;
@@ -732,24 +775,25 @@ bool path_variant_exists(int model, int k_assets, int n_slots, int n_outputs, bo
 }
 
 hipError_t launch_path(int model, int k, int ns, int nout, bool debug, int n_scen, const PathArgs& a, int n_blocks,
-                       hipStream_t s) {
+                       hipStream_t s, bool anti) {
   if (n_scen != 1) {
+    if (anti) return hipErrorInvalidValue;
     if (model == MK_GBM && n_scen == kScenarios && ns == 16 && nout == 1 && !debug)
       return go<MK_GBM, 1, 16, 1, kScenarios, kScenarioStats, false>(a, n_blocks, s);
     return hipErrorInvalidValue;
   }
   switch (model) {
-    case MK_GBM: return go_ns<MK_GBM, 1>(ns, nout, debug, a, n_blocks, s);
-    case MK_HESTON: return go_ns<MK_HESTON, 1>(ns, nout, debug, a, n_blocks, s);
+    case MK_GBM: return go_ns<MK_GBM, 1>(ns, nout, debug, anti, a, n_blocks, s);
+    case MK_HESTON: return go_ns<MK_HESTON, 1>(ns, nout, debug, anti, a, n_blocks, s);
     case MK_BASKET:
       switch (k) {
-        case 2: return go_ns<MK_BASKET, 2>(ns, nout, debug, a, n_blocks, s);
-        case 3: return go_ns<MK_BASKET, 3>(ns, nout, debug, a, n_blocks, s);
-        case 4: return go_ns<MK_BASKET, 4>(ns, nout, debug, a, n_blocks, s);
-        case 5: return go_ns<MK_BASKET, 5>(ns, nout, debug, a, n_blocks, s);
-        case 6: return go_ns<MK_BASKET, 6>(ns, nout, debug, a, n_blocks, s);
-        case 7: return go_ns<MK_BASKET, 7>(ns, nout, debug, a, n_blocks, s);
-        case 8: return go_ns<MK_BASKET, 8>(ns, nout, debug, a, n_blocks, s);
+        case 2: return go_ns<MK_BASKET, 2>(ns, nout, debug, anti, a, n_blocks, s);
+        case 3: return go_ns<MK_BASKET, 3>(ns, nout, debug, anti, a, n_blocks, s);
+        case 4: return go_ns<MK_BASKET, 4>(ns, nout, debug, anti, a, n_blocks, s);
+        case 5: return go_ns<MK_BASKET, 5>(ns, nout, debug, anti, a, n_blocks, s);
+        case 6: return go_ns<MK_BASKET, 6>(ns, nout, debug, anti, a, n_blocks, s);
+        case 7: return go_ns<MK_BASKET, 7>(ns, nout, debug, anti, a, n_blocks, s);
+        case 8: return go_ns<MK_BASKET, 8>(ns, nout, debug, anti, a, n_blocks, s);
       }
   }
   return hipErrorInvalidValue;

@@ -47,6 +47,7 @@ struct PathArgs {
   float s0_scen[kScenarios];      // scenario kernel: spot per scenario
   int df_index[kScenarios];       // discount table of each scenario (0 when SC == 1)
   double stat_w[kScenarioStats][kScenarios];  // scenario kernel: output j = sum_sc w[j][sc] pv_sc
+  int combine;  // one scenario: > 0 makes output 0 sum_{o < combine} stat_w[0][o] pv_o
   KStats* partials;  // [n_blocks][NOUT]
   unsigned* err;     // bit e set <=> error code e raised on a live path
   double* pv_out;    // debug: per-path pv of output 0 (nullable)
@@ -56,10 +57,12 @@ struct PathArgs {
 constexpr int kBlock = 256;
 
 // Path kernel variants available: returns false if no variant matches.
+// anti: each lane prices an antithetic pair (GBM, one scenario, no debug
+// output); n_paths then counts pairs.
 bool path_variant_exists(int model, int k_assets, int n_slots, int n_outputs, bool debug, int n_scen,
-                         int* ns_chosen, int* nout_chosen);
+                         int* ns_chosen, int* nout_chosen, bool anti = false);
 hipError_t launch_path(int model, int k_assets, int ns, int nout, bool debug, int n_scen, const PathArgs& a,
-                       int n_blocks, hipStream_t s);
+                       int n_blocks, hipStream_t s, bool anti = false);
 // Pathwise Greeks.  Supported programs: one flow date (the last grid point),
 // arithmetic ops, and running sums / extremes fed directly by a spot (see
 // classify_adjoint).  Outputs per path: price, then d/d params:

@@ -27,6 +27,7 @@ struct Blob {
 };
 
 struct JobPlan {
+  std::unique_ptr<Con> con;  // kept for variance-reduced jobs
   ModelDesc md;
   std::vector<double> grid;
   Program prog;
@@ -66,11 +67,17 @@ void price_portfolio(kt_engine* E, const kt_job* jobs, uint64_t n_jobs, kt_price
     if (!J.contract_json) raise(KT_ERR_INVALID_ARGUMENT, "null contract in job " + std::to_string(j));
     JobPlan& P = plans[j];
     P.md = model_from_abi(&J.model);
-    PreparedContract pc = prepare(J.contract_json, P.md, &J.cfg);
+    PreparedContract pc = prepare(J.contract_json, P.md, &J.cfg, true);
     P.grid = pc.grid;
+    out[j] = kt_price_result{};
+    if (J.cfg.antithetic || J.cfg.control_variate) {
+      // variance reduction runs its own launches, like a loop's price_gbm
+      P.con = std::move(pc.con);
+      solo.push_back(j);
+      continue;
+    }
     P.prog = lower({pc.con.get()}, P.grid, P.md.names);
     P.variant = variant_of(P.md, P.prog);
-    out[j] = kt_price_result{};
     if (J.cfg.n_paths == 0) {
       out[j] = result_from_stats(KStats{0.0, 0.0, 0.0});
       continue;
@@ -89,6 +96,14 @@ void price_portfolio(kt_engine* E, const kt_job* jobs, uint64_t n_jobs, kt_price
     P.cost = per_point * (double)P.grid.size() + 4.0 * (double)P.prog.ops.size();
     book.push_back(j);
   }
+
+  auto price_solo = [&](uint64_t j) {
+    const JobPlan& P = plans[j];
+    const kt_mc_config& c = jobs[j].cfg;
+    if (P.con)
+      return price_vr(E, *P.con, P.md, P.grid, c.rate, c.n_paths, c.seed, c.antithetic != 0, c.control_variate != 0);
+    return result_from_stats(run_program(E, P.prog, P.md, P.grid, c.rate, c.n_paths, c.seed, nullptr, nullptr)[0]);
+  };
 
   if (!book.empty()) {
     // Work items, longest job first (then by job index), blocks in order.
@@ -236,19 +251,13 @@ void price_portfolio(kt_engine* E, const kt_job* jobs, uint64_t n_jobs, kt_price
       }
     for (uint64_t j : solo) {
       if (j > first_err) break;
-      auto s1 = run_program(E, plans[j].prog, plans[j].md, plans[j].grid, jobs[j].cfg.rate, jobs[j].cfg.n_paths,
-                            jobs[j].cfg.seed, nullptr, nullptr);
-      out[j] = result_from_stats(s1[0]);
+      out[j] = price_solo(j);
     }
     if (first_code) raise(first_code, error_text(first_code) + std::string(" (job ") + std::to_string(first_err) + ")");
     for (size_t i = 0; i < book.size(); ++i) out[book[i]] = result_from_stats(st[i]);
     return;
   }
-  for (uint64_t j : solo) {
-    auto s1 = run_program(E, plans[j].prog, plans[j].md, plans[j].grid, jobs[j].cfg.rate, jobs[j].cfg.n_paths,
-                          jobs[j].cfg.seed, nullptr, nullptr);
-    out[j] = result_from_stats(s1[0]);
-  }
+  for (uint64_t j : solo) out[j] = price_solo(j);
 }
 
 }  // namespace kt

@@ -352,10 +352,14 @@ class Contract:
         return PriceResult(out)
 
     def greeks(self, model: "GBM", n_paths: int = 200_000, seed: int = 42, steps_per_year: int = 50,
-               bumps: tuple[float, float, float] | None = None, device: int | None = None) -> Greeks:
+               bumps: tuple[float, float, float] | None = None, device: int | None = None,
+               antithetic: bool = False, control_variate: bool = False) -> Greeks:
+        """greeks_gbm (src/greeks.rs:60-98).  antithetic / control_variate
+        make every repricing variance-reduced, as a McConfig carrying them
+        does in the reference."""
         if not isinstance(model, GBM):
             raise TypeError("greeks expects a GBM model")
-        cfg = _cfg(n_paths, seed, steps_per_year, model.r, None)
+        cfg = _cfg(n_paths, seed, steps_per_year, model.r, None, antithetic, control_variate)
         b = N.KtBumps(*(bumps or (1e-2, 1e-2, 1e-4)))
         eng = N.engine(device)
         out = N.KtGreeks()
@@ -670,8 +674,10 @@ def _jobs_array(jobs):
     for i, job in enumerate(jobs):
         con, model, n_paths, seed, spy = job[:5]
         grid = job[5] if len(job) > 5 else None
+        anti = bool(job[6]) if len(job) > 6 else False
+        cv = bool(job[7]) if len(job) > 7 else False
         m = _model_struct(model)
-        cfg = _cfg(n_paths, seed, spy, model.rate, grid)
+        cfg = _cfg(n_paths, seed, spy, model.rate, grid, anti, cv)
         js = con.to_json().encode()
         keep += [m, cfg, js]
         arr[i].contract_json = js
@@ -682,9 +688,10 @@ def _jobs_array(jobs):
 
 def price_portfolio(jobs: Sequence[tuple], device: int | None = None) -> list[PriceResult]:
     """Heterogeneous portfolio: jobs are (contract, model, n_paths, seed,
-    steps_per_year[, grid]) tuples, priced in one persistent launch; each
-    result equals ``contract.price(model, n_paths, seed, steps_per_year,
-    grid=grid)``."""
+    steps_per_year[, grid[, antithetic[, control_variate]]]) tuples, priced in
+    one persistent launch; each result equals ``contract.price(model, n_paths,
+    seed, steps_per_year, antithetic, control_variate, grid=grid)``
+    (variance-reduced jobs run their own launches)."""
     jobs = list(jobs)
     if not jobs:
         return []

@@ -949,15 +949,21 @@ static void validate_model(const Model& md) {
 
 // Normal source: either the Philox stream or a caller-provided float array
 // laid out [path][grid point][component].
+// flip = 0xffffffff draws the antithetic path: every Philox word complemented,
+// so u -> 1 - u on the dyadic cell grid and each normal exactly negated.
 struct Normals {
   uint64_t seed;
   const float* fed;
   size_t n_grid;
   int m;
+  uint32_t flip = 0;
   double get(uint64_t path, size_t k, int c) const {
     uint64_t j = (uint64_t)k * (uint64_t)m + (uint64_t)c;
-    if (fed) return (double)fed[((size_t)path * n_grid + k) * (size_t)m + (size_t)c];
-    return inv_norm(uniform_from_word(philox_word(seed, path, j, 0u)));
+    if (fed) {
+      double z = (double)fed[((size_t)path * n_grid + k) * (size_t)m + (size_t)c];
+      return flip ? -z : z;
+    }
+    return inv_norm(uniform_from_word(philox_word(seed, path, j, 0u) ^ flip));
   }
 };
 
@@ -1062,13 +1068,14 @@ static void parallel_paths(uint64_t n, int threads, const std::function<void(uin
 // price_on_paths for a list of contracts over one shared simulation.
 static void price_contracts(std::vector<std::unique_ptr<Con>>& cons, const Model& md,
                             const std::vector<double>& grid, uint64_t n_paths, uint64_t seed, double rate,
-                            const float* fed, int threads, std::vector<std::vector<double>>& pvs) {
+                            const float* fed, int threads, std::vector<std::vector<double>>& pvs,
+                            uint32_t flip = 0) {
   validate_model(md);
   validate_grid(grid);
   for (auto& c : cons) resolve_con(*c, md.names);
   size_t n = grid.size(), K = md.names.size();
   pvs.assign(cons.size(), std::vector<double>(n_paths));
-  Normals nz{seed, fed, n, md.m};
+  Normals nz{seed, fed, n, md.m, flip};
   parallel_paths(n_paths, threads, [&](uint64_t b, uint64_t e) {
     std::vector<double> row(K * n);
     Flows scratch;
@@ -1254,6 +1261,83 @@ int orc_price_batch(const char* const* jsons, uint64_t n_contracts, const orc_mo
     std::vector<std::vector<double>> pvs;
     price_contracts(cons, md, grid, cfg->n_paths, cfg->seed, cfg->rate, nullptr, cfg->threads, pvs);
     for (uint64_t i = 0; i < n_contracts; ++i) out[i] = summarize(pvs[i]);
+    return 0;
+  } catch (KErr& e) {
+    return report(e, err, errcap);
+  }
+}
+
+// Black-Scholes call with the Abramowitz-Stegun 7.1.26 normal CDF
+// (src/numerics.rs:22-41, 63-71): the control variate's correction.
+static double ncdf_as(double x) {
+  double z = x / std::sqrt(2.0);
+  double sign = z < 0.0 ? -1.0 : 1.0, a = std::fabs(z);
+  double t = 1.0 / (1.0 + 0.3275911 * a);
+  double poly = ((((1.061405429 * t - 1.453152027) * t) + 1.421413741) * t - 0.284496736) * t + 0.254829592;
+  return 0.5 * (1.0 + sign * (1.0 - poly * t * std::exp(-a * a)));
+}
+double orc_bs_call_as(double s, double k, double t, double r, double sigma) {
+  if (t <= 0.0 || sigma <= 0.0) return std::max(s - k * std::exp(-r * t), 0.0);
+  double vst = sigma * std::sqrt(t);
+  double d1 = (std::log(s / k) + (r + 0.5 * sigma * sigma) * t) / vst, d2 = d1 - vst;
+  return s * ncdf_as(d1) - k * std::exp(-r * t) * ncdf_as(d2);
+}
+double orc_norm_cdf_as(double x) { return ncdf_as(x); }
+
+// price_gbm with cfg.variance_reduction (src/pricer.rs:117-228,
+// src/variance_reduction.rs:76-235).  Antithetic pairs exist for the one-asset
+// GBM only (src/simulator.rs:132-140, 297-314): pair i is path i of the
+// stream and its negation.  The control is the ATM call on the model's first
+// asset at the grid's last date with beta = 1.
+int orc_price_vr(const char* json, const orc_model* m, const orc_cfg* cfg, int antithetic, int control_variate,
+                 orc_result* res, char* err, int errcap) {
+  try {
+    Model md = make_model(m);
+    std::vector<std::unique_ptr<Con>> cons;
+    cons.push_back(parse_contract(json));
+    auto grid = grid_for(*cons[0], md, cfg);
+    const bool gbm = md.kind == 0;
+    const bool anti = antithetic && gbm;
+    const double s0 = gbm ? md.s0[0] : 100.0, sg = gbm ? md.sigma[0] : 0.2;
+    const double t_mat = grid.empty() ? 1.0 : grid.back();
+    const double bs = orc_bs_call_as(s0, s0, t_mat, cfg->rate, sg);
+    if (control_variate) {
+      char buf[512];
+      snprintf(buf, sizeof buf,
+               "{\"When\":[{\"At\":%.17g},{\"Scale\":[{\"Max\":[{\"Sub\":[{\"Spot\":\"%s\"},{\"Const\":%.17g}]},"
+               "{\"Const\":0.0}]},{\"One\":\"USD\"}]}]}",
+               t_mat, md.names[0].c_str(), s0);
+      cons.push_back(parse_contract(buf));
+    }
+    if (anti) {
+      const uint64_t n_half = cfg->n_paths / 2;
+      std::vector<std::vector<double>> base, neg;
+      price_contracts(cons, md, grid, n_half, cfg->seed, cfg->rate, nullptr, cfg->threads, base, 0u);
+      price_contracts(cons, md, grid, n_half, cfg->seed, cfg->rate, nullptr, cfg->threads, neg, 0xffffffffu);
+      std::vector<std::vector<double>> pair(cons.size(), std::vector<double>(n_half));
+      for (size_t c = 0; c < cons.size(); ++c)
+        for (uint64_t i = 0; i < n_half; ++i) pair[c][i] = 0.5 * (base[c][i] + neg[c][i]);
+      orc_result rt = summarize(pair[0]);
+      if (!control_variate) {
+        *res = rt;
+        return 0;
+      }
+      orc_result rc = summarize(pair[1]);
+      double price = rt.price - 1.0 * (rc.price - bs);
+      uint64_t n = 2 * n_half;
+      double se = rt.sample_std / std::sqrt((double)n);
+      *res = orc_result{price, rt.sample_std, se, price - Z95 * se, price + Z95 * se, n};
+      return 0;
+    }
+    std::vector<std::vector<double>> pvs;
+    price_contracts(cons, md, grid, cfg->n_paths, cfg->seed, cfg->rate, nullptr, cfg->threads, pvs);
+    if (!control_variate) {
+      *res = summarize(pvs[0]);
+      return 0;
+    }
+    std::vector<double> y(cfg->n_paths);
+    for (uint64_t i = 0; i < cfg->n_paths; ++i) y[i] = pvs[0][i] - 1.0 * (pvs[1][i] - bs);
+    *res = summarize(y);
     return 0;
   } catch (KErr& e) {
     return report(e, err, errcap);

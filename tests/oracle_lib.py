@@ -58,9 +58,11 @@ def load():
     L = C.CDLL(ORACLE_LIB)
     d = C.c_double
     for f in ["orc_bs_call", "orc_bs_put", "orc_uo_call", "orc_heston_call", "orc_crr_american", "orc_inv_norm",
-              "orc_uniform_from_word"]:
+              "orc_uniform_from_word", "orc_bs_call_as", "orc_norm_cdf_as"]:
         getattr(L, f).restype = d
     L.orc_bs_call.argtypes = [d] * 5
+    L.orc_bs_call_as.argtypes = [d] * 5
+    L.orc_norm_cdf_as.argtypes = [d]
     L.orc_bs_put.argtypes = [d] * 5
     L.orc_uo_call.argtypes = [d] * 6
     L.orc_heston_call.argtypes = [d] * 9
@@ -131,6 +133,18 @@ def price(json_s, model, c, normals=None, want_pv=False):
     if rc:
         raise OracleError(rc, e.value.decode())
     return res.as_dict(), pv, list(grid[: ng.value])
+
+
+def price_vr(json_s, model, c, antithetic=False, control_variate=False):
+    """price_gbm with a VarianceReductionConfig (src/pricer.rs:117-228)."""
+    L = load()
+    res = OrcResult()
+    e = _err()
+    rc = L.orc_price_vr(json_s.encode(), C.byref(model.s), C.byref(c), int(antithetic), int(control_variate),
+                        C.byref(res), e, 512)
+    if rc:
+        raise OracleError(rc, e.value.decode())
+    return res.as_dict()
 
 
 def price_batch(jsons, model, c):
@@ -244,6 +258,15 @@ def normals(seed, path, first, count):
 
 def bs_call(s, k, t, r, sg):
     return load().orc_bs_call(s, k, t, r, sg)
+
+
+def bs_call_as(s, k, t, r, sg):
+    """The reference's black_scholes_call (Abramowitz-Stegun normal CDF)."""
+    return load().orc_bs_call_as(s, k, t, r, sg)
+
+
+def norm_cdf_as(x):
+    return load().orc_norm_cdf_as(x)
 
 
 def bs_put(s, k, t, r, sg):
