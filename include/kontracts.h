@@ -159,6 +159,17 @@ kt_status kt_price_batch(kt_engine* engine, const char* const* contract_jsons, u
 kt_status kt_greeks_gbm(kt_engine* engine, const char* contract_json, const char* asset, double s0,
                         double sigma, const kt_mc_config* cfg, const kt_bumps* bumps, kt_greeks* out);
 
+/* greek_surface (src/surface.rs:104-138): greeks_gbm at every cell of the
+ * (spots x vols) grid, out[i * n_vols + j] for (spots[i], vols[j]) (the
+ * reference's [spot][vol] matrices; rho and the standard errors ride along).
+ * Every cell uses cfg->seed, as the reference's cells do; all cells run as
+ * one launch, each cell's result equal to kt_greeks_gbm at that point.  An
+ * empty axis returns KT_ERR_INCONSISTENT_PATH; a failing cell fails the
+ * call (the lowest cell index wins). */
+kt_status kt_greek_surface(kt_engine* engine, const char* contract_json, const char* asset, const double* spots,
+                           uint64_t n_spots, const double* vols, uint64_t n_vols, const kt_mc_config* cfg,
+                           const kt_bumps* bumps, kt_greeks* out);
+
 /* CRN bump-and-reprice for any model, the greeks_gbm pattern (central
  * differences, common random numbers; src/greeks.rs:69-89) applied to the
  * model's parameters.  Parameters: GBM {s0, sigma, mu}; HESTON {s0, v0, kappa,

@@ -7,9 +7,11 @@ and ``Contract.price_american`` calls, executed by hand-written gfx950 kernels
 behind the C ABI in include/kontracts.h.  Beyond the reference:
 ``Contract.pathwise_greeks`` / ``Contract.bump_greeks`` (derivatives with
 respect to every model parameter) and ``kontracts_amd.autograd`` (a
-torch.autograd.Function over the pathwise Greeks).
+torch.autograd.Function over the pathwise Greeks).  ``greek_surface``
+follows the reference's Rust ``surface::greek_surface``.
 """
 from ._native import DeviceError, KontractError
+from .surface import GreekSurface, greek_surface
 from .dsl import (
     EUR,
     GBM,
@@ -71,4 +73,5 @@ __all__ = [
     "correlated_gbm", "describe_program", "down_and_out_call", "european_call", "european_put", "forward", "give",
     "heston", "konst", "one", "or_", "plan", "price_batch", "price_portfolio", "running_max", "running_min",
     "scale", "spot", "straddle", "until", "up_and_out_call", "when", "zero", "zero_coupon_bond",
+    "GreekSurface", "greek_surface",
 ]

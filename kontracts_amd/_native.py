@@ -143,6 +143,11 @@ _SIGNATURES = {
          _P(C.c_double), _P(C.c_double), _P(KtPriceResult)],
     ),
     "kt_price_portfolio": (C.c_int, [C.c_void_p, _P(KtJob), C.c_uint64, _P(KtPriceResult)]),
+    "kt_greek_surface": (
+        C.c_int,
+        [C.c_void_p, C.c_char_p, C.c_char_p, _P(C.c_double), C.c_uint64, _P(C.c_double), C.c_uint64,
+         _P(KtMcConfig), _P(KtBumps), _P(KtGreeks)],
+    ),
     "kt_engine_set_profiling": (C.c_int, [C.c_void_p, C.c_int32]),
     "kt_engine_set_option": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int64]),
     "kt_engine_kernel_time": (

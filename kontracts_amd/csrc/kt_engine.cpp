@@ -281,45 +281,57 @@ std::vector<KGroup> knockout_tables(const Program& prog, const std::vector<Model
   return groups;
 }
 
+LaunchTables launch_tables(const Program& prog, const std::vector<ModelDesc>& models,
+                           const std::vector<double>& rates, const std::vector<double>& grid) {
+  const int SC = (int)models.size();
+  const ModelDesc& md = models[0];
+  LaunchTables T;
+  if (SC == 1) {
+    T.tab = model_table(md, grid, &T.stride);
+  } else {
+    if (md.kind != KT_MODEL_GBM) raise(KT_ERR_UNSUPPORTED, "scenario kernel is GBM only");
+    std::vector<std::vector<float>> per;
+    for (const ModelDesc& m : models) per.push_back(model_table(m, grid, &T.stride));
+    T.tab.resize(grid.size() * SC * 2);
+    for (size_t k = 0; k < grid.size(); ++k)
+      for (int sc = 0; sc < SC; ++sc) {
+        T.tab[(k * SC + sc) * 2] = per[sc][k * 2];
+        T.tab[(k * SC + sc) * 2 + 1] = per[sc][k * 2 + 1];
+      }
+    T.stride = 2 * SC;
+  }
+  // discount tables, one per distinct rate
+  std::vector<double> distinct;
+  T.df_index.assign(SC, 0);
+  for (int sc = 0; sc < SC; ++sc) {
+    auto it = std::find(distinct.begin(), distinct.end(), rates[sc]);
+    if (it == distinct.end()) {
+      T.df_index[sc] = (int)distinct.size();
+      distinct.push_back(rates[sc]);
+    } else {
+      T.df_index[sc] = (int)(it - distinct.begin());
+    }
+  }
+  T.df.resize(distinct.size() * grid.size());
+  for (size_t t = 0; t < distinct.size(); ++t)
+    for (size_t k = 0; k < grid.size(); ++k) T.df[t * grid.size() + k] = std::exp(-distinct[t] * grid[k]);
+  T.groups = knockout_tables(prog, models, &T.thr);
+  return T;
+}
+
 std::vector<KStats> run_program_scen(kt_engine* E, const Program& prog, const std::vector<ModelDesc>& models,
                                      const std::vector<double>& rates, const std::vector<std::vector<double>>& stat_w,
                                      const std::vector<double>& grid, uint64_t n_paths, uint64_t seed,
                                      double* pv_host, float* z_host, double* pv_keep, bool anti, int combine) {
   const int SC = (int)models.size();
   const ModelDesc& md = models[0];
-  int stride = 0;
-  std::vector<float> tab;
-  if (SC == 1) {
-    tab = model_table(md, grid, &stride);
-  } else {
-    if (md.kind != KT_MODEL_GBM) raise(KT_ERR_UNSUPPORTED, "scenario kernel is GBM only");
-    std::vector<std::vector<float>> per;
-    for (const ModelDesc& m : models) per.push_back(model_table(m, grid, &stride));
-    tab.resize(grid.size() * SC * 2);
-    for (size_t k = 0; k < grid.size(); ++k)
-      for (int sc = 0; sc < SC; ++sc) {
-        tab[(k * SC + sc) * 2] = per[sc][k * 2];
-        tab[(k * SC + sc) * 2 + 1] = per[sc][k * 2 + 1];
-      }
-    stride = 2 * SC;
-  }
-  // discount tables, one per distinct rate
-  std::vector<double> distinct;
-  std::vector<int> df_index(SC);
-  for (int sc = 0; sc < SC; ++sc) {
-    auto it = std::find(distinct.begin(), distinct.end(), rates[sc]);
-    if (it == distinct.end()) {
-      df_index[sc] = (int)distinct.size();
-      distinct.push_back(rates[sc]);
-    } else {
-      df_index[sc] = (int)(it - distinct.begin());
-    }
-  }
-  std::vector<double> df(distinct.size() * grid.size());
-  for (size_t t = 0; t < distinct.size(); ++t)
-    for (size_t k = 0; k < grid.size(); ++k) df[t * grid.size() + k] = std::exp(-distinct[t] * grid[k]);
-  std::vector<float> thr;
-  std::vector<KGroup> groups = knockout_tables(prog, models, &thr);
+  LaunchTables T = launch_tables(prog, models, rates, grid);
+  const int stride = T.stride;
+  const std::vector<float>& tab = T.tab;
+  const std::vector<double>& df = T.df;
+  const std::vector<int>& df_index = T.df_index;
+  const std::vector<KGroup>& groups = T.groups;
+  const std::vector<float>& thr = T.thr;
 
   int kind = model_kernel_kind(md);
   int K = (int)md.names.size();
@@ -598,6 +610,20 @@ kt_status kt_greeks_gbm(kt_engine* E, const char* json, const char* asset, doubl
   check_hip(hipSetDevice(E->device), "hipSetDevice");
   kt_bumps h = bumps ? *bumps : kt_bumps{1e-2, 1e-2, 1e-4};
   greeks_gbm_crn(E, json, asset, s0, sigma, cfg, h, out);
+  E->err.clear();
+  return KT_OK;
+  KT_GUARD_END(E)
+}
+
+kt_status kt_greek_surface(kt_engine* E, const char* json, const char* asset, const double* spots, uint64_t n_spots,
+                           const double* vols, uint64_t n_vols, const kt_mc_config* cfg, const kt_bumps* bumps,
+                           kt_greeks* out) {
+  KT_GUARD_BEGIN
+  if (!E || !cfg || !asset || !json) raise(KT_ERR_INVALID_ARGUMENT, "null argument");
+  if (!out && n_spots && n_vols) raise(KT_ERR_INVALID_ARGUMENT, "null output");
+  check_hip(hipSetDevice(E->device), "hipSetDevice");
+  kt_bumps h = bumps ? *bumps : kt_bumps{1e-2, 1e-2, 1e-4};
+  greek_surface(E, json, asset, spots, n_spots, vols, n_vols, cfg, h, out);
   E->err.clear();
   return KT_OK;
   KT_GUARD_END(E)

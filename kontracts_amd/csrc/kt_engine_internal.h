@@ -84,6 +84,20 @@ std::vector<KStats> run_program(kt_engine* E, const Program& prog, const ModelDe
 std::vector<KGroup> knockout_tables(const Program& prog, const std::vector<ModelDesc>& models,
                                     std::vector<float>* thr);
 
+// Model-dependent tables of one launch of the path kernel: per-point model
+// table (scenario-interleaved when SC > 1), discount tables per distinct
+// rate, and the knock-out groups with their thresholds.
+struct LaunchTables {
+  std::vector<float> tab;
+  int stride = 0;
+  std::vector<double> df;
+  std::vector<int> df_index;
+  std::vector<KGroup> groups;
+  std::vector<float> thr;
+};
+LaunchTables launch_tables(const Program& prog, const std::vector<ModelDesc>& models,
+                           const std::vector<double>& rates, const std::vector<double>& grid);
+
 // Model constants of the path kernels (spots, Cholesky factor, Heston
 // correlation split) and the Philox key.
 void fill_model_args(PathArgs& A, const ModelDesc& md, uint64_t seed);
@@ -97,6 +111,8 @@ std::vector<KStats> run_program_scen(kt_engine* E, const Program& prog, const st
 // Implemented in kt_greeks.cpp / kt_lsm.cpp / kt_portfolio.cpp.
 void greeks_gbm_crn(kt_engine* E, const char* json, const char* asset, double s0, double sigma,
                     const kt_mc_config* cfg, const kt_bumps& h, kt_greeks* out);
+void greek_surface(kt_engine* E, const char* json, const char* asset, const double* spots, uint64_t n_spots,
+                   const double* vols, uint64_t n_vols, const kt_mc_config* cfg, const kt_bumps& h, kt_greeks* out);
 void greeks_bump(kt_engine* E, const char* json, const kt_model* model, const kt_mc_config* cfg, const double* h,
                  uint64_t n_params, double* out, double* stderr_out);
 void greeks_adjoint(kt_engine* E, const char* json, const kt_model* model, const kt_mc_config* cfg, double* out,

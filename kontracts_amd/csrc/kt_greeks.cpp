@@ -1,6 +1,7 @@
 // kt_greeks.cpp — Greeks by CRN bump-and-reprice (src/greeks.rs:60-98).
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <limits>
 #include <string>
 
@@ -8,6 +9,54 @@
 
 namespace kt {
 
+namespace {
+
+// The seven repricings of greeks_gbm (src/greeks.rs:68-89): base, spot up /
+// down, vol up / down, rate up / down (the drift follows the rate).
+std::vector<ModelDesc> gbm_scenarios(const char* asset, double s0, double sigma, double r, const kt_bumps& h) {
+  auto model = [&](double s, double sg, double rr) {
+    ModelDesc md;
+    md.kind = KT_MODEL_GBM;
+    md.names = {asset};
+    md.s0 = {s};
+    md.mu = {rr};
+    md.sigma = {sg};
+    return md;
+  };
+  return {model(s0, sigma, r),         model(s0 + h.spot, sigma, r), model(s0 - h.spot, sigma, r),
+          model(s0, sigma + h.vol, r), model(s0, sigma - h.vol, r),  model(s0, sigma, r + h.rate),
+          model(s0, sigma, r - h.rate)};
+}
+
+std::vector<double> scenario_rates(double r, const kt_bumps& h) { return {r, r, r, r, r, r + h.rate, r - h.rate}; }
+
+// Output j of the scenario kernel = sum_sc w[j][sc] pv_sc: price, delta,
+// gamma, vega, rho as central differences of the seven present values.
+std::vector<std::vector<double>> scenario_weights(const kt_bumps& h) {
+  const double hs = h.spot, hv = h.vol, hr = h.rate;
+  return {{1, 0, 0, 0, 0, 0, 0},
+          {0, 1 / (2 * hs), -1 / (2 * hs), 0, 0, 0, 0},
+          {-2 / (hs * hs), 1 / (hs * hs), 1 / (hs * hs), 0, 0, 0, 0},
+          {0, 0, 0, 1 / (2 * hv), -1 / (2 * hv), 0, 0},
+          {0, 0, 0, 0, 0, 1 / (2 * hr), -1 / (2 * hr)}};
+}
+
+std::vector<double> greeks_grid(const Con& con, const kt_mc_config* cfg) {
+  Plan plan = compile_plan(con);
+  return cfg->grid && cfg->n_grid ? std::vector<double>(cfg->grid, cfg->grid + cfg->n_grid)
+                                  : time_grid_forced(plan, cfg->steps_per_year, false);
+}
+
+void fill_greeks(const std::vector<KStats>& st, size_t first, kt_greeks* out) {
+  double* vals[5] = {&out->price, &out->delta, &out->gamma, &out->vega, &out->rho};
+  for (int j = 0; j < 5; ++j) {
+    kt_price_result pr = result_from_stats(st[first + j]);
+    *vals[j] = pr.price;
+    out->std_error[j] = pr.std_error;
+  }
+}
+
+}  // namespace
 
 // greeks_gbm: drift = rate, seven repricings at the same seed.  The seven
 // scenarios run in one launch that shares every normal (common random
@@ -17,24 +66,10 @@ namespace kt {
 void greeks_gbm_crn(kt_engine* E, const char* json, const char* asset, double s0, double sigma,
                     const kt_mc_config* cfg, const kt_bumps& h, kt_greeks* out) {
   auto con = parse_contract(json);
-  Plan plan = compile_plan(*con);
-  std::vector<double> grid = cfg->grid && cfg->n_grid ? std::vector<double>(cfg->grid, cfg->grid + cfg->n_grid)
-                                                      : time_grid_forced(plan, cfg->steps_per_year, false);
-  auto model = [&](double s, double sg, double r) {
-    ModelDesc md;
-    md.kind = KT_MODEL_GBM;
-    md.names = {asset};
-    md.s0 = {s};
-    md.mu = {r};
-    md.sigma = {sg};
-    return md;
-  };
+  std::vector<double> grid = greeks_grid(*con, cfg);
   const double r = cfg->rate;
-  std::vector<ModelDesc> scen = {model(s0, sigma, r),         model(s0 + h.spot, sigma, r),
-                                 model(s0 - h.spot, sigma, r), model(s0, sigma + h.vol, r),
-                                 model(s0, sigma - h.vol, r),  model(s0, sigma, r + h.rate),
-                                 model(s0, sigma, r - h.rate)};
-  std::vector<double> rates = {r, r, r, r, r, r + h.rate, r - h.rate};
+  std::vector<ModelDesc> scen = gbm_scenarios(asset, s0, sigma, r, h);
+  std::vector<double> rates = scenario_rates(r, h);
   for (const ModelDesc& m : scen) validate_model(m);
   validate_grid(grid);
   *out = kt_greeks{};
@@ -63,20 +98,9 @@ void greeks_gbm_crn(kt_engine* E, const char* json, const char* asset, double s0
   Program prog = lower({con.get()}, grid, {asset});
   int ns = 0, nout = 0;
   if (path_variant_exists(MK_GBM, 1, prog.n_slots, 1, false, kScenarios, &ns, &nout)) {
-    const double hs = h.spot, hv = h.vol, hr = h.rate;
-    std::vector<std::vector<double>> w = {
-        {1, 0, 0, 0, 0, 0, 0},
-        {0, 1 / (2 * hs), -1 / (2 * hs), 0, 0, 0, 0},
-        {-2 / (hs * hs), 1 / (hs * hs), 1 / (hs * hs), 0, 0, 0, 0},
-        {0, 0, 0, 1 / (2 * hv), -1 / (2 * hv), 0, 0},
-        {0, 0, 0, 0, 0, 1 / (2 * hr), -1 / (2 * hr)}};
-    auto st = run_program_scen(E, prog, scen, rates, w, grid, cfg->n_paths, cfg->seed, nullptr, nullptr);
-    double* vals[5] = {&out->price, &out->delta, &out->gamma, &out->vega, &out->rho};
-    for (int j = 0; j < 5; ++j) {
-      kt_price_result pr = result_from_stats(st[j]);
-      *vals[j] = pr.price;
-      out->std_error[j] = pr.std_error;
-    }
+    auto st = run_program_scen(E, prog, scen, rates, scenario_weights(h), grid, cfg->n_paths, cfg->seed, nullptr,
+                               nullptr);
+    fill_greeks(st, 0, out);
     return;
   }
   // Programs too large for the scenario kernel: seven launches at the same seed.
@@ -95,6 +119,131 @@ void greeks_gbm_crn(kt_engine* E, const char* json, const char* asset, double s0
   out->rho = (p[5] - p[6]) / (2.0 * h.rate);
   out->std_error[0] = se0;
   for (int i = 1; i < 5; ++i) out->std_error[i] = nan;
+}
+
+// greek_surface (src/surface.rs:104-138): greeks_gbm at every (spot, vol)
+// cell, row-major [spot][vol].  Every cell is the seven-scenario CRN launch of
+// greeks_gbm at the same seed; all cells run in one grid (blockIdx.y = cell)
+// and one finalize, so a cell's result equals greeks_gbm at that point.
+// Variance-reduced configurations and programs beyond the scenario kernel go
+// cell by cell through greeks_gbm.
+void greek_surface(kt_engine* E, const char* json, const char* asset, const double* spots, uint64_t n_spots,
+                   const double* vols, uint64_t n_vols, const kt_mc_config* cfg, const kt_bumps& h, kt_greeks* out) {
+  if (n_spots == 0 || n_vols == 0) raise(KT_ERR_INCONSISTENT_PATH, "empty surface grid");
+  if (!spots || !vols) raise(KT_ERR_INVALID_ARGUMENT, "null surface axis");
+  if (n_spots > (1u << 20) || n_vols > (1u << 20) || n_spots * n_vols > (1u << 24))
+    raise(KT_ERR_INVALID_ARGUMENT, "surface too large");
+  const uint64_t n_cells = n_spots * n_vols;
+  auto con = parse_contract(json);
+  std::vector<double> grid = greeks_grid(*con, cfg);
+  const double r = cfg->rate;
+  const std::vector<double> rates = scenario_rates(r, h);
+  std::vector<std::vector<ModelDesc>> scen(n_cells);
+  for (uint64_t c = 0; c < n_cells; ++c) {
+    scen[c] = gbm_scenarios(asset, spots[c / n_vols], vols[c % n_vols], r, h);
+    for (const ModelDesc& m : scen[c]) validate_model(m);
+  }
+  validate_grid(grid);
+  for (uint64_t c = 0; c < n_cells; ++c) out[c] = kt_greeks{};
+  Program prog = lower({con.get()}, grid, {asset});
+  const uint64_t N = cfg->n_paths;
+  const uint64_t nb = (N + kBlock - 1) / kBlock;
+  int ns = 0, nout = 0;
+  const bool fast = !cfg->antithetic && !cfg->control_variate && N > 0 && nb <= (1u << 20) &&
+                    path_variant_exists(MK_GBM, 1, prog.n_slots, 1, false, kScenarios, &ns, &nout) && ns == 16;
+  if (!fast) {
+    for (uint64_t c = 0; c < n_cells; ++c)
+      greeks_gbm_crn(E, json, asset, spots[c / n_vols], vols[c % n_vols], cfg, h, &out[c]);
+    return;
+  }
+  std::vector<KOp> ops = prog.ops;
+  if (ops.empty()) ops.push_back(KOp{OP_NOP, 0, 0, 0, 0.0f});
+  const std::vector<std::vector<double>> w = scenario_weights(h);
+  auto a256 = [](size_t b) { return (b + 255) & ~(size_t)255; };
+  // Cells per launch: within the grid's y limit and ~2 GB of partials.
+  const uint64_t per_launch = std::min<uint64_t>(65535, std::max<uint64_t>(1, (1ull << 24) / nb));
+  for (uint64_t c0 = 0; c0 < n_cells; c0 += per_launch) {
+    const uint64_t nc = std::min(per_launch, n_cells - c0);
+    std::vector<LaunchTables> T(nc);
+    for (uint64_t i = 0; i < nc; ++i) T[i] = launch_tables(prog, scen[c0 + i], rates, grid);
+    // Arena: [cells' PathArgs][ops][steps][init][df][per cell: groups, tab, thr]
+    size_t off = a256(nc * sizeof(PathArgs));
+    const size_t o_ops = off;
+    off += a256(ops.size() * sizeof(KOp));
+    const size_t o_steps = off;
+    off += a256(prog.steps.size() * sizeof(KStep));
+    const size_t o_init = off;
+    off += a256(std::max<size_t>(prog.init.size(), 1) * sizeof(KOp));
+    const size_t o_df = off;
+    off += a256(T[0].df.size() * sizeof(double));
+    std::vector<size_t> o_grp(nc), o_tab(nc), o_thr(nc);
+    for (uint64_t i = 0; i < nc; ++i) {
+      o_grp[i] = off;
+      off += a256(std::max<size_t>(T[i].groups.size(), 1) * sizeof(KGroup));
+      o_tab[i] = off;
+      off += a256(T[i].tab.size() * sizeof(float));
+      o_thr[i] = off;
+      off += a256(T[i].thr.size() * sizeof(float));
+    }
+    char* dev = (char*)E->ensure(E->arena, off);
+    KStats* partials = (KStats*)E->ensure(E->partials, nc * nb * kScenarioStats * sizeof(KStats));
+    KStats* stats = (KStats*)E->ensure(E->stats, nc * kScenarioStats * sizeof(KStats));
+    unsigned* err = (unsigned*)E->ensure(E->errw, 256);
+    std::vector<char>& st = E->host_stage;
+    st.assign(off, 0);
+    auto put = [&](size_t o, const void* p, size_t bytes) {
+      if (bytes) memcpy(st.data() + o, p, bytes);
+    };
+    put(o_ops, ops.data(), ops.size() * sizeof(KOp));
+    put(o_steps, prog.steps.data(), prog.steps.size() * sizeof(KStep));
+    put(o_init, prog.init.data(), prog.init.size() * sizeof(KOp));
+    put(o_df, T[0].df.data(), T[0].df.size() * sizeof(double));
+    for (uint64_t i = 0; i < nc; ++i) {
+      put(o_grp[i], T[i].groups.data(), T[i].groups.size() * sizeof(KGroup));
+      put(o_tab[i], T[i].tab.data(), T[i].tab.size() * sizeof(float));
+      put(o_thr[i], T[i].thr.data(), T[i].thr.size() * sizeof(float));
+      PathArgs A{};
+      A.ops = (const KOp*)(dev + o_ops);
+      A.steps = (const KStep*)(dev + o_steps);
+      A.groups = (const KGroup*)(dev + o_grp[i]);
+      A.init = (const KOp*)(dev + o_init);
+      A.df = (const double*)(dev + o_df);
+      A.tab = (const float*)(dev + o_tab[i]);
+      A.thr = (const float*)(dev + o_thr[i]);
+      A.n_init = (int)prog.init.size();
+      A.n_grid = (int)grid.size();
+      A.n_groups = (int)T[i].groups.size();
+      A.tab_stride = T[i].stride;
+      fill_model_args(A, scen[c0 + i][0], cfg->seed);
+      for (int sc = 0; sc < kScenarios; ++sc) {
+        A.s0_scen[sc] = (float)scen[c0 + i][sc].s0[0];
+        A.df_index[sc] = T[i].df_index[sc];
+      }
+      for (int j = 0; j < kScenarioStats; ++j)
+        for (int sc = 0; sc < kScenarios; ++sc) A.stat_w[j][sc] = w[j][sc];
+      A.n_paths = N;
+      A.path_base = 0;
+      A.partials = partials + i * nb * kScenarioStats;
+      A.err = err;
+      memcpy(st.data() + i * sizeof(PathArgs), &A, sizeof(PathArgs));
+    }
+    check_hip(hipMemcpyAsync(dev, st.data(), off, hipMemcpyHostToDevice, E->stream), "upload");
+    check_hip(hipMemsetAsync(err, 0, sizeof(unsigned), E->stream), "memset");
+    E->prof_begin();
+    check_hip(launch_surface((const PathArgs*)dev, (int)nc, (int)nb, partials, stats, E->stream), "surface launch");
+    E->prof_end((double)N * (double)(grid.size() - 1) * kScenarios * (double)nc);
+    std::vector<KStats> res(nc * kScenarioStats);
+    unsigned err_bits = 0;
+    check_hip(hipMemcpyAsync(res.data(), stats, res.size() * sizeof(KStats), hipMemcpyDeviceToHost, E->stream),
+              "copy stats");
+    check_hip(hipMemcpyAsync(&err_bits, err, sizeof(unsigned), hipMemcpyDeviceToHost, E->stream), "copy err");
+    check_hip(hipStreamSynchronize(E->stream), "stream sync");
+    if (err_bits) {
+      int code = lowest_error_code(err_bits);
+      raise(code, error_text(code));
+    }
+    for (uint64_t i = 0; i < nc; ++i) fill_greeks(res, i * kScenarioStats, &out[c0 + i]);
+  }
 }
 
 // Central differences over the model parameters with common random numbers.

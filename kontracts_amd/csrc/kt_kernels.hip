@@ -387,6 +387,24 @@ __global__ __launch_bounds__(256) void portfolio_finalize_kernel(const Portfolio
   finalize_body(partials, n_blocks[j], 1, 0, out + j);
 }
 
+// Greek surface: one seven-scenario CRN launch (the greeks_gbm kernel) per
+// (spot, vol) cell, all cells in one grid: blockIdx.y is the cell, whose
+// PathArgs is read in place through the constant address space.
+template <int NS>
+__global__ __launch_bounds__(kBlock) void surface_kernel(const PathArgs* __restrict__ cells) {
+  const KT_CONST PathArgs& A = ((const KT_CONST PathArgs*)cells)[blockIdx.y];
+  float r[kScenarios][NS];
+  path_block<MK_GBM, 1, NS, 1, kScenarios, kScenarioStats, false>(A, blockIdx.x, r);
+}
+
+// Block c * kScenarioStats + j reduces output j of cell c (cell c's partials
+// are [n_blocks][kScenarioStats] at partials + c * n_blocks * kScenarioStats).
+__global__ __launch_bounds__(256) void surface_finalize_kernel(const KStats* __restrict__ partials, int n_blocks,
+                                                               KStats* __restrict__ out) {
+  const int cell = blockIdx.x / kScenarioStats, j = blockIdx.x % kScenarioStats;
+  finalize_body(partials + (size_t)cell * n_blocks * kScenarioStats, n_blocks, kScenarioStats, j, out + blockIdx.x);
+}
+
 __global__ __launch_bounds__(256) void diff_stats_kernel(const double* __restrict__ a, const double* __restrict__ b,
                                                          double scale, uint64_t n, KStats* __restrict__ partials) {
   const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
@@ -871,6 +889,15 @@ hipError_t launch_portfolio(const PortfolioArgs& q, const int* n_blocks, int n_j
                      q.counter, err, n_jobs);
   hipLaunchKernelGGL(portfolio_kernel<16>, dim3(grid), dim3(kBlock), 0, s, q);
   hipLaunchKernelGGL(portfolio_finalize_kernel, dim3(n_jobs), dim3(256), 0, s, q.jobs, n_blocks, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_surface(const PathArgs* cells, int n_cells, int n_blocks, const KStats* partials, KStats* out,
+                          hipStream_t s) {
+  if (n_cells <= 0 || n_cells > 65535 || n_blocks <= 0) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(surface_kernel<16>, dim3(n_blocks, n_cells), dim3(kBlock), 0, s, cells);
+  hipLaunchKernelGGL(surface_finalize_kernel, dim3(n_cells * kScenarioStats), dim3(256), 0, s, partials, n_blocks,
+                     out);
   return hipGetLastError();
 }
 

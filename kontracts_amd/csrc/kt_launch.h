@@ -142,6 +142,11 @@ hipError_t launch_diff_stats(const double* a, const double* b, double scale, uin
                              hipStream_t s);
 hipError_t launch_finalize(const KStats* partials, int n_blocks, int nout, int n_used, KStats* out,
                            hipStream_t s);
+// Greek surface: cells[c] (device) is the seven-scenario launch of cell c
+// (16 registers), its partials [n_blocks][kScenarioStats] at
+// partials + c * n_blocks * kScenarioStats; out[c * kScenarioStats + j].
+hipError_t launch_surface(const PathArgs* cells, int n_cells, int n_blocks, const KStats* partials, KStats* out,
+                          hipStream_t s);
 hipError_t launch_debug_philox(const uint32_t* ctr, uint32_t k0, uint32_t k1, uint64_t n, uint32_t* out,
                                hipStream_t s);
 hipError_t launch_debug_copy(const void* src, void* dst, uint64_t bytes, hipStream_t s);

@@ -1375,6 +1375,21 @@ int orc_greeks_gbm(const char* json, const char* asset, double s0, double sigma,
   }
 }
 
+// greek_surface (src/surface.rs:104-138): greeks_gbm at every (spots[i],
+// vols[j]), out[(i * n_vols + j) * 5 + q]; an empty axis is
+// InconsistentPath; the lowest failing cell's error is returned.
+int orc_greek_surface(const char* json, const char* asset, const double* spots, uint64_t n_spots, const double* vols,
+                      uint64_t n_vols, const orc_cfg* cfg, double h_spot, double h_vol, double h_rate, double* out,
+                      char* err, int errcap) {
+  if (n_spots == 0 || n_vols == 0) return report(KErr{4, "empty surface grid"}, err, errcap);
+  for (uint64_t c = 0; c < n_spots * n_vols; ++c) {
+    int rc = orc_greeks_gbm(json, asset, spots[c / n_vols], vols[c % n_vols], cfg, h_spot, h_vol, h_rate,
+                            out + 5 * c, err, errcap);
+    if (rc) return rc;
+  }
+  return 0;
+}
+
 // Heston CRN bump-and-reprice, the same central-difference pattern as
 // greeks_gbm applied to (spot, v0, kappa, theta, sigma_v, rho).
 // h[6] absolute bumps; out[7] = {price, dP/dspot, dP/dv0, dP/dkappa,

@@ -169,6 +169,21 @@ def greeks_gbm(json_s, asset, s0, sigma, c, h_spot=1e-2, h_vol=1e-2, h_rate=1e-4
     return dict(zip(["price", "delta", "gamma", "vega", "rho"], list(out)))
 
 
+def greek_surface(json_s, asset, spots, vols, c, h_spot=1e-2, h_vol=1e-2, h_rate=1e-4):
+    """Cells row-major [spot][vol], each a greeks_gbm dict."""
+    L = load()
+    ns, nv = len(spots), len(vols)
+    out = (C.c_double * max(5 * ns * nv, 1))()
+    e = _err()
+    rc = L.orc_greek_surface(json_s.encode(), asset.encode(), _dbl(spots) if ns else None, ns,
+                             _dbl(vols) if nv else None, nv, C.byref(c), C.c_double(h_spot), C.c_double(h_vol),
+                             C.c_double(h_rate), out, e, 512)
+    if rc:
+        raise OracleError(rc, e.value.decode())
+    keys = ["price", "delta", "gamma", "vega", "rho"]
+    return [dict(zip(keys, out[5 * i:5 * i + 5])) for i in range(ns * nv)]
+
+
 def greeks_heston_bump(json_s, model, c, h):
     L = load()
     out = (C.c_double * 7)()
