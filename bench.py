@@ -258,6 +258,70 @@ def main():
         tss = time.perf_counter() - t
         extras["config4_small_book_speedup"] = tss / tps
         extras["config4_small_book_portfolio_ms"] = tps * 1e3
+        # Calibration-sized calls (1,000 paths x 50 steps): per-call latency of
+        # a single price, and one objective evaluation over 10 quotes as one
+        # portfolio call (direct launches, then graph replay) vs a loop.
+        ko = K.up_and_out_call("X", 100.0, 130.0, 1.0)
+        for _ in range(20):
+            ko.price(K.GBM(100.0, 0.2, 0.05, asset="X"), n_paths=1000, seed=1)
+        t = time.perf_counter()
+        for i in range(200):
+            ko.price(K.GBM(100.0, 0.2 + 1e-4 * i, 0.05, asset="X"), n_paths=1000, seed=1)
+        extras["small_call_price_us"] = (time.perf_counter() - t) / 200 * 1e6
+        quotes = [(ko, K.GBM(s, 0.2, 0.05, asset="X"), 1000, 1, 50) for s in range(90, 110, 2)]
+        from kontracts_amd import _native as NN
+        eng = NN.engine(0)
+        for graph in (0, 1):
+            eng.set_option("portfolio_graph", graph)
+            for _ in range(10):
+                K.price_portfolio(quotes)
+            t = time.perf_counter()
+            for _ in range(100):
+                K.price_portfolio(quotes)
+            extras[f"objective_10_quotes_portfolio_{'graph' if graph else 'direct'}_us"] = \
+                (time.perf_counter() - t) / 100 * 1e6
+        eng.set_option("portfolio_graph", 0)
+        t = time.perf_counter()
+        for _ in range(20):
+            for q in quotes:
+                q[0].price(q[1], n_paths=q[2], seed=q[3], steps_per_year=q[4])
+        extras["objective_10_quotes_loop_us"] = (time.perf_counter() - t) / 20 * 1e6
+        # Greek surface: 20 x 20 (spot, vol) cells of the config-2 barrier at
+        # 2^16 paths x 252 steps, one launch vs a loop of greeks calls.
+        spots = [150.0 + 2.5 * i for i in range(20)]
+        vols = [0.15 + 0.01 * j for j in range(20)]
+        bar = K.up_and_out_call("X", 100, 200, 1.0)
+        K.greek_surface(bar, "X", spots[:2], vols[:2], n_paths=1 << 16, seed=42, steps_per_year=252)
+        eng.set_profiling(True)
+        eng.kernel_time(reset=True)
+        t = time.perf_counter()
+        K.greek_surface(bar, "X", spots, vols, n_paths=1 << 16, seed=42, steps_per_year=252)
+        tsurf = time.perf_counter() - t
+        ksurf_ms = eng.kernel_time(reset=True)[0]
+        eng.set_profiling(False)
+        extras["surface_20x20_kernel_ms"] = ksurf_ms
+        t = time.perf_counter()
+        for s_ in spots[:5]:
+            for v_ in vols:
+                bar.greeks(K.GBM(s_, v_, 0.05, asset="X"), n_paths=1 << 16, seed=42, steps_per_year=252)
+        tloop = (time.perf_counter() - t) * 4  # 100 of the 400 cells, scaled
+        extras["surface_20x20_s"] = tsurf
+        extras["surface_20x20_loop_s"] = tloop
+        extras["surface_scenario_path_steps_per_s"] = 400 * 7 * (1 << 16) * 252 / tsurf
+        # Variance reduction on a daily-average Asian call (1e6 paths x 252 steps)
+        call = K.scale((K.average(K.spot("X")) - 100.0).clip(0.0), K.one("USD")) @ K.at(1.0)  # Asian call
+        g252 = [i / 252 for i in range(253)]
+        call.price(m1, n_paths=1_000_000, seed=42, grid=g252)
+        t = time.perf_counter()
+        rv = call.price(m1, n_paths=1_000_000, seed=42, grid=g252)
+        extras["vr_none_ms"] = (time.perf_counter() - t) * 1e3
+        extras["vr_none_std_error"] = rv.std_error
+        for label, anti, cv in (("antithetic", True, False), ("control_variate", False, True), ("both", True, True)):
+            call.price(m1, n_paths=1_000_000, seed=42, grid=g252, antithetic=anti, control_variate=cv)
+            t = time.perf_counter()
+            rv = call.price(m1, n_paths=1_000_000, seed=42, grid=g252, antithetic=anti, control_variate=cv)
+            extras[f"vr_{label}_ms"] = (time.perf_counter() - t) * 1e3
+            extras[f"vr_{label}_std_error"] = rv.std_error
 
     line = {
         "metric": METRIC,

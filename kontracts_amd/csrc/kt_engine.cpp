@@ -180,8 +180,12 @@ kt_engine::~kt_engine() {
   (void)hipSetDevice(device);
   if (pf_exec) (void)hipGraphExecDestroy(pf_exec);
   if (pf_graph) (void)hipGraphDestroy(pf_graph);
-  for (DevBuf* b : {&arena, &partials, &stats, &errw, &pv, &z, &scratch, &paths, &aux, &book})
+  if (stage_busy && stage_ev) (void)hipEventSynchronize(stage_ev);
+  for (DevBuf* b : {&arena, &partials, &stats, &errw, &pv, &z, &scratch, &paths, &aux, &book, &errp})
     if (b->p) (void)hipFree(b->p);
+  for (PinBuf* b : {&stage_buf, &readback_buf})
+    if (b->p) (void)hipHostFree(b->p);
+  if (stage_ev) (void)hipEventDestroy(stage_ev);
   if (ev0) (void)hipEventDestroy(ev0);
   if (ev1) (void)hipEventDestroy(ev1);
   if (stream) (void)hipStreamDestroy(stream);
@@ -222,6 +226,39 @@ void* kt_engine::ensure(DevBuf& b, size_t bytes) {
   return b.p;
 }
 
+static char* grow_pinned(PinBuf& b, size_t bytes) {
+  if (bytes <= b.cap) return b.p;
+  if (b.p) check_hip(hipHostFree(b.p), "hipHostFree");
+  b.p = nullptr;
+  b.cap = 0;
+  const size_t want = std::max<size_t>(bytes, (size_t)1 << 16);
+  hipError_t e = hipHostMalloc((void**)&b.p, want, hipHostMallocDefault);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    b.p = nullptr;
+    raise(KT_ERR_OUT_OF_MEMORY, "page-locked host allocation of " + std::to_string(want) + " bytes failed");
+  }
+  b.cap = want;
+  return b.p;
+}
+
+char* kt_engine::stage(size_t bytes) {
+  if (stage_busy) {
+    check_hip(hipEventSynchronize(stage_ev), "hipEventSynchronize");
+    stage_busy = false;
+  }
+  return grow_pinned(stage_buf, std::max<size_t>(bytes, 256));
+}
+
+void kt_engine::stage_copy(void* dev, size_t bytes) {
+  if (!stage_ev) check_hip(hipEventCreateWithFlags(&stage_ev, hipEventDisableTiming), "hipEventCreate");
+  check_hip(hipMemcpyAsync(dev, stage_buf.p, bytes, hipMemcpyHostToDevice, stream), "upload");
+  check_hip(hipEventRecord(stage_ev, stream), "hipEventRecord");
+  stage_busy = true;
+}
+
+char* kt_engine::readback(size_t bytes) { return grow_pinned(readback_buf, std::max<size_t>(bytes, 256)); }
+
 // Upload a set of host blobs into the arena; returns device pointers.
 std::vector<void*> kt_engine::upload(const std::vector<std::pair<const void*, size_t>>& blobs) {
   size_t total = 0;
@@ -230,11 +267,12 @@ std::vector<void*> kt_engine::upload(const std::vector<std::pair<const void*, si
     offs.push_back(total);
     total += (b.second + 255) & ~(size_t)255;
   }
-  host_stage.resize(std::max<size_t>(total, 256));
+  total = std::max<size_t>(total, 256);
+  char* host = stage(total);
   for (size_t i = 0; i < blobs.size(); ++i)
-    if (blobs[i].second) memcpy(host_stage.data() + offs[i], blobs[i].first, blobs[i].second);
-  char* dev = (char*)ensure(arena, std::max<size_t>(total, 256));
-  check_hip(hipMemcpyAsync(dev, host_stage.data(), total, hipMemcpyHostToDevice, stream), "upload");
+    if (blobs[i].second) memcpy(host + offs[i], blobs[i].first, blobs[i].second);
+  char* dev = (char*)ensure(arena, total);
+  stage_copy(dev, total);
   std::vector<void*> out;
   for (size_t o : offs) out.push_back(dev + o);
   return out;
@@ -282,7 +320,7 @@ std::vector<KGroup> knockout_tables(const Program& prog, const std::vector<Model
 }
 
 LaunchTables launch_tables(const Program& prog, const std::vector<ModelDesc>& models,
-                           const std::vector<double>& rates, const std::vector<double>& grid) {
+                           const std::vector<double>& rates, const std::vector<double>& grid, bool discount) {
   const int SC = (int)models.size();
   const ModelDesc& md = models[0];
   LaunchTables T;
@@ -290,8 +328,14 @@ LaunchTables launch_tables(const Program& prog, const std::vector<ModelDesc>& mo
     T.tab = model_table(md, grid, &T.stride);
   } else {
     if (md.kind != KT_MODEL_GBM) raise(KT_ERR_UNSUPPORTED, "scenario kernel is GBM only");
+    // A GBM table depends on (mu, sigma) only: spot-bumped scenarios share it.
     std::vector<std::vector<float>> per;
-    for (const ModelDesc& m : models) per.push_back(model_table(m, grid, &T.stride));
+    for (int sc = 0; sc < SC; ++sc) {
+      int same = -1;
+      for (int q = 0; q < sc && same < 0; ++q)
+        if (models[q].mu == models[sc].mu && models[q].sigma == models[sc].sigma) same = q;
+      per.push_back(same >= 0 ? per[same] : model_table(models[sc], grid, &T.stride));
+    }
     T.tab.resize(grid.size() * SC * 2);
     for (size_t k = 0; k < grid.size(); ++k)
       for (int sc = 0; sc < SC; ++sc) {
@@ -312,9 +356,11 @@ LaunchTables launch_tables(const Program& prog, const std::vector<ModelDesc>& mo
       T.df_index[sc] = (int)(it - distinct.begin());
     }
   }
-  T.df.resize(distinct.size() * grid.size());
-  for (size_t t = 0; t < distinct.size(); ++t)
-    for (size_t k = 0; k < grid.size(); ++k) T.df[t * grid.size() + k] = std::exp(-distinct[t] * grid[k]);
+  if (discount) {
+    T.df.resize(distinct.size() * grid.size());
+    for (size_t t = 0; t < distinct.size(); ++t)
+      for (size_t k = 0; k < grid.size(); ++k) T.df[t * grid.size() + k] = std::exp(-distinct[t] * grid[k]);
+  }
   T.groups = knockout_tables(prog, models, &T.thr);
   return T;
 }
@@ -360,9 +406,12 @@ std::vector<KStats> run_program_scen(kt_engine* E, const Program& prog, const st
   uint64_t total_blocks = (n_paths + kBlock - 1) / kBlock;
   if (total_blocks > (1ull << 31)) raise(KT_ERR_INVALID_ARGUMENT, "too many paths");
   KStats* partials = (KStats*)E->ensure(E->partials, std::max<uint64_t>(total_blocks, 1) * nstat * sizeof(KStats));
-  KStats* stats = (KStats*)E->ensure(E->stats, 8 * sizeof(KStats));
-  unsigned* err = (unsigned*)E->ensure(E->errw, 256);
-  check_hip(hipMemsetAsync(err, 0, sizeof(unsigned), E->stream), "memset");
+  // [n_used statistics][error word copy]: one readback
+  KStats* stats = (KStats*)E->ensure(E->stats, (kScenarioStats + 8 + 1) * sizeof(KStats));
+  unsigned* err_copy = (unsigned*)(stats + n_used);
+  unsigned* err = (unsigned*)E->ensure(E->errp, 256);
+  if (!E->errp_zero) check_hip(hipMemsetAsync(err, 0, sizeof(unsigned), E->stream), "memset");
+  E->errp_zero = false;  // until the finalize below has cleared it again
   double* pv_dev = nullptr;
   float* z_dev = nullptr;
   int M = model_normals(md);
@@ -407,12 +456,11 @@ std::vector<KStats> run_program_scen(kt_engine* E, const Program& prog, const st
     check_hip(launch_path(kind, K, ns, nout, debug, SC, A, (int)nb, E->stream, anti), "path kernel launch");
   }
   E->prof_end((double)n_paths * (double)(grid.size() > 0 ? grid.size() - 1 : 0) * SC * (anti ? 2 : 1));
-  check_hip(launch_finalize(partials, (int)total_blocks, nstat, n_used, stats, E->stream), "finalize launch");
-  std::vector<KStats> out(n_used);
-  unsigned err_bits = 0;
-  check_hip(hipMemcpyAsync(out.data(), stats, n_used * sizeof(KStats), hipMemcpyDeviceToHost, E->stream),
-            "copy stats");
-  check_hip(hipMemcpyAsync(&err_bits, err, sizeof(unsigned), hipMemcpyDeviceToHost, E->stream), "copy err");
+  check_hip(launch_finalize(partials, (int)total_blocks, nstat, n_used, stats, E->stream, err, err_copy),
+            "finalize launch");
+  const size_t rb_bytes = (n_used + 1) * sizeof(KStats);
+  char* rb = E->readback(rb_bytes);
+  check_hip(hipMemcpyAsync(rb, stats, rb_bytes, hipMemcpyDeviceToHost, E->stream), "copy stats");
   if (pv_host && pv_dev)
     check_hip(hipMemcpyAsync(pv_host, pv_dev, n_paths * sizeof(double), hipMemcpyDeviceToHost, E->stream), "copy pv");
   if (z_host)
@@ -420,6 +468,11 @@ std::vector<KStats> run_program_scen(kt_engine* E, const Program& prog, const st
                              E->stream),
               "copy normals");
   check_hip(hipStreamSynchronize(E->stream), "stream sync");
+  E->errp_zero = true;
+  std::vector<KStats> out(n_used);
+  memcpy(out.data(), rb, n_used * sizeof(KStats));
+  unsigned err_bits = 0;
+  memcpy(&err_bits, rb + n_used * sizeof(KStats), sizeof(unsigned));
   if (err_bits) {
     int code = lowest_error_code(err_bits);
     raise(code, error_text(code));
@@ -687,6 +740,9 @@ kt_status kt_engine_set_option(kt_engine* E, const char* name, int64_t value) {
   } else if (key == "portfolio_chunk") {
     if (value < 1 || value > 1024) raise(KT_ERR_INVALID_ARGUMENT, "portfolio_chunk must be in [1, 1024]");
     E->pf_chunk = (int)value;
+  } else if (key == "portfolio_graph") {
+    if (value != 0 && value != 1) raise(KT_ERR_INVALID_ARGUMENT, "portfolio_graph must be 0 or 1");
+    E->pf_use_graph = value == 1;
   } else {
     raise(KT_ERR_INVALID_ARGUMENT, "unknown engine option: " + key);
   }

@@ -17,6 +17,13 @@ struct DevBuf {
   size_t cap = 0;
 };
 
+// Page-locked host buffer: copies from / to it are plain DMA, asynchronous
+// to the host, instead of a staged copy through a driver bounce buffer.
+struct PinBuf {
+  char* p = nullptr;
+  size_t cap = 0;
+};
+
 struct kt_engine {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -25,10 +32,26 @@ struct kt_engine {
   // persistent portfolio launch: tile choice and the captured launch graph
   int pf_blocks_per_cu = 0;  // 0: the kernel's occupancy limit
   int pf_chunk = 1;
+  // Direct launches by default: on this stack a hipGraphLaunch of the
+  // three-kernel sequence costs more host time than the three launches.
+  bool pf_use_graph = false;
   hipGraph_t pf_graph = nullptr;
   hipGraphExec_t pf_exec = nullptr;
   std::vector<uint64_t> pf_key;
-  std::vector<char> host_stage;
+  // Host -> device staging (tables, arguments) and device -> host readback
+  // (statistics, error words), both page-locked.  stage_ev marks the last
+  // copy out of the staging buffer; it is waited on before the buffer is
+  // rewritten.
+  PinBuf stage_buf, readback_buf;
+  hipEvent_t stage_ev = nullptr;
+  bool stage_busy = false;
+  // Error word of the per-contract path launches: the finalize kernel copies
+  // it next to the statistics and clears it, so it is known zero between
+  // calls unless a call was interrupted.
+  DevBuf errp;
+  bool errp_zero = false;
+  int occupancy_portfolio = 0;  // cached occupancies (blocks per CU)
+  int occupancy_surface = 0;
   bool profiling = false;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   double prof_ms = 0.0, prof_path_steps = 0.0;
@@ -40,6 +63,12 @@ struct kt_engine {
   ~kt_engine();
   void* ensure(DevBuf& b, size_t bytes);
   std::vector<void*> upload(const std::vector<std::pair<const void*, size_t>>& blobs);
+  // Staging buffer of at least `bytes`, free to write (an earlier copy out of
+  // it has completed); stage_copy sends its first `bytes` to dev.
+  char* stage(size_t bytes);
+  void stage_copy(void* dev, size_t bytes);
+  // Readback buffer of at least `bytes` (valid after the stream syncs).
+  char* readback(size_t bytes);
 };
 
 namespace kt {
@@ -95,8 +124,9 @@ struct LaunchTables {
   std::vector<KGroup> groups;
   std::vector<float> thr;
 };
+// discount = false leaves df empty (a caller that shares one discount table).
 LaunchTables launch_tables(const Program& prog, const std::vector<ModelDesc>& models,
-                           const std::vector<double>& rates, const std::vector<double>& grid);
+                           const std::vector<double>& rates, const std::vector<double>& grid, bool discount = true);
 
 // Model constants of the path kernels (spots, Cholesky factor, Heston
 // correlation split) and the Philox key.

@@ -3,6 +3,7 @@
 #include <cmath>
 #include <cstring>
 #include <limits>
+#include <map>
 #include <string>
 
 #include "kt_engine_internal.h"
@@ -160,13 +161,22 @@ void greek_surface(kt_engine* E, const char* json, const char* asset, const doub
   if (ops.empty()) ops.push_back(KOp{OP_NOP, 0, 0, 0, 0.0f});
   const std::vector<std::vector<double>> w = scenario_weights(h);
   auto a256 = [](size_t b) { return (b + 255) & ~(size_t)255; };
-  // Cells per launch: within the grid's y limit and ~2 GB of partials.
-  const uint64_t per_launch = std::min<uint64_t>(65535, std::max<uint64_t>(1, (1ull << 24) / nb));
+  // Cells run vol-major: the model table depends on the vol only and the
+  // thresholds on the spot only, so identical tables are stored once and the
+  // cells resident together mostly read one table.  slot p of a launch is
+  // cell order[p].
+  std::vector<uint64_t> order;
+  order.reserve(n_cells);
+  for (uint64_t j = 0; j < n_vols; ++j)
+    for (uint64_t i = 0; i < n_spots; ++i) order.push_back(i * n_vols + j);
+  // Cells per launch: ~2 GB of partials at most.
+  const uint64_t per_launch = std::max<uint64_t>(1, (1ull << 24) / nb);
   for (uint64_t c0 = 0; c0 < n_cells; c0 += per_launch) {
     const uint64_t nc = std::min(per_launch, n_cells - c0);
     std::vector<LaunchTables> T(nc);
-    for (uint64_t i = 0; i < nc; ++i) T[i] = launch_tables(prog, scen[c0 + i], rates, grid);
-    // Arena: [cells' PathArgs][ops][steps][init][df][per cell: groups, tab, thr]
+    // the discount tables depend on the rates only: the first cell's serve all
+    for (uint64_t i = 0; i < nc; ++i) T[i] = launch_tables(prog, scen[order[c0 + i]], rates, grid, i == 0);
+    // Arena: [cells' PathArgs][ops][steps][init][df][distinct groups / tab / thr blobs]
     size_t off = a256(nc * sizeof(PathArgs));
     const size_t o_ops = off;
     off += a256(ops.size() * sizeof(KOp));
@@ -176,32 +186,42 @@ void greek_surface(kt_engine* E, const char* json, const char* asset, const doub
     off += a256(std::max<size_t>(prog.init.size(), 1) * sizeof(KOp));
     const size_t o_df = off;
     off += a256(T[0].df.size() * sizeof(double));
+    std::map<std::string, size_t> interned;
+    std::vector<std::pair<const void*, size_t>> blobs;  // (data, bytes) at offsets blob_off
+    std::vector<size_t> blob_off;
+    auto intern = [&](const void* p, size_t bytes) {
+      std::string key((const char*)p, bytes);
+      auto it = interned.find(key);
+      if (it != interned.end()) return it->second;
+      const size_t o = off;
+      off += a256(std::max<size_t>(bytes, 1));
+      interned.emplace(std::move(key), o);
+      blobs.push_back({p, bytes});
+      blob_off.push_back(o);
+      return o;
+    };
     std::vector<size_t> o_grp(nc), o_tab(nc), o_thr(nc);
     for (uint64_t i = 0; i < nc; ++i) {
-      o_grp[i] = off;
-      off += a256(std::max<size_t>(T[i].groups.size(), 1) * sizeof(KGroup));
-      o_tab[i] = off;
-      off += a256(T[i].tab.size() * sizeof(float));
-      o_thr[i] = off;
-      off += a256(T[i].thr.size() * sizeof(float));
+      o_grp[i] = intern(T[i].groups.data(), T[i].groups.size() * sizeof(KGroup));
+      o_tab[i] = intern(T[i].tab.data(), T[i].tab.size() * sizeof(float));
+      o_thr[i] = intern(T[i].thr.data(), T[i].thr.size() * sizeof(float));
     }
     char* dev = (char*)E->ensure(E->arena, off);
     KStats* partials = (KStats*)E->ensure(E->partials, nc * nb * kScenarioStats * sizeof(KStats));
     KStats* stats = (KStats*)E->ensure(E->stats, nc * kScenarioStats * sizeof(KStats));
     unsigned* err = (unsigned*)E->ensure(E->errw, 256);
-    std::vector<char>& st = E->host_stage;
-    st.assign(off, 0);
+    char* st = E->stage(off);
+    memset(st, 0, off);
     auto put = [&](size_t o, const void* p, size_t bytes) {
-      if (bytes) memcpy(st.data() + o, p, bytes);
+      if (bytes) memcpy(st + o, p, bytes);
     };
     put(o_ops, ops.data(), ops.size() * sizeof(KOp));
     put(o_steps, prog.steps.data(), prog.steps.size() * sizeof(KStep));
     put(o_init, prog.init.data(), prog.init.size() * sizeof(KOp));
     put(o_df, T[0].df.data(), T[0].df.size() * sizeof(double));
+    for (size_t b = 0; b < blobs.size(); ++b) put(blob_off[b], blobs[b].first, blobs[b].second);
     for (uint64_t i = 0; i < nc; ++i) {
-      put(o_grp[i], T[i].groups.data(), T[i].groups.size() * sizeof(KGroup));
-      put(o_tab[i], T[i].tab.data(), T[i].tab.size() * sizeof(float));
-      put(o_thr[i], T[i].thr.data(), T[i].thr.size() * sizeof(float));
+      const std::vector<ModelDesc>& sc_models = scen[order[c0 + i]];
       PathArgs A{};
       A.ops = (const KOp*)(dev + o_ops);
       A.steps = (const KStep*)(dev + o_steps);
@@ -214,9 +234,9 @@ void greek_surface(kt_engine* E, const char* json, const char* asset, const doub
       A.n_grid = (int)grid.size();
       A.n_groups = (int)T[i].groups.size();
       A.tab_stride = T[i].stride;
-      fill_model_args(A, scen[c0 + i][0], cfg->seed);
+      fill_model_args(A, sc_models[0], cfg->seed);
       for (int sc = 0; sc < kScenarios; ++sc) {
-        A.s0_scen[sc] = (float)scen[c0 + i][sc].s0[0];
+        A.s0_scen[sc] = (float)sc_models[sc].s0[0];
         A.df_index[sc] = T[i].df_index[sc];
       }
       for (int j = 0; j < kScenarioStats; ++j)
@@ -225,12 +245,17 @@ void greek_surface(kt_engine* E, const char* json, const char* asset, const doub
       A.path_base = 0;
       A.partials = partials + i * nb * kScenarioStats;
       A.err = err;
-      memcpy(st.data() + i * sizeof(PathArgs), &A, sizeof(PathArgs));
+      memcpy(st + i * sizeof(PathArgs), &A, sizeof(PathArgs));
     }
-    check_hip(hipMemcpyAsync(dev, st.data(), off, hipMemcpyHostToDevice, E->stream), "upload");
+    E->stage_copy(dev, off);
     check_hip(hipMemsetAsync(err, 0, sizeof(unsigned), E->stream), "memset");
     E->prof_begin();
-    check_hip(launch_surface((const PathArgs*)dev, (int)nc, (int)nb, partials, stats, E->stream), "surface launch");
+    int n_cu = 0;
+    check_hip(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, E->device), "device attribute");
+    if (E->occupancy_surface <= 0) E->occupancy_surface = surface_occupancy_blocks_per_cu();
+    check_hip(launch_surface((const PathArgs*)dev, (int)nc, (int)nb, partials, stats, n_cu, E->occupancy_surface,
+                             E->stream),
+              "surface launch");
     E->prof_end((double)N * (double)(grid.size() - 1) * kScenarios * (double)nc);
     std::vector<KStats> res(nc * kScenarioStats);
     unsigned err_bits = 0;
@@ -242,7 +267,7 @@ void greek_surface(kt_engine* E, const char* json, const char* asset, const doub
       int code = lowest_error_code(err_bits);
       raise(code, error_text(code));
     }
-    for (uint64_t i = 0; i < nc; ++i) fill_greeks(res, i * kScenarioStats, &out[c0 + i]);
+    for (uint64_t i = 0; i < nc; ++i) fill_greeks(res, i * kScenarioStats, &out[order[c0 + i]]);
   }
 }
 

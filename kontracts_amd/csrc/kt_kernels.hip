@@ -331,10 +331,15 @@ __device__ __forceinline__ void finalize_body(const KStats* __restrict__ partial
   if (threadIdx.x == 0) *out = sh[0];
 }
 
-// Block j reduces output j.
+// Block j reduces output j.  err (nullable): block 0 copies the launch's
+// error word to *err_copy and clears it for the next launch.
 __global__ __launch_bounds__(256) void finalize_kernel(const KStats* __restrict__ partials, int n_blocks, int nout,
-                                                       KStats* __restrict__ out) {
+                                                       KStats* __restrict__ out, unsigned* err, unsigned* err_copy) {
   finalize_body(partials, n_blocks, nout, blockIdx.x, out + blockIdx.x);
+  if (err && blockIdx.x == 0 && threadIdx.x == 0) {
+    *err_copy = *err;
+    *err = 0u;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -378,23 +383,45 @@ __global__ __launch_bounds__(kBlock) void portfolio_kernel(const PortfolioArgs Q
   }
 }
 
-// Block j finalizes job j's partials (the per-contract finalize, unchanged).
+// Block j finalizes job j's partials (the per-contract finalize, unchanged)
+// and copies its error word to err_copy[j].
 __global__ __launch_bounds__(256) void portfolio_finalize_kernel(const PortfolioJob* __restrict__ jobs,
                                                                  const int* __restrict__ n_blocks,
-                                                                 KStats* __restrict__ out) {
+                                                                 KStats* __restrict__ out, unsigned* err_copy) {
   const int j = blockIdx.x;
   const KStats* partials = jobs[j].a.partials;
   finalize_body(partials, n_blocks[j], 1, 0, out + j);
+  if (threadIdx.x == 0) err_copy[j] = *jobs[j].a.err;
 }
 
 // Greek surface: one seven-scenario CRN launch (the greeks_gbm kernel) per
-// (spot, vol) cell, all cells in one grid: blockIdx.y is the cell, whose
-// PathArgs is read in place through the constant address space.
+// (spot, vol) cell, all cells in one 1-D grid of n_cells * n_blocks
+// workgroups; each cell's PathArgs is read in place through the constant
+// address space.
+//
+// Workgroup -> (cell, block) map: workgroups are dispatched round-robin over
+// the XCDs and then over each XCD's CUs, so the workgroups resident on one CU
+// at a time are L, L + n_cu, L + 2 n_cu, ...  Within each full chunk of
+// n_cu * per_cu workgroups the map sends those co-resident workgroups to
+// consecutive blocks of one cell, so a CU's scalar cache holds one cell's
+// tables instead of per_cu of them.  The map is a bijection on every full
+// chunk (identity on the last partial one), so it only changes which
+// workgroup computes a block, never a result.
 template <int NS>
-__global__ __launch_bounds__(kBlock) void surface_kernel(const PathArgs* __restrict__ cells) {
-  const KT_CONST PathArgs& A = ((const KT_CONST PathArgs*)cells)[blockIdx.y];
+__global__ __launch_bounds__(kBlock) void surface_kernel(const PathArgs* __restrict__ cells, uint32_t n_blocks,
+                                                         uint32_t total, uint32_t n_cu, uint32_t per_cu) {
+  const uint32_t L = blockIdx.x;
+  const uint32_t chunk = n_cu * per_cu;
+  uint32_t v = L;
+  const uint32_t q = L / chunk;
+  if ((q + 1) * chunk <= total) {
+    const uint32_t r = L - q * chunk;
+    v = q * chunk + (r % n_cu) * per_cu + r / n_cu;
+  }
+  const uint32_t cell = v / n_blocks, blk = v - cell * n_blocks;
+  const KT_CONST PathArgs& A = ((const KT_CONST PathArgs*)cells)[cell];
   float r[kScenarios][NS];
-  path_block<MK_GBM, 1, NS, 1, kScenarios, kScenarioStats, false>(A, blockIdx.x, r);
+  path_block<MK_GBM, 1, NS, 1, kScenarios, kScenarioStats, false>(A, blk, r);
 }
 
 // Block c * kScenarioStats + j reduces output j of cell c (cell c's partials
@@ -884,26 +911,35 @@ int portfolio_occupancy_blocks_per_cu() {
 }
 
 hipError_t launch_portfolio(const PortfolioArgs& q, const int* n_blocks, int n_jobs, int grid, unsigned* err,
-                            KStats* out, hipStream_t s) {
+                            KStats* out, unsigned* err_copy, hipStream_t s) {
   hipLaunchKernelGGL(portfolio_reset_kernel, dim3(std::max(1, std::min(64, (n_jobs + 255) / 256))), dim3(256), 0, s,
                      q.counter, err, n_jobs);
   hipLaunchKernelGGL(portfolio_kernel<16>, dim3(grid), dim3(kBlock), 0, s, q);
-  hipLaunchKernelGGL(portfolio_finalize_kernel, dim3(n_jobs), dim3(256), 0, s, q.jobs, n_blocks, out);
+  hipLaunchKernelGGL(portfolio_finalize_kernel, dim3(n_jobs), dim3(256), 0, s, q.jobs, n_blocks, out, err_copy);
   return hipGetLastError();
 }
 
+int surface_occupancy_blocks_per_cu() {
+  int n = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, surface_kernel<16>, kBlock, 0) != hipSuccess) return 4;
+  return n > 0 ? n : 1;
+}
+
 hipError_t launch_surface(const PathArgs* cells, int n_cells, int n_blocks, const KStats* partials, KStats* out,
-                          hipStream_t s) {
-  if (n_cells <= 0 || n_cells > 65535 || n_blocks <= 0) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(surface_kernel<16>, dim3(n_blocks, n_cells), dim3(kBlock), 0, s, cells);
+                          int n_cu, int per_cu, hipStream_t s) {
+  if (n_cells <= 0 || n_blocks <= 0 || n_cu <= 0 || per_cu <= 0) return hipErrorInvalidValue;
+  const uint64_t total = (uint64_t)n_cells * (uint64_t)n_blocks;
+  if (total > 0x7fffffffull || (uint64_t)n_cu * (uint64_t)per_cu > 0xffffffffull) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(surface_kernel<16>, dim3((unsigned)total), dim3(kBlock), 0, s, cells, (uint32_t)n_blocks,
+                     (uint32_t)total, (uint32_t)n_cu, (uint32_t)per_cu);
   hipLaunchKernelGGL(surface_finalize_kernel, dim3(n_cells * kScenarioStats), dim3(256), 0, s, partials, n_blocks,
                      out);
   return hipGetLastError();
 }
 
 hipError_t launch_finalize(const KStats* partials, int n_blocks, int nout, int n_used, KStats* out,
-                           hipStream_t s) {
-  hipLaunchKernelGGL(finalize_kernel, dim3(n_used), dim3(256), 0, s, partials, n_blocks, nout, out);
+                           hipStream_t s, unsigned* err, unsigned* err_copy) {
+  hipLaunchKernelGGL(finalize_kernel, dim3(n_used), dim3(256), 0, s, partials, n_blocks, nout, out, err, err_copy);
   return hipGetLastError();
 }
 

@@ -131,22 +131,27 @@ struct PortfolioArgs {
   unsigned* counter;      // queue head (reset by the launch)
 };
 // Reset the queue and the jobs' error words err[n_jobs], run the persistent
-// grid, finalize each job into out[job].
+// grid, finalize each job into out[job] and copy its error word to
+// err_copy[job].
 hipError_t launch_portfolio(const PortfolioArgs& q, const int* n_blocks, int n_jobs, int grid, unsigned* err,
-                            KStats* out, hipStream_t s);
+                            KStats* out, unsigned* err_copy, hipStream_t s);
 // Resident 256-lane blocks of the persistent kernel per compute unit.
 int portfolio_occupancy_blocks_per_cu();
 
 // partials[block] = statistics of (a[i] - b[i]) * scale over the block's paths.
 hipError_t launch_diff_stats(const double* a, const double* b, double scale, uint64_t n, KStats* partials,
                              hipStream_t s);
+// err (nullable): the launch's error word, copied to *err_copy and cleared.
 hipError_t launch_finalize(const KStats* partials, int n_blocks, int nout, int n_used, KStats* out,
-                           hipStream_t s);
+                           hipStream_t s, unsigned* err = nullptr, unsigned* err_copy = nullptr);
 // Greek surface: cells[c] (device) is the seven-scenario launch of cell c
 // (16 registers), its partials [n_blocks][kScenarioStats] at
 // partials + c * n_blocks * kScenarioStats; out[c * kScenarioStats + j].
+// n_cu, per_cu: compute units and resident blocks per CU, for the
+// co-residency-aware workgroup map (never changes results).
 hipError_t launch_surface(const PathArgs* cells, int n_cells, int n_blocks, const KStats* partials, KStats* out,
-                          hipStream_t s);
+                          int n_cu, int per_cu, hipStream_t s);
+int surface_occupancy_blocks_per_cu();
 hipError_t launch_debug_philox(const uint32_t* ctr, uint32_t k0, uint32_t k1, uint64_t n, uint32_t* out,
                                hipStream_t s);
 hipError_t launch_debug_copy(const void* src, void* dst, uint64_t bytes, hipStream_t s);

@@ -150,12 +150,15 @@ void price_portfolio(kt_engine* E, const kt_job* jobs, uint64_t n_jobs, kt_price
     }
     char* dev = (char*)E->ensure(E->book, cursor);
     KStats* partials = (KStats*)E->ensure(E->partials, n_partials * sizeof(KStats));
-    KStats* stats = (KStats*)E->ensure(E->stats, std::max<size_t>(book.size(), kAdjMaxOut) * sizeof(KStats));
+    // [statistics of each job][error word of each job]: one readback
+    const size_t rb_bytes = book.size() * sizeof(KStats) + align256(book.size() * sizeof(unsigned));
+    KStats* stats = (KStats*)E->ensure(E->stats, std::max<size_t>(rb_bytes, kAdjMaxOut * sizeof(KStats)));
+    unsigned* err_copy = (unsigned*)(stats + book.size());
     unsigned* errw = (unsigned*)E->ensure(E->errw, align256((book.size() + 64) * sizeof(unsigned)));
     unsigned* counter = errw + book.size();
 
-    std::vector<char>& H = E->host_stage;
-    H.assign(cursor, 0);
+    char* H = E->stage(cursor);
+    std::memset(H, 0, cursor);
     std::vector<PortfolioJob> pj(book.size());
     for (size_t i = 0; i < book.size(); ++i) {
       JobPlan& P = plans[book[i]];
@@ -188,12 +191,12 @@ void price_portfolio(kt_engine* E, const kt_job* jobs, uint64_t n_jobs, kt_price
                              {P.tab.data(), P.tab.size() * sizeof(float)},
                              {P.thr.data(), P.thr.size() * sizeof(float)}};
       for (int t = 0; t < 7; ++t)
-        if (blobs[t].bytes) std::memcpy(H.data() + P.off[t], blobs[t].p, blobs[t].bytes);
+        if (blobs[t].bytes) std::memcpy(H + P.off[t], blobs[t].p, blobs[t].bytes);
     }
-    std::memcpy(H.data(), pj.data(), pj.size() * sizeof(PortfolioJob));
-    std::memcpy(H.data() + jobs_bytes, items.data(), items.size() * sizeof(uint32_t));
-    std::memcpy(H.data() + jobs_bytes + items_bytes, nblk.data(), nblk.size() * sizeof(int));
-    check_hip(hipMemcpyAsync(dev, H.data(), cursor, hipMemcpyHostToDevice, E->stream), "upload book");
+    std::memcpy(H, pj.data(), pj.size() * sizeof(PortfolioJob));
+    std::memcpy(H + jobs_bytes, items.data(), items.size() * sizeof(uint32_t));
+    std::memcpy(H + jobs_bytes + items_bytes, nblk.data(), nblk.size() * sizeof(int));
+    E->stage_copy(dev, cursor);
 
     PortfolioArgs Q{};
     Q.jobs = (const PortfolioJob*)dev;
@@ -204,21 +207,25 @@ void price_portfolio(kt_engine* E, const kt_job* jobs, uint64_t n_jobs, kt_price
     const int* nblk_dev = (const int*)(dev + jobs_bytes + items_bytes);
     int n_cu = 0;
     check_hip(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, E->device), "device attribute");
-    const int bpc = E->pf_blocks_per_cu > 0 ? E->pf_blocks_per_cu : portfolio_occupancy_blocks_per_cu();
+    if (E->occupancy_portfolio <= 0) E->occupancy_portfolio = portfolio_occupancy_blocks_per_cu();
+    const int bpc = E->pf_blocks_per_cu > 0 ? E->pf_blocks_per_cu : E->occupancy_portfolio;
     const int grid = (int)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)n_cu * bpc, n_items));
 
     // Replay the captured launch sequence while the book keeps its shape.
     const std::vector<uint64_t> key = {(uint64_t)dev, (uint64_t)partials, (uint64_t)stats, (uint64_t)errw,
                                        n_items, book.size(), (uint64_t)grid, (uint64_t)Q.chunk};
     E->prof_begin();
-    if (!E->pf_exec || E->pf_key != key) {
+    if (!E->pf_use_graph) {
+      check_hip(launch_portfolio(Q, nblk_dev, (int)book.size(), grid, errw, stats, err_copy, E->stream),
+                "portfolio launch");
+    } else if (!E->pf_exec || E->pf_key != key) {
       if (E->pf_exec) (void)hipGraphExecDestroy(E->pf_exec);
       if (E->pf_graph) (void)hipGraphDestroy(E->pf_graph);
       E->pf_exec = nullptr;
       E->pf_graph = nullptr;
       E->pf_key.clear();
       check_hip(hipStreamBeginCapture(E->stream, hipStreamCaptureModeThreadLocal), "begin capture");
-      hipError_t le = launch_portfolio(Q, nblk_dev, (int)book.size(), grid, errw, stats, E->stream);
+      hipError_t le = launch_portfolio(Q, nblk_dev, (int)book.size(), grid, errw, stats, err_copy, E->stream);
       hipGraph_t g = nullptr;
       hipError_t ce = hipStreamEndCapture(E->stream, &g);
       check_hip(le, "portfolio launch");
@@ -227,19 +234,18 @@ void price_portfolio(kt_engine* E, const kt_job* jobs, uint64_t n_jobs, kt_price
       check_hip(hipGraphInstantiate(&E->pf_exec, g, nullptr, nullptr, 0), "graph instantiate");
       E->pf_key = key;
     }
-    check_hip(hipGraphLaunch(E->pf_exec, E->stream), "graph launch");
+    if (E->pf_use_graph) check_hip(hipGraphLaunch(E->pf_exec, E->stream), "graph launch");
     double path_steps = 0.0;
     for (uint64_t j : book) path_steps += (double)jobs[j].cfg.n_paths * (double)(plans[j].grid.size() - 1);
     E->prof_end(path_steps);
 
+    char* rb = E->readback(rb_bytes);
+    check_hip(hipMemcpyAsync(rb, stats, rb_bytes, hipMemcpyDeviceToHost, E->stream), "copy stats");
+    check_hip(hipStreamSynchronize(E->stream), "stream sync");
     std::vector<KStats> st(book.size());
     std::vector<unsigned> err_bits(book.size());
-    check_hip(hipMemcpyAsync(st.data(), stats, book.size() * sizeof(KStats), hipMemcpyDeviceToHost, E->stream),
-              "copy stats");
-    check_hip(hipMemcpyAsync(err_bits.data(), errw, book.size() * sizeof(unsigned), hipMemcpyDeviceToHost,
-                             E->stream),
-              "copy err");
-    check_hip(hipStreamSynchronize(E->stream), "stream sync");
+    std::memcpy(st.data(), rb, book.size() * sizeof(KStats));
+    std::memcpy(err_bits.data(), rb + book.size() * sizeof(KStats), book.size() * sizeof(unsigned));
     // A path error fails the call like the reference loop would at that job,
     // unless an earlier job (solo or in the book) fails first.
     uint64_t first_err = n_jobs;

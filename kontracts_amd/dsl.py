@@ -281,12 +281,14 @@ def _param_names(model) -> list:
 
 
 class Contract:
-    """Peyton-Jones combinators (src/ast.rs:187-206)."""
+    """Peyton-Jones combinators (src/ast.rs:187-206).  A contract is an
+    immutable value: its serde JSON is computed once and kept."""
 
-    __slots__ = ("node",)
+    __slots__ = ("node", "_json")
 
     def __init__(self, node):
         self.node = node
+        self._json = None
 
     def __matmul__(self, cond: Condition) -> "Contract":
         return Contract({"When": [cond.node, self.node]})
@@ -313,7 +315,9 @@ class Contract:
         return -self
 
     def to_json(self) -> str:
-        return _dump(self.node)
+        if self._json is None:
+            self._json = _dump(self.node)
+        return self._json
 
     @staticmethod
     def from_json(s: str) -> "Contract":

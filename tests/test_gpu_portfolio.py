@@ -72,17 +72,22 @@ def test_portfolio_tile_configs_and_graph_replay():
     jobs = small_book(24)
     want = price_each(jobs)
     try:
-        for bpc, chunk in ((0, 1), (2, 4), (8, 1), (1, 64)):
-            eng.set_option("portfolio_blocks_per_cu", bpc)
-            eng.set_option("portfolio_chunk", chunk)
-            assert_identical(K.price_portfolio(jobs), want)
-            assert_identical(K.price_portfolio(jobs), want)  # replay of the captured graph
+        for graph in (1, 0):
+            eng.set_option("portfolio_graph", graph)
+            for bpc, chunk in ((0, 1), (2, 4), (8, 1), (1, 64)):
+                eng.set_option("portfolio_blocks_per_cu", bpc)
+                eng.set_option("portfolio_chunk", chunk)
+                assert_identical(K.price_portfolio(jobs), want)
+                assert_identical(K.price_portfolio(jobs), want)  # replay of the captured graph
+            eng.set_option("portfolio_blocks_per_cu", 0)
+            eng.set_option("portfolio_chunk", 1)
+            other = small_book(10, paths=(2048,))  # another shape: a new graph
+            assert_identical(K.price_portfolio(other), price_each(other))
     finally:
         eng.set_option("portfolio_blocks_per_cu", 0)
         eng.set_option("portfolio_chunk", 1)
-    other = small_book(10, paths=(2048,))  # another shape: a new graph
-    assert_identical(K.price_portfolio(other), price_each(other))
-    for bad in (("portfolio_chunk", 0), ("portfolio_blocks_per_cu", -1)):
+        eng.set_option("portfolio_graph", 0)
+    for bad in (("portfolio_chunk", 0), ("portfolio_blocks_per_cu", -1), ("portfolio_graph", 2)):
         with pytest.raises(N.KontractError):
             eng.set_option(*bad)
 
