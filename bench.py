@@ -246,6 +246,28 @@ def main():
         extras["config4_portfolio_path_steps_per_s"] = B.path_steps(jobs) / tp
         extras["config4_identical_to_per_contract"] = all(a.price == b.price and a.std_error == b.std_error
                                                           for a, b in zip(rp, rs))
+        # the book with pathwise Greeks on its Lipschitz payoffs (European,
+        # Asian, basket): persistent Greeks launches + the price-only launch
+        # for the barriers, vs per contract pathwise Greeks / a price
+        K.price_portfolio(jobs, greeks=True)
+        t = time.perf_counter()
+        rg = K.price_portfolio(jobs, greeks=True)
+        tpg = time.perf_counter() - t
+        t = time.perf_counter()
+        sg = []
+        for i, job in enumerate(jobs):
+            kw = dict(n_paths=job[2], seed=job[3], steps_per_year=job[4], grid=job[5])
+            if B.KINDS[i % 4] == "barrier":
+                sg.append((job[0].price(job[1], **kw), None))
+            else:
+                sg.append((None, job[0].pathwise_greeks(job[1], **kw)))
+        tsg = time.perf_counter() - t
+        extras["config4_greeks_portfolio_s"] = tpg
+        extras["config4_greeks_per_contract_s"] = tsg
+        extras["config4_greeks_speedup"] = tsg / tpg
+        extras["config4_greeks_identical_to_per_contract"] = all(
+            (a[1] is None and a[0].price == b[0].price) or (a[1] is not None and a[1].values == b[1].values)
+            for a, b in zip(rg, sg))
         # launch-bound regime: a book of the same kinds at 2^14 paths and 3-month maturities (63 steps)
         small = B.book(1000, paths_exp=(14, 14), maturity=(0.25, 0.25))
         K.price_portfolio(small)
@@ -258,6 +280,20 @@ def main():
         tss = time.perf_counter() - t
         extras["config4_small_book_speedup"] = tss / tps
         extras["config4_small_book_portfolio_ms"] = tps * 1e3
+        K.price_portfolio(small, greeks=True)
+        t = time.perf_counter()
+        K.price_portfolio(small, greeks=True)
+        tpsg = time.perf_counter() - t
+        t = time.perf_counter()
+        for i, job in enumerate(small):
+            kw = dict(n_paths=job[2], seed=job[3], steps_per_year=job[4], grid=job[5])
+            if B.KINDS[i % 4] == "barrier":
+                job[0].price(job[1], **kw)
+            else:
+                job[0].pathwise_greeks(job[1], **kw)
+        tssg = time.perf_counter() - t
+        extras["config4_small_book_greeks_speedup"] = tssg / tpsg
+        extras["config4_small_book_greeks_portfolio_ms"] = tpsg * 1e3
         # Calibration-sized calls (1,000 paths x 50 steps): per-call latency of
         # a single price, and one objective evaluation over 10 quotes as one
         # portfolio call (direct launches, then graph replay) vs a loop.

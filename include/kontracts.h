@@ -222,6 +222,20 @@ typedef struct kt_job {
  * identical to kt_price on that job alone. */
 kt_status kt_price_portfolio(kt_engine* engine, const kt_job* jobs, uint64_t n_jobs, kt_price_result* out);
 
+/* kt_price_portfolio plus the pathwise Greeks of every job that admits them
+ * (the kt_greeks_adjoint payoffs: one flow date, arithmetic ops, running
+ * sums / extremes fed by a spot).  Row j of greeks / greeks_se (stride >= 24
+ * doubles, nullable) holds the derivatives of job j in kt_greeks_adjoint's
+ * order without the price (GBM {s0, sigma, mu}; Heston {s0, v0, kappa, theta,
+ * sigma_v, rho}; basket {s0_a, sigma_a, mu_a}), NaN for jobs priced without
+ * Greeks (barriers, digitals, variance-reduced configurations, programs too
+ * large for the Greeks kernels).  For a job with Greeks, out[j] and its row
+ * equal kt_greeks_adjoint on that job alone (its price comes from the Greeks
+ * kernel); for the others out[j] equals kt_price.  Covered jobs run in
+ * persistent launches over (job, block) queues, one per kernel variant. */
+kt_status kt_price_portfolio_greeks(kt_engine* engine, const kt_job* jobs, uint64_t n_jobs, kt_price_result* out,
+                                    double* greeks, double* greeks_se, uint64_t stride);
+
 /* Kernel timing: when enabled, HIP events on the engine's stream bracket every
  * path-kernel launch group; the totals accumulate until reset.  path_steps
  * counts paths x grid intervals x scenarios simulated by those launches. */

@@ -143,6 +143,8 @@ _SIGNATURES = {
          _P(C.c_double), _P(C.c_double), _P(KtPriceResult)],
     ),
     "kt_price_portfolio": (C.c_int, [C.c_void_p, _P(KtJob), C.c_uint64, _P(KtPriceResult)]),
+    "kt_price_portfolio_greeks": (
+        C.c_int, [C.c_void_p, _P(KtJob), C.c_uint64, _P(KtPriceResult), _P(C.c_double), _P(C.c_double), C.c_uint64]),
     "kt_greek_surface": (
         C.c_int,
         [C.c_void_p, C.c_char_p, C.c_char_p, _P(C.c_double), C.c_uint64, _P(C.c_double), C.c_uint64,

@@ -30,7 +30,8 @@ namespace {
 constexpr float kLn2f = 0.693147180559945309f;
 constexpr float kLog2ef = 1.44269504088896341f;
 
-__device__ __forceinline__ int acc_index(const AdjArgs& A, uint32_t slot) {
+template <class AA>
+__device__ __forceinline__ int acc_index(const AA& A, uint32_t slot) {
   int q = 0;
 #pragma unroll
   for (int i = 1; i < kAdjMaxAcc; ++i)
@@ -43,10 +44,10 @@ __device__ __forceinline__ int acc_index(const AdjArgs& A, uint32_t slot) {
 // `leaf` at this point; leaf = K + q: accumulator q).  Accumulator registers
 // keep their forward values (their update ops only set the leaf's tangent),
 // every other register is recomputed.  Returns d(pv)/d(leaf) through EMIT.
-template <int NS, int K>
-__device__ float dual_point(const AdjArgs& A, int k, int begin, int count, const float (&S)[K],
+template <int NS, int K, class AA>
+__device__ float dual_point(const AA& A, int k, int begin, int count, const float (&S)[K],
                             const float (&r)[1][NS], int leaf) {
-  const PathArgs& P = A.p;
+  const auto& P = A.p;
   float rv[NS], rt[NS];
 #pragma unroll
   for (int i = 0; i < NS; ++i) {
@@ -114,9 +115,10 @@ __device__ float dual_point(const AdjArgs& A, int k, int begin, int count, const
   return dpv;
 }
 
-// Per-block reduction of NOUT per-path samples into partials[block][NOUT].
+// Per-block reduction of NOUT per-path samples into partials[blk][NOUT].
 template <int NOUT>
-__device__ __forceinline__ void block_stats_out(const double (&vals)[NOUT], bool live, KStats* partials) {
+__device__ __forceinline__ void block_stats_out(const double (&vals)[NOUT], bool live, KStats* partials,
+                                                uint32_t blk) {
   __shared__ KStats sh[kBlock / 64][NOUT];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
@@ -129,14 +131,15 @@ __device__ __forceinline__ void block_stats_out(const double (&vals)[NOUT], bool
     KStats t = sh[0][threadIdx.x];
 #pragma unroll
     for (int wv = 1; wv < kBlock / 64; ++wv) t = stats_combine(t, sh[wv][threadIdx.x]);
-    partials[(size_t)blockIdx.x * NOUT + threadIdx.x] = t;
+    partials[(size_t)blk * NOUT + threadIdx.x] = t;
   }
+  __syncthreads();  // sh is reused by the block's next work item
 }
 
 // Forward part shared by both kernels: one grid point's program ops, with
 // the point at which each running extreme was last set.
-template <int NS, int K>
-__device__ __forceinline__ void run_point_ops(const AdjArgs& A, int k, float (&r)[1][NS], double (&pv)[1][1],
+template <int NS, int K, class AA>
+__device__ __forceinline__ void run_point_ops(const AA& A, int k, float (&r)[1][NS], double (&pv)[1][1],
                                               const float (&x)[1][K], bool live, uint32_t fm,
                                               int (&argstep)[kAdjMaxAcc]) {
   const uint64_t st = ld_u64(A.p.steps, (size_t)k);
@@ -151,8 +154,8 @@ __device__ __forceinline__ void run_point_ops(const AdjArgs& A, int k, float (&r
     if (((fm >> q) & 1u) && A.acc_kind[q] != OP_AVG && r[0][A.acc_slot[q] & (NS - 1)] != prev[q]) argstep[q] = k;
 }
 
-template <int NS>
-__device__ __forceinline__ void init_registers(const PathArgs& P, float (&r)[1][NS]) {
+template <int NS, class PA>
+__device__ __forceinline__ void init_registers(const PA& P, float (&r)[1][NS]) {
 #pragma unroll
   for (int i = 0; i < NS; ++i) r[0][i] = 0.0f;
   for (int i = 0; i < P.n_init; ++i) {
@@ -163,14 +166,15 @@ __device__ __forceinline__ void init_registers(const PathArgs& P, float (&r)[1][
 
 }  // namespace
 
-// GBM (K = 1) and correlated GBM (K >= 2).  Outputs: price, d/ds0_a, d/dsigma_a, d/dmu_a.
-template <int K, int NS>
-__global__ __launch_bounds__(kBlock) void adjoint_gbm_kernel(const AdjArgs A) {
+// GBM (K = 1) and correlated GBM (K >= 2): block blk of the job (paths
+// blk*256 .. blk*256+255).  Outputs: price, d/ds0_a, d/dsigma_a, d/dmu_a.
+template <int K, int NS, class AA>
+__device__ __forceinline__ void adjoint_gbm_block(const AA& A, uint32_t blk) {
   constexpr int MODEL = K == 1 ? MK_GBM : MK_BASKET;
   constexpr int M = K;
   constexpr int NP = 3 * K, NOUT = NP + 1;
-  const PathArgs& P = A.p;
-  const uint64_t local = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+  const auto& P = A.p;
+  const uint64_t local = (uint64_t)blk * kBlock + threadIdx.x;
   const bool live = local < P.n_paths;
   const uint64_t path = P.path_base + local;
   const uint32_t p_lo = (uint32_t)path, p_hi = (uint32_t)(path >> 32);
@@ -292,12 +296,18 @@ __global__ __launch_bounds__(kBlock) void adjoint_gbm_kernel(const AdjArgs A) {
   vals[0] = pv[0][0];
 #pragma unroll
   for (int i = 0; i < NP; ++i) vals[1 + i] = (double)gr[i];
-  block_stats_out<NOUT>(vals, live, P.partials);
+  block_stats_out<NOUT>(vals, live, P.partials, blk);
+}
+
+template <int K, int NS>
+__global__ __launch_bounds__(kBlock) void adjoint_gbm_kernel(const AdjArgs A) {
+  adjoint_gbm_block<K, NS>(A, blockIdx.x);
 }
 
 // One Heston Euler step of the variance only (the same expression as
 // advance<MK_HESTON>, so the recomputed variance is bit-identical).
-__device__ __forceinline__ float heston_v_step(const PathArgs& P, int k, float z1, float z2, float v) {
+template <class PA>
+__device__ __forceinline__ float heston_v_step(const PA& P, int k, float z1, float z2, float v) {
   const float t3 = ld_f32(P.tab, (size_t)kHestonStride * k + 3);
   const float t4 = ld_f32(P.tab, (size_t)kHestonStride * k + 4);
   const float t5 = ld_f32(P.tab, (size_t)kHestonStride * k + 5);
@@ -315,16 +325,17 @@ __device__ __forceinline__ float heston_v_step(const PathArgs& P, int k, float z
 // walked back by inverting the log step); without them only the last point
 // seeds d/dx, which then stays constant, and the sweep carries the variance
 // alone.
-template <int NS, int SEG, bool FEEDS>
-__global__ __launch_bounds__(kBlock) void adjoint_heston_kernel(const AdjArgs A) {
+// Block blk of the job; this lane's checkpoints are ck[(2c + i) * cs + lane].
+template <int NS, int SEG, bool FEEDS, class AA>
+__device__ __forceinline__ void adjoint_heston_block(const AA& A, uint32_t blk, float* __restrict__ ck, size_t cs,
+                                                     size_t lane) {
   constexpr int NOUT = 7;
-  const PathArgs& P = A.p;
-  const uint64_t local = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+  const auto& P = A.p;
+  const uint64_t local = (uint64_t)blk * kBlock + threadIdx.x;
   const bool live = local < P.n_paths;
   const uint64_t path = P.path_base + local;
   const uint32_t p_lo = (uint32_t)path, p_hi = (uint32_t)(path >> 32);
   const int n = P.n_grid - 1;
-  const size_t cs = A.ck_stride;
   float r[1][NS];
   init_registers<NS>(P, r);
   float x[1][1] = {{0.0f}};
@@ -349,8 +360,8 @@ __global__ __launch_bounds__(kBlock) void adjoint_heston_kernel(const AdjArgs A)
       if (k <= n) {
         if ((k & (SEG - 1)) == 0 && live) {
           const size_t c = (size_t)(k / SEG);
-          if constexpr (FEEDS) A.ck[(2 * c) * cs + path] = x[0][0];
-          A.ck[(2 * c + 1) * cs + path] = v;
+          if constexpr (FEEDS) ck[(2 * c) * cs + lane] = x[0][0];
+          ck[(2 * c + 1) * cs + lane] = v;
         }
         advance<MK_HESTON, 1, 1>(P, k, &z[2 * s], x, v);
         run_point_ops<NS, 1>(A, k, r, pv, x, live, FEEDS ? ld_u32(A.feed, (size_t)k) : 0u, argstep);
@@ -387,8 +398,8 @@ __global__ __launch_bounds__(kBlock) void adjoint_heston_kernel(const AdjArgs A)
   const int n_seg = n / SEG + 1;
   for (int c = n_seg - 1; c >= 0; --c) {
     // recompute segment c (points SEG*c .. SEG*c + SEG - 1) from its checkpoint
-    float xs[1][1] = {{FEEDS && live ? A.ck[(2 * (size_t)c) * cs + path] : 0.0f}};
-    float vs = live ? A.ck[(2 * (size_t)c + 1) * cs + path] : 0.0f;
+    float xs[1][1] = {{FEEDS && live ? ck[(2 * (size_t)c) * cs + lane] : 0.0f}};
+    float vs = live ? ck[(2 * (size_t)c + 1) * cs + lane] : 0.0f;
     float vb[SEG], z1s[SEG], z2s[SEG];
 #pragma unroll
     for (int gq = 0; gq < SEG / 4; ++gq) {
@@ -462,7 +473,57 @@ __global__ __launch_bounds__(kBlock) void adjoint_heston_kernel(const AdjArgs A)
     }
   }
   double vals[NOUT] = {pv[0][0], (double)g_s0, (double)vbar, (double)g_k, (double)g_th, (double)g_sv, (double)g_rho};
-  block_stats_out<NOUT>(vals, live, P.partials);
+  block_stats_out<NOUT>(vals, live, P.partials, blk);
+}
+
+template <int NS, int SEG, bool FEEDS>
+__global__ __launch_bounds__(kBlock) void adjoint_heston_kernel(const AdjArgs A) {
+  const uint64_t path = A.p.path_base + (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+  adjoint_heston_block<NS, SEG, FEEDS>(A, blockIdx.x, A.ck, A.ck_stride, path);
+}
+
+// ---------------------------------------------------------------------------
+// Persistent portfolio launch of the pathwise Greeks: one launch per variant
+// (each with the register budget of its own block code) takes (job, block)
+// items from a queue and runs the same block function as the per-contract
+// launch, so every job's statistics equal its kt_greeks_adjoint result.  The
+// Heston checkpoints live in a per-workgroup slot (the workgroup finishes an
+// item's forward and backward sweeps before it takes the next item).
+// ---------------------------------------------------------------------------
+template <int VARIANT>
+__global__ __launch_bounds__(kBlock) void portfolio_adjoint_kernel(const AdjPortfolioArgs Q) {
+  __shared__ uint32_t s_first;
+  const KT_CONST AdjArgs* jobs = (const KT_CONST AdjArgs*)Q.jobs;
+  const KT_CONST uint32_t* items = (const KT_CONST uint32_t*)Q.items;
+  float* ck = Q.ck ? Q.ck + (size_t)blockIdx.x * Q.ck_floats : nullptr;
+  for (;;) {
+    __syncthreads();  // every lane has read the previous s_first
+    if (threadIdx.x == 0) s_first = atomicAdd(Q.counter, (unsigned)Q.chunk);
+    __syncthreads();
+    const uint32_t first = __builtin_amdgcn_readfirstlane(s_first);
+    if (first >= Q.n_items) return;
+    const uint32_t last = min(first + (uint32_t)Q.chunk, Q.n_items);
+    for (uint32_t it = first; it < last; ++it) {
+      const uint32_t j = items[2 * (size_t)it], blk = items[2 * (size_t)it + 1];
+      const KT_CONST AdjArgs& A = jobs[j];
+      if constexpr (VARIANT == APV_GBM) adjoint_gbm_block<1, 16>(A, blk);
+      if constexpr (VARIANT == APV_HESTON) adjoint_heston_block<16, kAdjSeg, false>(A, blk, ck, kBlock, threadIdx.x);
+      if constexpr (VARIANT == APV_HESTON_FEEDS)
+        adjoint_heston_block<16, kAdjSeg, true>(A, blk, ck, kBlock, threadIdx.x);
+      if constexpr (VARIANT == APV_BASKET5) adjoint_gbm_block<5, 16>(A, blk);
+    }
+  }
+}
+
+// Block b: output b % kAdjMaxOut of job b / kAdjMaxOut (outputs beyond the
+// job's count are skipped), into out[job][kAdjMaxOut].
+__global__ __launch_bounds__(256) void portfolio_adjoint_finalize_kernel(const AdjArgs* __restrict__ jobs,
+                                                                         const int* __restrict__ n_blocks,
+                                                                         const int* __restrict__ n_out,
+                                                                         KStats* __restrict__ out) {
+  const int job = blockIdx.x / kAdjMaxOut, j = blockIdx.x % kAdjMaxOut;
+  if (j >= n_out[job]) return;
+  finalize_body(jobs[job].p.partials, n_blocks[job], n_out[job], j, out + (size_t)job * kAdjMaxOut + j);
 }
 
 int adjoint_outputs(int model, int k_assets) { return model == MK_HESTON ? 7 : 1 + 3 * k_assets; }
@@ -470,8 +531,61 @@ int adjoint_outputs(int model, int k_assets) { return model == MK_HESTON ? 7 : 1
 bool adjoint_variant_exists(int model, int k_assets, int n_slots, int* ns_chosen) {
   if (n_slots > 32) return false;
   if (model == MK_BASKET && (k_assets < 2 || k_assets > kMaxAssets)) return false;
-  *ns_chosen = (n_slots <= 16 && model != MK_BASKET) ? 16 : 32;
+  *ns_chosen = (n_slots <= 16 && (model != MK_BASKET || k_assets <= 5)) ? 16 : 32;
   return true;
+}
+
+int portfolio_adjoint_variant(int model, int k_assets, int n_slots, int n_acc) {
+  if (n_slots > 16) return -1;
+  if (model == MK_GBM) return APV_GBM;
+  if (model == MK_HESTON) return n_acc > 0 ? APV_HESTON_FEEDS : APV_HESTON;
+  if (model == MK_BASKET && k_assets == 5) return APV_BASKET5;
+  return -1;
+}
+
+int portfolio_adjoint_occupancy(int variant) {
+  int n = 0;
+  hipError_t e = hipErrorInvalidValue;
+  switch (variant) {
+    case APV_GBM: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, portfolio_adjoint_kernel<APV_GBM>, kBlock, 0); break;
+    case APV_HESTON:
+      e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, portfolio_adjoint_kernel<APV_HESTON>, kBlock, 0);
+      break;
+    case APV_HESTON_FEEDS:
+      e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, portfolio_adjoint_kernel<APV_HESTON_FEEDS>, kBlock, 0);
+      break;
+    case APV_BASKET5:
+      e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, portfolio_adjoint_kernel<APV_BASKET5>, kBlock, 0);
+      break;
+  }
+  if (e != hipSuccess) return 1;
+  return n > 0 ? n : 1;
+}
+
+hipError_t launch_portfolio_adjoint(int variant, const AdjPortfolioArgs& q, int grid, hipStream_t s) {
+  if (grid <= 0) return hipErrorInvalidValue;
+  switch (variant) {
+    case APV_GBM: hipLaunchKernelGGL(portfolio_adjoint_kernel<APV_GBM>, dim3(grid), dim3(kBlock), 0, s, q); break;
+    case APV_HESTON:
+      hipLaunchKernelGGL(portfolio_adjoint_kernel<APV_HESTON>, dim3(grid), dim3(kBlock), 0, s, q);
+      break;
+    case APV_HESTON_FEEDS:
+      hipLaunchKernelGGL(portfolio_adjoint_kernel<APV_HESTON_FEEDS>, dim3(grid), dim3(kBlock), 0, s, q);
+      break;
+    case APV_BASKET5:
+      hipLaunchKernelGGL(portfolio_adjoint_kernel<APV_BASKET5>, dim3(grid), dim3(kBlock), 0, s, q);
+      break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_portfolio_adjoint_finalize(const AdjArgs* jobs, const int* n_blocks, const int* n_out, int n_jobs,
+                                             KStats* out, hipStream_t s) {
+  if (n_jobs <= 0) return hipSuccess;
+  hipLaunchKernelGGL(portfolio_adjoint_finalize_kernel, dim3(n_jobs * kAdjMaxOut), dim3(256), 0, s, jobs, n_blocks,
+                     n_out, out);
+  return hipGetLastError();
 }
 
 hipError_t launch_adjoint(int model, int k, int ns, const AdjArgs& a, int n_blocks, hipStream_t s) {
@@ -492,6 +606,14 @@ hipError_t launch_adjoint(int model, int k, int ns, const AdjArgs& a, int n_bloc
       case 5: KT_ADJ((adjoint_heston_kernel<32, 16, true>)); break;
       case 6: KT_ADJ((adjoint_heston_kernel<32, 32, false>)); break;
       default: KT_ADJ((adjoint_heston_kernel<32, 32, true>)); break;
+    }
+  } else if (ns == 16) {
+    switch (k) {
+      case 2: KT_ADJ((adjoint_gbm_kernel<2, 16>)); break;
+      case 3: KT_ADJ((adjoint_gbm_kernel<3, 16>)); break;
+      case 4: KT_ADJ((adjoint_gbm_kernel<4, 16>)); break;
+      case 5: KT_ADJ((adjoint_gbm_kernel<5, 16>)); break;
+      default: return hipErrorInvalidValue;
     }
   } else {
     switch (k) {

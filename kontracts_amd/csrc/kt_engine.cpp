@@ -763,6 +763,18 @@ kt_status kt_price_portfolio(kt_engine* E, const kt_job* jobs, uint64_t n_jobs, 
   KT_GUARD_END(E)
 }
 
+kt_status kt_price_portfolio_greeks(kt_engine* E, const kt_job* jobs, uint64_t n_jobs, kt_price_result* out,
+                                    double* greeks, double* greeks_se, uint64_t stride) {
+  KT_GUARD_BEGIN
+  if (!E) raise(KT_ERR_INVALID_ARGUMENT, "null engine");
+  if (n_jobs == 0) return KT_OK;
+  check_hip(hipSetDevice(E->device), "hipSetDevice");
+  price_portfolio_greeks(E, jobs, n_jobs, out, greeks, greeks_se, stride);
+  E->err.clear();
+  return KT_OK;
+  KT_GUARD_END(E)
+}
+
 kt_status kt_debug_philox(kt_engine* E, const uint32_t* ctr, const uint32_t* key, uint64_t n, uint32_t* out) {
   KT_GUARD_BEGIN
   if (!E || !ctr || !key || !out) raise(KT_ERR_INVALID_ARGUMENT, "null argument");

@@ -52,6 +52,7 @@ struct kt_engine {
   bool errp_zero = false;
   int occupancy_portfolio = 0;  // cached occupancies (blocks per CU)
   int occupancy_surface = 0;
+  int occupancy_adjoint[kt::APV_COUNT] = {0, 0, 0, 0};
   bool profiling = false;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   double prof_ms = 0.0, prof_path_steps = 0.0;
@@ -145,15 +146,36 @@ void greek_surface(kt_engine* E, const char* json, const char* asset, const doub
                    const double* vols, uint64_t n_vols, const kt_mc_config* cfg, const kt_bumps& h, kt_greeks* out);
 void greeks_bump(kt_engine* E, const char* json, const kt_model* model, const kt_mc_config* cfg, const double* h,
                  uint64_t n_params, double* out, double* stderr_out);
+// stats_out (nullable): the per-output statistics (price first).
 void greeks_adjoint(kt_engine* E, const char* json, const kt_model* model, const kt_mc_config* cfg, double* out,
-                    double* stderr_out, uint64_t n_out);
+                    double* stderr_out, uint64_t n_out, std::vector<KStats>* stats_out = nullptr);
 // pv_host[n_paths] and coef_host[(n_dates-1) x (n_basis+1)] (nullable)
 // receive each path's discounted value and each date's regression
 // coefficients followed by a fitted flag.
 void price_american(kt_engine* E, const char* json, const double* dates, uint64_t n_dates, const kt_model* model,
                     const kt_mc_config* cfg, int n_basis, kt_price_result* out, double* pv_host = nullptr,
                     double* coef_host = nullptr);
-void price_portfolio(kt_engine* E, const kt_job* jobs, uint64_t n_jobs, kt_price_result* out);
+// job_index (nullable): the caller's number of each job, for messages.
+void price_portfolio(kt_engine* E, const kt_job* jobs, uint64_t n_jobs, kt_price_result* out,
+                     const uint64_t* job_index = nullptr);
+// Host tables of one pathwise-Greeks job (the per-contract launch and the
+// portfolio share them): blobs in the order fill_adjoint_args expects.
+struct AdjJobHost {
+  std::vector<KOp> ops;
+  std::vector<float> tab, adj, thr;
+  std::vector<double> df;
+  int stride = 0;
+  AdjPlan plan;
+};
+AdjJobHost adjoint_job_host(const Program& prog, const ModelDesc& md, const std::vector<double>& grid, double rate,
+                            AdjPlan plan);
+std::vector<std::pair<const void*, size_t>> adjoint_blobs(const AdjJobHost& H, const Program& prog);
+void fill_adjoint_args(AdjArgs& A, const AdjJobHost& H, const Program& prog, const ModelDesc& md,
+                       const std::vector<double>& grid, uint64_t seed, const std::vector<void*>& dev);
+// Pricing of a portfolio with the pathwise Greeks of every job they cover.
+void price_portfolio_greeks(kt_engine* E, const kt_job* jobs, uint64_t n_jobs, kt_price_result* out, double* greeks,
+                            double* greeks_se, uint64_t stride);
+
 // kt_vr.cpp: price_gbm with variance reduction (antithetic pairs and/or the
 // at-the-money call control variate).
 kt_price_result price_vr(kt_engine* E, const Con& con, const ModelDesc& md, const std::vector<double>& grid,

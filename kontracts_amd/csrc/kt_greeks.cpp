@@ -338,8 +338,75 @@ void greeks_bump(kt_engine* E, const char* json, const kt_model* model, const kt
 
 // Pathwise Greeks (kt_adjoint.hip): one launch; per-path samples of the
 // price and of each parameter derivative, reduced in FP64 like prices.
+AdjJobHost adjoint_job_host(const Program& prog, const ModelDesc& md, const std::vector<double>& grid, double rate,
+                            AdjPlan plan) {
+  AdjJobHost H;
+  H.tab = model_table(md, grid, &H.stride);
+  H.df.resize(grid.size());
+  for (size_t k = 0; k < grid.size(); ++k) H.df[k] = std::exp(-rate * grid[k]);
+  H.adj.assign(2 * grid.size(), 0.0f);
+  for (size_t k = 1; k < grid.size(); ++k) {
+    const double dt = grid[k] - grid[k - 1];
+    if (dt > 0.0) {
+      H.adj[2 * k] = (float)dt;
+      H.adj[2 * k + 1] = (float)std::sqrt(dt);
+    }
+  }
+  H.ops = prog.ops;
+  if (H.ops.empty()) H.ops.push_back(KOp{OP_NOP, 0, 0, 0, 0.0f});
+  H.thr.assign(1, 0.0f);
+  H.plan = std::move(plan);
+  return H;
+}
+
+std::vector<std::pair<const void*, size_t>> adjoint_blobs(const AdjJobHost& H, const Program& prog) {
+  return {{H.ops.data(), H.ops.size() * sizeof(KOp)},
+          {prog.steps.data(), prog.steps.size() * sizeof(KStep)},
+          {prog.groups.data(), prog.groups.size() * sizeof(KGroup)},
+          {prog.init.data(), prog.init.size() * sizeof(KOp)},
+          {H.df.data(), H.df.size() * sizeof(double)},
+          {H.tab.data(), H.tab.size() * sizeof(float)},
+          {H.thr.data(), H.thr.size() * sizeof(float)},
+          {H.adj.data(), H.adj.size() * sizeof(float)},
+          {H.plan.feed.data(), H.plan.feed.size() * sizeof(uint32_t)}};
+}
+
+void fill_adjoint_args(AdjArgs& A, const AdjJobHost& H, const Program& prog, const ModelDesc& md,
+                       const std::vector<double>& grid, uint64_t seed, const std::vector<void*>& dev) {
+  PathArgs& P = A.p;
+  P.ops = (const KOp*)dev[0];
+  P.steps = (const KStep*)dev[1];
+  P.groups = (const KGroup*)dev[2];
+  P.init = (const KOp*)dev[3];
+  P.df = (const double*)dev[4];
+  P.tab = (const float*)dev[5];
+  P.thr = (const float*)dev[6];
+  P.n_init = (int)prog.init.size();
+  P.n_grid = (int)grid.size();
+  P.n_groups = (int)((grid.size() + 3) / 4);
+  P.tab_stride = H.stride;
+  fill_model_args(P, md, seed);
+  A.adj_tab = (const float*)dev[7];
+  A.feed = (const uint32_t*)dev[8];
+  A.n_acc = H.plan.n_acc;
+  for (int q = 0; q < 4; ++q) {
+    A.acc_slot[q] = H.plan.slot[q];
+    A.acc_kind[q] = H.plan.kind[q];
+    A.acc_asset[q] = H.plan.asset[q];
+  }
+  const int K = (int)md.names.size();
+  for (int a = 0; a < K && a < kMaxAssets; ++a) {
+    A.sigma[a] = (float)md.sigma[a];
+    A.mu[a] = (float)md.mu[a];
+  }
+  A.kappa = (float)md.kappa;
+  A.theta = (float)md.theta;
+  A.sigma_v = (float)md.sigma_v;
+  A.seg = kAdjSeg;
+}
+
 void greeks_adjoint(kt_engine* E, const char* json, const kt_model* model, const kt_mc_config* cfg, double* out,
-                    double* stderr_out, uint64_t n_out) {
+                    double* stderr_out, uint64_t n_out, std::vector<KStats>* stats_out) {
   ModelDesc md = model_from_abi(model);
   PreparedContract pc = prepare(json, md, cfg);
   const int kind = model_kernel_kind(md);
@@ -366,30 +433,8 @@ void greeks_adjoint(kt_engine* E, const char* json, const kt_model* model, const
   const uint64_t N = cfg->n_paths;
   if (N == 0) return;
 
-  int stride = 0;
-  std::vector<float> tab = model_table(md, grid, &stride);
-  std::vector<double> df(grid.size());
-  for (size_t k = 0; k < grid.size(); ++k) df[k] = std::exp(-cfg->rate * grid[k]);
-  std::vector<float> adj(2 * grid.size(), 0.0f);
-  for (size_t k = 1; k < grid.size(); ++k) {
-    const double dt = grid[k] - grid[k - 1];
-    if (dt > 0.0) {
-      adj[2 * k] = (float)dt;
-      adj[2 * k + 1] = (float)std::sqrt(dt);
-    }
-  }
-  std::vector<KOp> ops = prog.ops;
-  if (ops.empty()) ops.push_back(KOp{OP_NOP, 0, 0, 0, 0.0f});
-  std::vector<float> thr(1, 0.0f);
-  auto dev = E->upload({{ops.data(), ops.size() * sizeof(KOp)},
-                        {prog.steps.data(), prog.steps.size() * sizeof(KStep)},
-                        {prog.groups.data(), prog.groups.size() * sizeof(KGroup)},
-                        {prog.init.data(), prog.init.size() * sizeof(KOp)},
-                        {df.data(), df.size() * sizeof(double)},
-                        {tab.data(), tab.size() * sizeof(float)},
-                        {thr.data(), thr.size() * sizeof(float)},
-                        {adj.data(), adj.size() * sizeof(float)},
-                        {plan.feed.data(), plan.feed.size() * sizeof(uint32_t)}});
+  AdjJobHost H = adjoint_job_host(prog, md, grid, cfg->rate, std::move(plan));
+  auto dev = E->upload(adjoint_blobs(H, prog));
 
   const uint64_t total_blocks = (N + kBlock - 1) / kBlock;
   if (total_blocks > (1ull << 31)) raise(KT_ERR_INVALID_ARGUMENT, "too many paths");
@@ -401,36 +446,10 @@ void greeks_adjoint(kt_engine* E, const char* json, const kt_model* model, const
   if (kind == MK_HESTON) ck = (float*)E->ensure(E->aux, n_ck * 2 * N * sizeof(float));
 
   AdjArgs A{};
+  fill_adjoint_args(A, H, prog, md, grid, cfg->seed, dev);
   PathArgs& P = A.p;
-  P.ops = (const KOp*)dev[0];
-  P.steps = (const KStep*)dev[1];
-  P.groups = (const KGroup*)dev[2];
-  P.init = (const KOp*)dev[3];
-  P.df = (const double*)dev[4];
-  P.tab = (const float*)dev[5];
-  P.thr = (const float*)dev[6];
-  P.n_init = (int)prog.init.size();
-  P.n_grid = (int)grid.size();
-  P.n_groups = (int)((grid.size() + 3) / 4);
-  P.tab_stride = stride;
-  fill_model_args(P, md, cfg->seed);
-  A.adj_tab = (const float*)dev[7];
-  A.feed = (const uint32_t*)dev[8];
   A.ck = ck;
   A.ck_stride = N;
-  A.n_acc = plan.n_acc;
-  for (int q = 0; q < 4; ++q) {
-    A.acc_slot[q] = plan.slot[q];
-    A.acc_kind[q] = plan.kind[q];
-    A.acc_asset[q] = plan.asset[q];
-  }
-  for (int a = 0; a < K && a < kMaxAssets; ++a) {
-    A.sigma[a] = (float)md.sigma[a];
-    A.mu[a] = (float)md.mu[a];
-  }
-  A.kappa = (float)md.kappa;
-  A.theta = (float)md.theta;
-  A.sigma_v = (float)md.sigma_v;
   A.seg = seg;
 
   E->prof_begin();
@@ -452,6 +471,7 @@ void greeks_adjoint(kt_engine* E, const char* json, const kt_model* model, const
     out[j] = r.price;
     if (stderr_out) stderr_out[j] = r.std_error;
   }
+  if (stats_out) *stats_out = st;
 }
 
 }  // namespace kt

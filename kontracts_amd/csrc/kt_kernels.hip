@@ -302,35 +302,6 @@ __global__ __launch_bounds__(kBlock) void path_kernel(const PathArgs A) {
   path_block<MODEL, K, NS, NOUT, SC, NSTAT, DEBUG, ANTI>(A, blockIdx.x, r);
 }
 
-// Deterministic combine of per-block statistics of output j (whole block).
-__device__ __forceinline__ void finalize_body(const KStats* __restrict__ partials, int n_blocks, int nout, int j,
-                                              KStats* __restrict__ out) {
-  KStats acc{0.0, 0.0, 0.0};
-  // Thread t takes blocks t, t + 256, ... eight at a time: the eight loads
-  // issue together and are combined as a pairwise tree (fixed order).
-  for (int b0 = threadIdx.x; b0 < n_blocks; b0 += 8 * 256) {
-    KStats q[8];
-#pragma unroll
-    for (int u = 0; u < 8; ++u) {
-      const int b = b0 + u * 256;
-      q[u] = b < n_blocks ? partials[(size_t)b * nout + j] : KStats{0.0, 0.0, 0.0};
-    }
-#pragma unroll
-    for (int w = 1; w < 8; w *= 2)
-#pragma unroll
-      for (int u = 0; u < 8; u += 2 * w) q[u] = stats_combine(q[u], q[u + w]);
-    acc = stats_combine(acc, q[0]);
-  }
-  __shared__ KStats sh[256];
-  sh[threadIdx.x] = acc;
-  __syncthreads();
-  for (int s = 128; s > 0; s >>= 1) {
-    if (threadIdx.x < s) sh[threadIdx.x] = stats_combine(sh[threadIdx.x], sh[threadIdx.x + s]);
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) *out = sh[0];
-}
-
 // Block j reduces output j.  err (nullable): block 0 copies the launch's
 // error word to *err_copy and clears it for the next launch.
 __global__ __launch_bounds__(256) void finalize_kernel(const KStats* __restrict__ partials, int n_blocks, int nout,

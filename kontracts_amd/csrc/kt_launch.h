@@ -92,6 +92,27 @@ int adjoint_outputs(int model, int k_assets);
 bool adjoint_variant_exists(int model, int k_assets, int n_slots, int* ns_chosen);
 hipError_t launch_adjoint(int model, int k_assets, int ns, const AdjArgs& a, int n_blocks, hipStream_t s);
 
+// Pathwise Greeks of a portfolio: one persistent launch per variant over
+// (job, block) items of that variant's jobs (16 registers, checkpoint
+// interval kAdjSeg).
+enum AdjPortfolioVariant { APV_GBM = 0, APV_HESTON = 1, APV_HESTON_FEEDS = 2, APV_BASKET5 = 3, APV_COUNT = 4 };
+struct AdjPortfolioArgs {
+  const AdjArgs* jobs;    // every adjoint job of the portfolio
+  const uint32_t* items;  // [n_items][2] {job, block within the job}
+  uint32_t n_items;
+  int chunk;              // items taken per queue fetch
+  unsigned* counter;      // queue head, zero at launch
+  float* ck;              // Heston: checkpoint slot of each workgroup, ck_floats each
+  uint64_t ck_floats;     // >= n_ck * 2 * kBlock of every Heston job of the launch
+};
+// -1 when no portfolio variant covers the job (it runs per contract).
+int portfolio_adjoint_variant(int model, int k_assets, int n_slots, int n_acc);
+int portfolio_adjoint_occupancy(int variant);
+hipError_t launch_portfolio_adjoint(int variant, const AdjPortfolioArgs& q, int grid, hipStream_t s);
+// out[job][kAdjMaxOut]: the first n_out[job] outputs of each job.
+hipError_t launch_portfolio_adjoint_finalize(const AdjArgs* jobs, const int* n_blocks, const int* n_out, int n_jobs,
+                                             KStats* out, hipStream_t s);
+
 // Longstaff-Schwartz kernels.
 constexpr int kLsmBasisMax = 6;
 constexpr int kLsmRegCount = (2 * kLsmBasisMax - 1) + kLsmBasisMax + 1;

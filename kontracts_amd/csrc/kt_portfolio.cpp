@@ -54,7 +54,10 @@ int variant_of(const ModelDesc& md, const Program& prog) {
 
 }  // namespace
 
-void price_portfolio(kt_engine* E, const kt_job* jobs, uint64_t n_jobs, kt_price_result* out) {
+void price_portfolio(kt_engine* E, const kt_job* jobs, uint64_t n_jobs, kt_price_result* out,
+                     const uint64_t* job_index) {
+  // job numbers in messages: the caller's numbering when it passes a subset
+  auto jid = [&](uint64_t j) { return std::to_string(job_index ? job_index[j] : j); };
   if (n_jobs == 0) return;
   if (!jobs || !out) raise(KT_ERR_INVALID_ARGUMENT, "null jobs or output");
   if (n_jobs > (1ull << 24)) raise(KT_ERR_INVALID_ARGUMENT, "too many jobs");
@@ -64,7 +67,7 @@ void price_portfolio(kt_engine* E, const kt_job* jobs, uint64_t n_jobs, kt_price
   std::vector<uint64_t> book;  // jobs in the persistent launch
   for (uint64_t j = 0; j < n_jobs; ++j) {
     const kt_job& J = jobs[j];
-    if (!J.contract_json) raise(KT_ERR_INVALID_ARGUMENT, "null contract in job " + std::to_string(j));
+    if (!J.contract_json) raise(KT_ERR_INVALID_ARGUMENT, "null contract in job " + jid(j));
     JobPlan& P = plans[j];
     P.md = model_from_abi(&J.model);
     PreparedContract pc = prepare(J.contract_json, P.md, &J.cfg, true);
@@ -91,7 +94,7 @@ void price_portfolio(kt_engine* E, const kt_job* jobs, uint64_t n_jobs, kt_price
     for (size_t k = 0; k < P.grid.size(); ++k) P.df[k] = std::exp(-J.cfg.rate * P.grid[k]);
     P.groups = knockout_tables(P.prog, {P.md}, &P.thr);
     P.n_blocks = (J.cfg.n_paths + kBlock - 1) / kBlock;
-    if (P.n_blocks > 0xffffffffull) raise(KT_ERR_INVALID_ARGUMENT, "too many paths in job " + std::to_string(j));
+    if (P.n_blocks > 0xffffffffull) raise(KT_ERR_INVALID_ARGUMENT, "too many paths in job " + jid(j));
     const double per_point = P.variant == PV_GBM ? 40.0 : P.variant == PV_HESTON ? 85.0 : 41.0 * P.md.names.size();
     P.cost = per_point * (double)P.grid.size() + 4.0 * (double)P.prog.ops.size();
     book.push_back(j);
@@ -259,7 +262,7 @@ void price_portfolio(kt_engine* E, const kt_job* jobs, uint64_t n_jobs, kt_price
       if (j > first_err) break;
       out[j] = price_solo(j);
     }
-    if (first_code) raise(first_code, error_text(first_code) + std::string(" (job ") + std::to_string(first_err) + ")");
+    if (first_code) raise(first_code, error_text(first_code) + std::string(" (job ") + jid(first_err) + ")");
     for (size_t i = 0; i < book.size(); ++i) out[book[i]] = result_from_stats(st[i]);
     return;
   }

@@ -690,20 +690,42 @@ def _jobs_array(jobs):
     return arr, keep
 
 
-def price_portfolio(jobs: Sequence[tuple], device: int | None = None) -> list[PriceResult]:
+def price_portfolio(jobs: Sequence[tuple], device: int | None = None, greeks: bool = False):
     """Heterogeneous portfolio: jobs are (contract, model, n_paths, seed,
     steps_per_year[, grid[, antithetic[, control_variate]]]) tuples, priced in
     one persistent launch; each result equals ``contract.price(model, n_paths,
     seed, steps_per_year, antithetic, control_variate, grid=grid)``
-    (variance-reduced jobs run their own launches)."""
+    (variance-reduced jobs run their own launches).
+
+    With ``greeks=True`` returns (PriceResult, ModelGreeks | None) pairs: the
+    jobs whose payoff admits pathwise Greeks (``Contract.pathwise_greeks``)
+    get their derivatives from persistent Greeks launches, equal to
+    ``pathwise_greeks`` on the job alone; the others (barriers, digitals,
+    variance-reduced jobs) are priced without Greeks (None)."""
     jobs = list(jobs)
     if not jobs:
         return []
     arr, keep = _jobs_array(jobs)
     out = (N.KtPriceResult * len(jobs))()
     eng = N.engine(device)
-    N.check(N.lib().kt_price_portfolio(eng.handle, arr, len(jobs), out), eng.handle)
-    return [PriceResult(r) for r in out]
+    if not greeks:
+        N.check(N.lib().kt_price_portfolio(eng.handle, arr, len(jobs), out), eng.handle)
+        return [PriceResult(r) for r in out]
+    stride = 24
+    g = (C.c_double * (stride * len(jobs)))()
+    se = (C.c_double * (stride * len(jobs)))()
+    N.check(N.lib().kt_price_portfolio_greeks(eng.handle, arr, len(jobs), out, g, se, stride), eng.handle)
+    res = []
+    for j, job in enumerate(jobs):
+        pr = PriceResult(out[j])
+        names = _param_names(job[1])
+        row = list(g[j * stride:j * stride + len(names)])
+        if any(math.isnan(x) for x in row):
+            res.append((pr, None))
+            continue
+        srow = list(se[j * stride:j * stride + len(names)])
+        res.append((pr, ModelGreeks(names, [pr.price] + row, [pr.std_error] + srow, "pathwise")))
+    return res
 
 
 class Plan:

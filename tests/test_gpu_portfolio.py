@@ -105,3 +105,83 @@ def test_portfolio_errors_like_a_loop():
         K.price_portfolio([ok, bad_model, unknown])
     assert e.value.kind == "MalformedContract"
     assert K.price_portfolio([]) == []
+
+
+# ---------------------------------------------------------------------------
+# Pathwise Greeks of a portfolio (kt_price_portfolio_greeks)
+# ---------------------------------------------------------------------------
+def per_contract_greeks(jobs):
+    """The per-contract equivalent: pathwise Greeks where the payoff admits
+    them, a price otherwise."""
+    want = []
+    for job in jobs:
+        con, model, n, seed, spy = job[:5]
+        grid = job[5] if len(job) > 5 else None
+        try:
+            g = con.pathwise_greeks(model, n_paths=n, seed=seed, steps_per_year=spy, grid=grid)
+        except K.KontractError as e:
+            assert e.kind == "Unsupported"
+            want.append((con.price(model, n_paths=n, seed=seed, steps_per_year=spy, grid=grid), None))
+            continue
+        want.append((g.price, g))
+    return want
+
+
+def assert_greeks_identical(got, want):
+    for i, ((pr, g), (wp, wg)) in enumerate(zip(got, want)):
+        if wg is None:
+            assert g is None, i
+            assert pr.price == wp.price and pr.std_error == wp.std_error, (i, pr.price, wp.price)
+        else:
+            assert g is not None, i
+            assert pr.price == wg.price and pr.std_error == wg.price_std_error, (i, pr.price, wg.price)
+            assert g.names == wg.names
+            assert g.values == wg.values, (i, g.values, wg.values)
+            assert g.std_errors == wg.std_errors, i
+
+
+def test_portfolio_greeks_equal_per_contract():
+    g = K.GBM(100.0, 0.25, 0.03, asset="X")
+    f3 = [K.GbmFactor(n, 100.0, 0.03, s) for n, s in zip("ABC", (0.2, 0.3, 0.25))]
+    b3 = K.correlated_gbm(f3, [[1.0, 0.2, 0.1], [0.2, 1.0, 0.3], [0.1, 0.3, 1.0]], 0.03)
+    big = K.european_call("X", 90.0, 1.0)
+    for k in range(95, 130, 5):  # a Lipschitz program beyond 16 registers: per-contract Greeks
+        big = big + K.european_call("X", float(k), 1.0)
+    jobs = small_book(32) + [
+        (K.scale(K.spot("A") - K.spot("C"), K.one("USD")) @ K.at(0.5), b3, 5000, 3, 12),   # 3 assets: per contract
+        (big, g, 3000, 9, 50),
+        (K.european_call("X", 100.0, 1.0), g, 0, 1, 50),                                      # no paths
+        (K.scale((K.running_max(K.spot("X")) - 95.0).clip(0.0), K.one("USD")) @ K.at(1.0), g, 4000, 5, 50),
+    ]
+    got = K.price_portfolio(jobs, greeks=True)
+    want = per_contract_greeks(jobs)
+    assert_greeks_identical(got, want)
+    kinds = {B.KINDS[i % 4]: got[i][1] is not None for i in range(32)}
+    assert kinds == {"european": True, "asian": True, "barrier": False, "basket": True}
+    assert got[-2][0].n_paths == 0 and got[-2][1].values == [0.0, 0.0, 0.0]
+    # a second call (same shapes) gives the same numbers
+    assert_greeks_identical(K.price_portfolio(jobs, greeks=True), want)
+
+
+def test_portfolio_greeks_errors_and_tiles():
+    g = K.GBM(100, 0.2, 0.05, asset="X")
+    ok = (K.european_call("X", 100.0, 1.0), g, 1000, 1, 50)
+    with pytest.raises(N.KontractError) as e:
+        K.price_portfolio([ok, (K.european_call("Y", 100.0, 1.0), g, 1000, 1, 50)], greeks=True)
+    assert e.value.kind == "UnknownAsset"
+    horizon = K.one("USD") @ K.at(1.0)
+    bad_or = (horizon + K.when(K.spot("X") >= 101.0, K.one("USD").or_(K.one("USD"))), g, 1000, 1, 12)
+    with pytest.raises(N.KontractError) as e:
+        K.price_portfolio([ok, ok, bad_or], greeks=True)
+    assert e.value.kind == "Unsupported" and "job 2" in str(e.value)
+    eng = N.engine(0)
+    jobs = small_book(12)
+    want = per_contract_greeks(jobs)
+    try:
+        for bpc, chunk in ((1, 1), (2, 8)):
+            eng.set_option("portfolio_blocks_per_cu", bpc)
+            eng.set_option("portfolio_chunk", chunk)
+            assert_greeks_identical(K.price_portfolio(jobs, greeks=True), want)
+    finally:
+        eng.set_option("portfolio_blocks_per_cu", 0)
+        eng.set_option("portfolio_chunk", 1)
