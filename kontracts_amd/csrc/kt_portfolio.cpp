@@ -9,7 +9,8 @@
 // longest-first, each job's block count, then each job's program and model
 // tables.  Each job owns a contiguous range of block partials and one error
 // word.  The launch sequence (queue reset, persistent grid, per-job finalize)
-// is captured once in a hipGraph and replayed while the book's shape holds.
+// runs as three launches, or, with the engine's portfolio_graph option, is
+// captured once in a hipGraph and replayed while the book's shape holds.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
