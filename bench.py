@@ -224,6 +224,27 @@ def main():
         extras["config3_pathwise_se"] = dict(zip(gA.names, gA.std_errors))
         extras["config3_bump"] = dict(zip(gB.names, gB.values))
         extras["config3_bump_se"] = dict(zip(gB.names, gB.std_errors))
+        # The same six Greeks with the variance-model derivatives carried
+        # forward as tangents (no checkpoints, no reverse sweep).
+        eng.set_option("pathwise_mode", 1)
+        try:
+            c1.pathwise_greeks(hm, n_paths=n3, steps_per_year=365)
+            eng.set_profiling(True)
+            eng.kernel_time(reset=True)
+            t = time.perf_counter()
+            gT = c1.pathwise_greeks(hm, n_paths=n3, steps_per_year=365)
+            tT = time.perf_counter() - t
+            kT_ms, _, _ = eng.kernel_time(reset=True)
+            eng.set_profiling(False)
+        finally:
+            eng.set_option("pathwise_mode", 0)
+        extras["config3_tangent_ms"] = tT * 1e3
+        extras["config3_tangent_kernel_ms"] = kT_ms
+        extras["config3_tangent_speedup_raw"] = tB / tT
+        extras["config3_tangent_speedup_equal_stderr"] = {
+            nm: (tB / tT) * (sb / sa) ** 2 for nm, sa, sb in zip(gT.names, gT.std_errors, gB.std_errors)}
+        extras["config3_tangent_max_rel_diff"] = max(
+            abs(a - b) / max(abs(b), 1e-12) for a, b in zip(gT.values, gA.values))
         # Config 4: the 1,000-contract book in one persistent launch vs one
         # launch per contract (the reference's loop of price_gbm calls).
         from kontracts_amd import book as B

@@ -250,6 +250,10 @@ kt_status kt_engine_kernel_time(kt_engine* engine, double* total_ms, uint64_t* n
  *                      persistent portfolio grid (0 = the kernel's occupancy
  *                      limit, the default).
  *   "portfolio_chunk": work items taken per queue fetch (default 1).
+ *   "pathwise_mode":   Heston pathwise Greeks by the reverse (adjoint) sweep
+ *                      with checkpoints, 0 (default), or by forward tangents
+ *                      of the five variance parameters in one pass, 1 (same
+ *                      estimator to rounding; no checkpoint memory).
  *   "portfolio_graph": 1 replays the portfolio's launch sequence from a
  *                      captured hipGraph while the book keeps its shape; 0
  *                      (default) launches the three kernels directly, which

@@ -238,8 +238,9 @@ __device__ __forceinline__ void adjoint_gbm_block(const AA& A, uint32_t blk) {
                   jma = jm[c];
                   s0a = P.s0[c];
                 }
-              const float S = s0a * exp2_fast(xa);
-              const float d0 = S * kLn2f * ja, d1 = S * kLn2f * jma, d2 = S / s0a;
+              // S / s0 = 2^x: no division on the per-point path
+              const float g = exp2_fast(xa), S = s0a * g;
+              const float d0 = S * kLn2f * ja, d1 = S * kLn2f * jma, d2 = g;
               if (A.acc_kind[q] == OP_AVG) {
                 W[q][0] += d0;
                 W[q][1] += d1;
@@ -318,6 +319,150 @@ __device__ __forceinline__ float heston_v_step(const PA& P, int k, float z1, flo
   return P.heston_full_truncation ? vn : fmaxf(vn, 0.0f);
 }
 
+// Heston in forward (tangent) mode: one pass carrying d x / d theta and
+// d v / d theta for theta in {v0, kappa, theta, sigma_v, rho} through the same
+// Euler step, sqrt and floor as the reverse sweep (its transpose), plus each
+// accumulator's derivatives at its fed points.  With five variance
+// parameters one forward pass of tangents costs less than the reverse
+// sweep's second generation of every normal; the estimator is the same
+// pathwise derivative, to rounding.  Outputs as the reverse kernel.
+template <int NS, bool FEEDS, class AA>
+__device__ __forceinline__ void tangent_heston_block(const AA& A, uint32_t blk) {
+  constexpr int NOUT = 7, NT = 5;  // tangents: v0, kappa, theta, sigma_v, rho
+  const auto& P = A.p;
+  const uint64_t local = (uint64_t)blk * kBlock + threadIdx.x;
+  const bool live = local < P.n_paths;
+  const uint64_t path = P.path_base + local;
+  const uint32_t p_lo = (uint32_t)path, p_hi = (uint32_t)(path >> 32);
+  const int n = P.n_grid - 1;
+  float r[1][NS];
+  init_registers<NS>(P, r);
+  float x[1][1] = {{0.0f}};
+  float v = P.v0;
+  int argstep[kAdjMaxAcc];
+#pragma unroll
+  for (int q = 0; q < kAdjMaxAcc; ++q) argstep[q] = 0;
+  double pv[1][1] = {{0.0}};
+  const bool ft = P.heston_full_truncation != 0;
+  const float theta = A.theta, rho = P.rho, rho_perp = P.rho_perp;
+  const float rho_ratio = rho_perp > 0.0f ? rho / rho_perp : 0.0f;
+  const float s0 = P.s0[0];
+  float dx[NT], dv[NT];
+#pragma unroll
+  for (int j = 0; j < NT; ++j) dx[j] = dv[j] = 0.0f;
+  dv[0] = P.v0 > 0.0f ? 1.0f : 0.0f;
+  float W[FEEDS ? kAdjMaxAcc : 1][NT + 1];  // per accumulator: d/ds0, then the tangents
+#pragma unroll
+  for (int q = 0; q < (FEEDS ? kAdjMaxAcc : 1); ++q)
+#pragma unroll
+    for (int j = 0; j <= NT; ++j) W[q][j] = 0.0f;
+  for (int g = 0; g < P.n_groups; ++g) {
+    float z[8];
+#pragma unroll
+    for (int bk = 0; bk < 2; ++bk) {
+      const uint4 q = philox4x32_10((uint32_t)(2 * g + bk), p_lo, p_hi, 0u, P.seed_lo, P.seed_hi);
+      z[4 * bk + 0] = normal_from_word(q.x);
+      z[4 * bk + 1] = normal_from_word(q.y);
+      z[4 * bk + 2] = normal_from_word(q.z);
+      z[4 * bk + 3] = normal_from_word(q.w);
+    }
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      const int k = 4 * g + s;
+      if (k <= n) {
+        const float z1 = z[2 * s], z2 = z[2 * s + 1];
+        if (k >= 1) {
+          const float t1 = ld_f32(A.rev, 8 * (size_t)k + 0);
+          const float t2 = ld_f32(A.rev, 8 * (size_t)k + 1);
+          const float t3 = ld_f32(A.rev, 8 * (size_t)k + 2);
+          const float t5 = ld_f32(A.rev, 8 * (size_t)k + 3);
+          const float dt = ld_f32(A.rev, 8 * (size_t)k + 4);
+          const float sq = ld_f32(A.rev, 8 * (size_t)k + 5);
+          const float kdt = ld_f32(A.rev, 8 * (size_t)k + 6);
+          const float vp = ft ? fmaxf(v, 0.0f) : v;
+          const float sv = __builtin_amdgcn_sqrtf(vp);
+          const float w = fmaf(rho, z1, rho_perp * z2);
+          const bool pass = !ft || v > 0.0f;  // d vp / d v
+          const float hs = sv > 0.0f ? 0.5f * __builtin_amdgcn_rsqf(vp) : 0.0f;  // d sqrt(vp) / d vp
+          const float cx = t2 * z1, cv = t5 * w;
+          float dvn[NT];
+#pragma unroll
+          for (int j = 0; j < NT; ++j) {
+            const float dvp = pass ? dv[j] : 0.0f;
+            const float dsv = hs * dvp;
+            dx[j] = fmaf(cx, dsv, fmaf(t1, dvp, dx[j]));
+            dvn[j] = fmaf(cv, dsv, fmaf(t3, dvp, dv[j]));
+          }
+          dvn[1] = fmaf(dt, theta - vp, dvn[1]);                  // kappa
+          dvn[2] += kdt;                                          // theta
+          dvn[3] = fmaf(sq * sv, w, dvn[3]);                      // sigma_v
+          dvn[4] = fmaf(t5 * sv, fmaf(-rho_ratio, z2, z1), dvn[4]);  // rho
+          advance<MK_HESTON, 1, 1>(P, k, &z[2 * s], x, v);
+          const bool keep = ft || v > 0.0f;  // the floor cuts the derivative
+#pragma unroll
+          for (int j = 0; j < NT; ++j) dv[j] = keep ? dvn[j] : 0.0f;
+        }
+        if constexpr (FEEDS) {
+          const uint32_t fm = ld_u32(A.feed, (size_t)k);
+          int before[kAdjMaxAcc];
+#pragma unroll
+          for (int q = 0; q < kAdjMaxAcc; ++q) before[q] = argstep[q];
+          run_point_ops<NS, 1>(A, k, r, pv, x, live, fm, argstep);
+          if (fm) {
+            const float e = exp2_fast(x[0][0]), S = s0 * e, sl = S * kLn2f;
+#pragma unroll
+            for (int q = 0; q < kAdjMaxAcc; ++q) {
+              if ((fm >> q) & 1u) {
+                if (A.acc_kind[q] == OP_AVG) {
+                  W[q][0] += e;
+#pragma unroll
+                  for (int j = 0; j < NT; ++j) W[q][1 + j] = fmaf(sl, dx[j], W[q][1 + j]);
+                } else if (argstep[q] != before[q] || k == 0) {
+                  W[q][0] = e;
+#pragma unroll
+                  for (int j = 0; j < NT; ++j) W[q][1 + j] = sl * dx[j];
+                }
+              }
+            }
+          }
+        } else {
+          run_point_ops<NS, 1>(A, k, r, pv, x, live, 0u, argstep);
+        }
+      }
+    }
+  }
+  const float S_n = s0 * exp2_fast(x[0][0]);
+  float g[NT + 1];
+#pragma unroll
+  for (int j = 0; j <= NT; ++j) g[j] = 0.0f;
+  const uint64_t st = ld_u64(P.steps, (size_t)n);
+  const int begin = (int)(uint32_t)st, count = (int)(uint32_t)(st >> 32);
+  if (count > 0) {
+    const float S[1] = {S_n};
+    const float ds_n = dual_point<NS, 1>(A, n, begin, count, S, r, 0);
+    g[0] = ds_n * S_n / s0;
+#pragma unroll
+    for (int j = 0; j < NT; ++j) g[1 + j] = ds_n * S_n * kLn2f * dx[j];
+    if constexpr (FEEDS) {
+#pragma unroll
+      for (int q = 0; q < kAdjMaxAcc; ++q) {
+        if (q < A.n_acc) {
+          const float ab = dual_point<NS, 1>(A, n, begin, count, S, r, 1 + q);
+#pragma unroll
+          for (int j = 0; j <= NT; ++j) g[j] = fmaf(ab, W[q][j], g[j]);
+        }
+      }
+    }
+  }
+  double vals[NOUT] = {pv[0][0], (double)g[0], (double)g[1], (double)g[2], (double)g[3], (double)g[4], (double)g[5]};
+  block_stats_out<NOUT>(vals, live, P.partials, blk);
+}
+
+template <int NS, bool FEEDS>
+__global__ __launch_bounds__(kBlock) void tangent_heston_kernel(const AdjArgs A) {
+  tangent_heston_block<NS, FEEDS>(A, blockIdx.x);
+}
+
 // Heston: forward with a checkpoint every SEG points, then the adjoint sweep
 // with per-segment recompute.  Outputs: price, d/d{s0, v0, kappa, theta,
 // sigma_v, rho}.  FEEDS: the payoff has running averages / extremes, so the
@@ -389,17 +534,25 @@ __device__ __forceinline__ void adjoint_heston_block(const AA& A, uint32_t blk, 
     }
   }
   const bool ft = P.heston_full_truncation != 0;
-  const float kappa = A.kappa, theta = A.theta, rho = P.rho, rho_perp = P.rho_perp;
+  const float theta = A.theta, rho = P.rho, rho_perp = P.rho_perp;
   const float rho_ratio = rho_perp > 0.0f ? rho / rho_perp : 0.0f;
   // without feeds the only spot seed is at n: d/dx is constant along the path
   float xb = FEEDS ? 0.0f : ds_n * S_n * kLn2f;
   float g_s0 = FEEDS ? 0.0f : ds_n * S_n / s0;
   float vbar = 0.0f, g_k = 0.0f, g_th = 0.0f, g_sv = 0.0f, g_rho = 0.0f;
   const int n_seg = n / SEG + 1;
+  // checkpoint of the segment to recompute next, loaded one segment ahead so
+  // that its latency hides behind the current segment's work
+  float x_ck = FEEDS && live ? ck[(2 * (size_t)(n_seg - 1)) * cs + lane] : 0.0f;
+  float v_ck = live ? ck[(2 * (size_t)(n_seg - 1) + 1) * cs + lane] : 0.0f;
   for (int c = n_seg - 1; c >= 0; --c) {
     // recompute segment c (points SEG*c .. SEG*c + SEG - 1) from its checkpoint
-    float xs[1][1] = {{FEEDS && live ? ck[(2 * (size_t)c) * cs + lane] : 0.0f}};
-    float vs = live ? ck[(2 * (size_t)c + 1) * cs + lane] : 0.0f;
+    float xs[1][1] = {{x_ck}};
+    float vs = v_ck;
+    if (c > 0) {
+      if constexpr (FEEDS) x_ck = live ? ck[(2 * (size_t)(c - 1)) * cs + lane] : 0.0f;
+      v_ck = live ? ck[(2 * (size_t)(c - 1) + 1) * cs + lane] : 0.0f;
+    }
     float vb[SEG], z1s[SEG], z2s[SEG];
 #pragma unroll
     for (int gq = 0; gq < SEG / 4; ++gq) {
@@ -439,16 +592,19 @@ __device__ __forceinline__ void adjoint_heston_block(const AA& A, uint32_t blk, 
             for (int q = 0; q < kAdjMaxAcc; ++q)
               if ((fm >> q) & 1u) seed += (A.acc_kind[q] == OP_AVG || argstep[q] == p) ? abar[q] : 0.0f;
           }
-          const float S = s0 * exp2_fast(xs[0][0]);
+          const float g = exp2_fast(xs[0][0]), S = s0 * g;  // S / s0 = 2^x
           xb = fmaf(seed * S, kLn2f, xb);
-          g_s0 = fmaf(seed, S / s0, g_s0);
+          g_s0 = fmaf(seed, g, g_s0);
         }
         if (p >= 1) {
-          const float t1 = ld_f32(P.tab, (size_t)kHestonStride * p + 1);
-          const float t2 = ld_f32(P.tab, (size_t)kHestonStride * p + 2);
-          const float t3 = ld_f32(P.tab, (size_t)kHestonStride * p + 3);
-          const float t5 = ld_f32(P.tab, (size_t)kHestonStride * p + 5);
-          const float dt = ld_f32(A.adj_tab, 2 * (size_t)p), sq = ld_f32(A.adj_tab, 2 * (size_t)p + 1);
+          // one 32-byte row per point: a single scalar load
+          const float t1 = ld_f32(A.rev, 8 * (size_t)p + 0);
+          const float t2 = ld_f32(A.rev, 8 * (size_t)p + 1);
+          const float t3 = ld_f32(A.rev, 8 * (size_t)p + 2);
+          const float t5 = ld_f32(A.rev, 8 * (size_t)p + 3);
+          const float dt = ld_f32(A.rev, 8 * (size_t)p + 4);
+          const float sq = ld_f32(A.rev, 8 * (size_t)p + 5);
+          const float kdt = ld_f32(A.rev, 8 * (size_t)p + 6);
           const float vprev = vb[i], z1 = z1s[i], z2 = z2s[i];
           const float vp = ft ? fmaxf(vprev, 0.0f) : vprev;
           const float sv = __builtin_amdgcn_sqrtf(vp);
@@ -458,10 +614,12 @@ __device__ __forceinline__ void adjoint_heston_block(const AA& A, uint32_t blk, 
           float vpb = fmaf(xb, t1, vnb * t3);
           const float svb = fmaf(xb * t2, z1, vnb * t5 * w);
           g_k = fmaf(vnb, dt * (theta - vp), g_k);
-          g_th = fmaf(vnb, kappa * dt, g_th);
+          g_th = fmaf(vnb, kdt, g_th);
           g_sv = fmaf(vnb, sq * sv * w, g_sv);
           g_rho = fmaf(vnb * t5 * sv, fmaf(-rho_ratio, z2, z1), g_rho);
-          vpb = sv > 0.0f ? fmaf(svb, 0.5f / sv, vpb) : vpb;
+          // d sqrt(v)/dv = 0.5 / sqrt(v) through the hardware reciprocal square
+          // root (one instruction instead of an IEEE division)
+          vpb = sv > 0.0f ? fmaf(svb, 0.5f * __builtin_amdgcn_rsqf(vp), vpb) : vpb;
           vbar = vnb + ((!ft || vprev > 0.0f) ? vpb : 0.0f);
           if constexpr (FEEDS) {
             const float t0 = ld_f32(P.tab, (size_t)kHestonStride * p + 0);
@@ -521,6 +679,8 @@ __device__ __forceinline__ void portfolio_adjoint_body(const AdjPortfolioArgs& Q
       if constexpr (VARIANT == APV_HESTON_FEEDS)
         adjoint_heston_block<16, kAdjSeg, true>(A, blk, ck, kBlock, threadIdx.x);
       if constexpr (VARIANT == APV_BASKET5) adjoint_gbm_block<5, 16>(A, blk);
+      if constexpr (VARIANT == APV_HESTON_T) tangent_heston_block<16, false>(A, blk);
+      if constexpr (VARIANT == APV_HESTON_FEEDS_T) tangent_heston_block<16, true>(A, blk);
     }
   }
 }
@@ -530,8 +690,9 @@ __global__ __launch_bounds__(kBlock) void portfolio_adjoint_kernel(const AdjPort
   portfolio_adjoint_body<VARIANT>(Q);
 }
 
-// The basket variant needs a few registers more than 128, which would cost a
-// quarter of its occupancy (3 waves per SIMD instead of 4): cap it at 128.
+// The basket and Heston variants need a few registers more than 128, which
+// would cost a quarter of their occupancy (3 waves per SIMD instead of 4):
+// cap them at 128.
 template <>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void
 portfolio_adjoint_kernel<APV_BASKET5>(const AdjPortfolioArgs Q) {
@@ -542,6 +703,18 @@ template <>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void
 portfolio_adjoint_kernel<APV_HESTON_FEEDS>(const AdjPortfolioArgs Q) {
   portfolio_adjoint_body<APV_HESTON_FEEDS>(Q);
+}
+
+template <>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void
+portfolio_adjoint_kernel<APV_HESTON>(const AdjPortfolioArgs Q) {
+  portfolio_adjoint_body<APV_HESTON>(Q);
+}
+
+template <>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void
+portfolio_adjoint_kernel<APV_HESTON_FEEDS_T>(const AdjPortfolioArgs Q) {
+  portfolio_adjoint_body<APV_HESTON_FEEDS_T>(Q);
 }
 
 // Block b: output b % kAdjMaxOut of job b / kAdjMaxOut (outputs beyond the
@@ -564,9 +737,10 @@ bool adjoint_variant_exists(int model, int k_assets, int n_slots, int* ns_chosen
   return true;
 }
 
-int portfolio_adjoint_variant(int model, int k_assets, int n_slots, int n_acc) {
+int portfolio_adjoint_variant(int model, int k_assets, int n_slots, int n_acc, bool tangent) {
   if (n_slots > 16) return -1;
   if (model == MK_GBM) return APV_GBM;
+  if (model == MK_HESTON && tangent) return n_acc > 0 ? APV_HESTON_FEEDS_T : APV_HESTON_T;
   if (model == MK_HESTON) return n_acc > 0 ? APV_HESTON_FEEDS : APV_HESTON;
   if (model == MK_BASKET && k_assets == 5) return APV_BASKET5;
   return -1;
@@ -586,6 +760,12 @@ int portfolio_adjoint_occupancy(int variant) {
     case APV_BASKET5:
       e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, portfolio_adjoint_kernel<APV_BASKET5>, kBlock, 0);
       break;
+    case APV_HESTON_T:
+      e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, portfolio_adjoint_kernel<APV_HESTON_T>, kBlock, 0);
+      break;
+    case APV_HESTON_FEEDS_T:
+      e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, portfolio_adjoint_kernel<APV_HESTON_FEEDS_T>, kBlock, 0);
+      break;
   }
   if (e != hipSuccess) return 1;
   return n > 0 ? n : 1;
@@ -603,6 +783,12 @@ hipError_t launch_portfolio_adjoint(int variant, const AdjPortfolioArgs& q, int 
       break;
     case APV_BASKET5:
       hipLaunchKernelGGL(portfolio_adjoint_kernel<APV_BASKET5>, dim3(grid), dim3(kBlock), 0, s, q);
+      break;
+    case APV_HESTON_T:
+      hipLaunchKernelGGL(portfolio_adjoint_kernel<APV_HESTON_T>, dim3(grid), dim3(kBlock), 0, s, q);
+      break;
+    case APV_HESTON_FEEDS_T:
+      hipLaunchKernelGGL(portfolio_adjoint_kernel<APV_HESTON_FEEDS_T>, dim3(grid), dim3(kBlock), 0, s, q);
       break;
     default: return hipErrorInvalidValue;
   }
@@ -622,6 +808,15 @@ hipError_t launch_adjoint(int model, int k, int ns, const AdjArgs& a, int n_bloc
   if (model == MK_GBM) {
     if (ns == 16) KT_ADJ((adjoint_gbm_kernel<1, 16>));
     else KT_ADJ((adjoint_gbm_kernel<1, 32>));
+  } else if (model == MK_HESTON && a.tangent) {
+    const bool feeds = a.n_acc > 0;
+    if (ns == 16) {
+      if (feeds) KT_ADJ((tangent_heston_kernel<16, true>));
+      else KT_ADJ((tangent_heston_kernel<16, false>));
+    } else {
+      if (feeds) KT_ADJ((tangent_heston_kernel<32, true>));
+      else KT_ADJ((tangent_heston_kernel<32, false>));
+    }
   } else if (model == MK_HESTON) {
     const bool feeds = a.n_acc > 0;
     if (a.seg != 16 && a.seg != 32) return hipErrorInvalidValue;

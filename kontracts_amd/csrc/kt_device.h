@@ -38,14 +38,15 @@ __device__ __forceinline__ uint4 philox4x32_10(uint32_t c0, uint32_t c1, uint32_
 // of 2^24 equal cells of (0,1): u = (2*floor(w/256)+1)/2^25, and z = Phi^-1(u)
 // = sqrt(2) erfinv(2u-1), with erfinv from M. Giles, "Approximating the erfinv
 // function" (GPU Computing Gems Jade, 2011), single-precision version; the
-// sqrt(2) is folded into the polynomial coefficients.  x = 2u-1 is an integer
-// times 2^-24 and is formed exactly; 1-x^2 is one fused multiply-add (one
-// rounding, exact cancellation near |x| = 1).  The tail branch (|z| > ~2.9,
-// 0.34% of draws) is evaluated only when some lane of the wave needs it.
+// sqrt(2) is folded into the polynomial coefficients.  x = 2u-1 = (2m+1)
+// 2^-24 - 1 with m = w >> 8 < 2^24 is formed exactly in one FMA: m converts
+// exactly, m 2^-23 is exact and the sum is a multiple of 2^-24 below 1 in
+// magnitude.  1-x^2 is one fused multiply-add (one rounding, exact
+// cancellation near |x| = 1).  The tail branch (|z| > ~2.9, 0.34% of draws)
+// is evaluated only when some lane of the wave needs it.
 #define KT_SQ2(c) ((c) * 1.41421356237309505f)
 __device__ __forceinline__ float normal_from_word(uint32_t w) {
-  const int i = (int)((w >> 7) | 1u) - (1 << 24);
-  const float x = (float)i * 5.9604644775390625e-08f;  // 2^-24
+  const float x = fmaf((float)(w >> 8), 1.1920928955078125e-07f, -0.999999940395355224609375f);  // 2^-23, -(1-2^-24)
   const float t = fmaf(-x, x, 1.0f);
   const float lg = __builtin_amdgcn_logf(t);  // log2(1 - x^2) <= 0
   const float wc = fmaf(lg, -0.693147180559945f, -2.5f);  // -ln(1-x^2) - 2.5

@@ -740,6 +740,9 @@ kt_status kt_engine_set_option(kt_engine* E, const char* name, int64_t value) {
   } else if (key == "portfolio_chunk") {
     if (value < 1 || value > 1024) raise(KT_ERR_INVALID_ARGUMENT, "portfolio_chunk must be in [1, 1024]");
     E->pf_chunk = (int)value;
+  } else if (key == "pathwise_mode") {
+    if (value != 0 && value != 1) raise(KT_ERR_INVALID_ARGUMENT, "pathwise_mode must be 0 (reverse) or 1 (tangent)");
+    E->pathwise_tangent = value == 1;
   } else if (key == "portfolio_graph") {
     if (value != 0 && value != 1) raise(KT_ERR_INVALID_ARGUMENT, "portfolio_graph must be 0 or 1");
     E->pf_use_graph = value == 1;

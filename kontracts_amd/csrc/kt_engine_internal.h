@@ -52,7 +52,8 @@ struct kt_engine {
   bool errp_zero = false;
   int occupancy_portfolio = 0;  // cached occupancies (blocks per CU)
   int occupancy_surface = 0;
-  int occupancy_adjoint[kt::APV_COUNT] = {0, 0, 0, 0};
+  int occupancy_adjoint[kt::APV_COUNT] = {0, 0, 0, 0, 0, 0};
+  bool pathwise_tangent = false;  // Heston pathwise Greeks: forward tangents instead of the reverse sweep
   bool profiling = false;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   double prof_ms = 0.0, prof_path_steps = 0.0;
@@ -162,7 +163,7 @@ void price_portfolio(kt_engine* E, const kt_job* jobs, uint64_t n_jobs, kt_price
 // portfolio share them): blobs in the order fill_adjoint_args expects.
 struct AdjJobHost {
   std::vector<KOp> ops;
-  std::vector<float> tab, adj, thr;
+  std::vector<float> tab, adj, thr, rev;
   std::vector<double> df;
   int stride = 0;
   AdjPlan plan;

@@ -356,6 +356,21 @@ AdjJobHost adjoint_job_host(const Program& prog, const ModelDesc& md, const std:
   if (H.ops.empty()) H.ops.push_back(KOp{OP_NOP, 0, 0, 0, 0.0f});
   H.thr.assign(1, 0.0f);
   H.plan = std::move(plan);
+  H.rev.assign(8, 0.0f);
+  if (md.kind == KT_MODEL_HESTON) {
+    H.rev.assign(8 * grid.size(), 0.0f);
+    for (size_t k = 0; k < grid.size(); ++k) {
+      const float* t = &H.tab[k * kHestonStride];
+      float* r = &H.rev[8 * k];
+      r[0] = t[1];
+      r[1] = t[2];
+      r[2] = t[3];
+      r[3] = t[5];
+      r[4] = H.adj[2 * k];
+      r[5] = H.adj[2 * k + 1];
+      r[6] = (float)md.kappa * H.adj[2 * k];
+    }
+  }
   return H;
 }
 
@@ -368,7 +383,8 @@ std::vector<std::pair<const void*, size_t>> adjoint_blobs(const AdjJobHost& H, c
           {H.tab.data(), H.tab.size() * sizeof(float)},
           {H.thr.data(), H.thr.size() * sizeof(float)},
           {H.adj.data(), H.adj.size() * sizeof(float)},
-          {H.plan.feed.data(), H.plan.feed.size() * sizeof(uint32_t)}};
+          {H.plan.feed.data(), H.plan.feed.size() * sizeof(uint32_t)},
+          {H.rev.data(), H.rev.size() * sizeof(float)}};
 }
 
 void fill_adjoint_args(AdjArgs& A, const AdjJobHost& H, const Program& prog, const ModelDesc& md,
@@ -388,6 +404,7 @@ void fill_adjoint_args(AdjArgs& A, const AdjJobHost& H, const Program& prog, con
   fill_model_args(P, md, seed);
   A.adj_tab = (const float*)dev[7];
   A.feed = (const uint32_t*)dev[8];
+  A.rev = (const float*)dev[9];
   A.n_acc = H.plan.n_acc;
   for (int q = 0; q < 4; ++q) {
     A.acc_slot[q] = H.plan.slot[q];
@@ -443,7 +460,8 @@ void greeks_adjoint(kt_engine* E, const char* json, const kt_model* model, const
   float* ck = nullptr;
   const int seg = E->adj_seg;
   const size_t n_ck = (grid.size() - 1) / seg + 1;
-  if (kind == MK_HESTON) ck = (float*)E->ensure(E->aux, n_ck * 2 * N * sizeof(float));
+  const bool tangent = kind == MK_HESTON && E->pathwise_tangent;
+  if (kind == MK_HESTON && !tangent) ck = (float*)E->ensure(E->aux, n_ck * 2 * N * sizeof(float));
 
   AdjArgs A{};
   fill_adjoint_args(A, H, prog, md, grid, cfg->seed, dev);
@@ -451,6 +469,7 @@ void greeks_adjoint(kt_engine* E, const char* json, const kt_model* model, const
   A.ck = ck;
   A.ck_stride = N;
   A.seg = seg;
+  A.tangent = tangent ? 1 : 0;
 
   E->prof_begin();
   const uint64_t kMaxBlocksPerLaunch = 1ull << 20;

@@ -78,6 +78,7 @@ struct AdjArgs {
   PathArgs p;               // program, model tables, RNG (n_grid points, last = flow date)
   const float* adj_tab;     // [n_grid][2] {dt, sqrt(dt)} (zero at point 0)
   const uint32_t* feed;     // [n_grid] bit q: accumulator q is fed at this point
+  const float* rev;         // Heston: [n_grid][8] {t1, t2, t3, t5, dt, sqrt(dt), kappa dt, 0} for the reverse sweep
   float* ck;                // Heston: [n_ck][2][ck_stride] {log2(S/s0), v} before point seg*c
   uint64_t ck_stride;       // >= total paths (checkpoints are indexed by global path)
   int n_acc;
@@ -87,6 +88,7 @@ struct AdjArgs {
   float sigma[kMaxAssets], mu[kMaxAssets];
   float kappa, theta, sigma_v;
   int seg;  // Heston checkpoint interval: 16 or 32 points
+  int tangent;  // Heston: 1 runs the forward (tangent) mode instead of the reverse sweep
 };
 int adjoint_outputs(int model, int k_assets);
 bool adjoint_variant_exists(int model, int k_assets, int n_slots, int* ns_chosen);
@@ -95,7 +97,15 @@ hipError_t launch_adjoint(int model, int k_assets, int ns, const AdjArgs& a, int
 // Pathwise Greeks of a portfolio: one persistent launch per variant over
 // (job, block) items of that variant's jobs (16 registers, checkpoint
 // interval kAdjSeg).
-enum AdjPortfolioVariant { APV_GBM = 0, APV_HESTON = 1, APV_HESTON_FEEDS = 2, APV_BASKET5 = 3, APV_COUNT = 4 };
+enum AdjPortfolioVariant {
+  APV_GBM = 0,
+  APV_HESTON = 1,
+  APV_HESTON_FEEDS = 2,
+  APV_BASKET5 = 3,
+  APV_HESTON_T = 4,        // Heston, forward (tangent) mode
+  APV_HESTON_FEEDS_T = 5,  // Heston with running accumulators, tangent mode
+  APV_COUNT = 6
+};
 struct AdjPortfolioArgs {
   const AdjArgs* jobs;    // every adjoint job of the portfolio
   const uint32_t* items;  // [n_items][2] {job, block within the job}
@@ -106,7 +116,7 @@ struct AdjPortfolioArgs {
   uint64_t ck_floats;     // >= n_ck * 2 * kBlock of every Heston job of the launch
 };
 // -1 when no portfolio variant covers the job (it runs per contract).
-int portfolio_adjoint_variant(int model, int k_assets, int n_slots, int n_acc);
+int portfolio_adjoint_variant(int model, int k_assets, int n_slots, int n_acc, bool tangent);
 int portfolio_adjoint_occupancy(int variant);
 hipError_t launch_portfolio_adjoint(int variant, const AdjPortfolioArgs& q, int grid, hipStream_t s);
 // out[job][kAdjMaxOut]: the first n_out[job] outputs of each job.

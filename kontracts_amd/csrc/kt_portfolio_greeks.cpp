@@ -32,7 +32,7 @@ struct AdjJob {
   int need = 0;
   uint64_t n_blocks = 0;
   double cost = 0.0;
-  size_t off[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  size_t off[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
   uint64_t part_off = 0;
 };
 
@@ -98,7 +98,7 @@ void price_portfolio_greeks(kt_engine* E, const kt_job* jobs, uint64_t n_jobs, k
       }
       continue;
     }
-    const int variant = portfolio_adjoint_variant(kind, K, prog.n_slots, plan.n_acc);
+    const int variant = portfolio_adjoint_variant(kind, K, prog.n_slots, plan.n_acc, E->pathwise_tangent);
     if (variant < 0) {
       solo.push_back(j);
       continue;
@@ -138,7 +138,7 @@ void price_portfolio_greeks(kt_engine* E, const kt_job* jobs, uint64_t n_jobs, k
       a.part_off = n_partials;
       n_partials += a.n_blocks * (uint64_t)a.need;
     }
-    // Arena: [AdjArgs per job][items of each variant][n_blocks][n_out][per job: 9 table blobs]
+    // Arena: [AdjArgs per job][items of each variant][n_blocks][n_out][per job: 10 table blobs]
     size_t cursor = a256(n_adj * sizeof(AdjArgs));
     size_t items_off[APV_COUNT];
     for (int v = 0; v < APV_COUNT; ++v) {
@@ -152,7 +152,7 @@ void price_portfolio_greeks(kt_engine* E, const kt_job* jobs, uint64_t n_jobs, k
     std::vector<std::vector<std::pair<const void*, size_t>>> blobs(n_adj);
     for (size_t i = 0; i < n_adj; ++i) {
       blobs[i] = adjoint_blobs(adj[i].H, adj[i].prog);
-      for (int t = 0; t < 9; ++t) {
+      for (int t = 0; t < 10; ++t) {
         adj[i].off[t] = cursor;
         cursor += a256(std::max<size_t>(blobs[i][t].second, 1));
       }
@@ -164,7 +164,7 @@ void price_portfolio_greeks(kt_engine* E, const kt_job* jobs, uint64_t n_jobs, k
     // Heston checkpoint slots: one per resident workgroup of the larger launch.
     int n_cu = 0;
     check_hip(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, E->device), "device attribute");
-    int grid_v[APV_COUNT] = {0, 0, 0, 0};
+    int grid_v[APV_COUNT] = {0, 0, 0, 0, 0, 0};
     for (int v = 0; v < APV_COUNT; ++v) {
       if (vitems[v].empty()) continue;
       if (E->occupancy_adjoint[v] <= 0) E->occupancy_adjoint[v] = portfolio_adjoint_occupancy(v);
@@ -187,8 +187,8 @@ void price_portfolio_greeks(kt_engine* E, const kt_job* jobs, uint64_t n_jobs, k
     for (size_t i = 0; i < n_adj; ++i) {
       AdjJob& a = adj[i];
       const kt_job& J = jobs[a.index];
-      std::vector<void*> dptr(9);
-      for (int t = 0; t < 9; ++t) {
+      std::vector<void*> dptr(10);
+      for (int t = 0; t < 10; ++t) {
         dptr[t] = dev + a.off[t];
         if (blobs[i][t].second) std::memcpy(H + a.off[t], blobs[i][t].first, blobs[i][t].second);
       }
@@ -199,6 +199,7 @@ void price_portfolio_greeks(kt_engine* E, const kt_job* jobs, uint64_t n_jobs, k
       A.p.partials = partials + a.part_off;
       A.p.err = counters + APV_COUNT;  // never written: the covered programs raise no path errors
       A.seg = kAdjSeg;
+      A.tangent = (a.variant == APV_HESTON_T || a.variant == APV_HESTON_FEEDS_T) ? 1 : 0;
       std::memcpy(H + i * sizeof(AdjArgs), &A, sizeof(AdjArgs));
       nblk[i] = (int)a.n_blocks;
       nout[i] = a.need;
