@@ -430,6 +430,7 @@ std::vector<KStats> run_program_scen(kt_engine* E, const Program& prog, const st
   A.n_init = (int)prog.init.size();
   A.n_grid = (int)grid.size();
   A.n_groups = (int)groups.size();
+  A.n_uniform = uniform_group_prefix(groups.data(), A.n_groups);
   A.tab_stride = stride;
   fill_model_args(A, md, seed);
   for (int sc = 0; sc < SC && sc < kScenarios; ++sc) {

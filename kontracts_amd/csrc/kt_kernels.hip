@@ -53,21 +53,46 @@ __device__ __forceinline__ void generic_point(const PA& A, int k, const float* z
 // registers across consecutive fast groups and only written back to the
 // register file when the monitored slot changes or an interpreted group
 // follows.  Knock-out groups keep the running max (min) of log-spot over the
-// group and compare it once with the threshold log2(H/s0): some point
+// group in ext and compare it once with the threshold log2(H/s0): some point
 // satisfies S >= H exactly when the group's maximum does.
+template <int KIND>
+constexpr bool knock_out_kind() {
+  return KIND >= G_KO_GE && KIND <= G_KO_LT;
+}
+
+template <int KIND, int SC>
+__device__ __forceinline__ void ko_init(float (&ext)[SC]) {
+  constexpr bool UP = KIND == G_KO_GE || KIND == G_KO_GT;
+  FOR_SC ext[sc] = UP ? -INFINITY : INFINITY;
+}
+
+// acc[sc] = 1 where scenario sc's extreme crosses its threshold thr[sc].
+template <int KIND, int SC, class PA>
+__device__ __forceinline__ void ko_apply(const PA& A, int g, const float (&ext)[SC], float (&acc)[SC]) {
+  FOR_SC {
+    const float thr = ld_f32(A.thr, (size_t)g * SC + sc);
+    const float e = ext[xsrc<SC>(sc)];
+    bool hit;
+    if constexpr (KIND == G_KO_GE) hit = e >= thr;
+    if constexpr (KIND == G_KO_GT) hit = e > thr;
+    if constexpr (KIND == G_KO_LE) hit = e <= thr;
+    if constexpr (KIND == G_KO_LT) hit = e < thr;
+    acc[sc] = hit ? 1.0f : acc[sc];
+  }
+}
+
+// The four points of a fast group: advance, and feed the monitor (knock-out
+// kinds update ext only).
 template <int KIND, int MODEL, int K, int SC, bool DEBUG, class PA>
-__device__ __forceinline__ void fast_group(const PA& A, const KGroup gd, int g, int k0, const float* z,
-                                           float (&x)[SC][K], float& v, float (&acc)[SC], float (&comp)[SC],
-                                           bool live, uint64_t local) {
+__device__ __forceinline__ void fast_points(const PA& A, const KGroup gd, int k0, const float* z, float (&x)[SC][K],
+                                            float& v, float (&acc)[SC], float (&comp)[SC], float (&ext)[SC],
+                                            bool live, uint64_t local) {
   constexpr int M = MODEL == MK_GBM ? 1 : (MODEL == MK_HESTON ? 2 : K);
-  constexpr bool KO = KIND >= G_KO_GE && KIND <= G_KO_LT;
+  constexpr bool KO = knock_out_kind<KIND>();
   constexpr bool UP = KIND == G_KO_GE || KIND == G_KO_GT;
   const uint32_t asset = (uint32_t)gd.asset < (uint32_t)K ? (uint32_t)gd.asset : 0u;
-  float ext[SC], s0v[SC];
-  FOR_SC {
-    ext[sc] = UP ? -INFINITY : INFINITY;
-    s0v[sc] = scen_s0<K, SC>(A, sc, asset);
-  }
+  float s0v[SC];
+  FOR_SC s0v[sc] = scen_s0<K, SC>(A, sc, asset);
 #pragma unroll
   for (int s = 0; s < 4; ++s) {
     advance<MODEL, K, SC>(A, k0 + s, &z[s * M], x, v);
@@ -91,18 +116,59 @@ __device__ __forceinline__ void fast_group(const PA& A, const KGroup gd, int g, 
       }
     }
   }
-  if constexpr (KO) {
-    FOR_SC {
-      const float thr = ld_f32(A.thr, (size_t)g * SC + sc);
-      const float e = ext[xsrc<SC>(sc)];
-      bool hit;
-      if constexpr (KIND == G_KO_GE) hit = e >= thr;
-      if constexpr (KIND == G_KO_GT) hit = e > thr;
-      if constexpr (KIND == G_KO_LE) hit = e <= thr;
-      if constexpr (KIND == G_KO_LT) hit = e < thr;
-      acc[sc] = hit ? 1.0f : acc[sc];
-    }
+}
+
+template <int KIND, int MODEL, int K, int SC, bool DEBUG, class PA>
+__device__ __forceinline__ void fast_group(const PA& A, const KGroup gd, int g, int k0, const float* z,
+                                           float (&x)[SC][K], float& v, float (&acc)[SC], float (&comp)[SC],
+                                           bool live, uint64_t local) {
+  float ext[SC];
+  if constexpr (knock_out_kind<KIND>()) ko_init<KIND, SC>(ext);
+  fast_points<KIND, MODEL, K, SC, DEBUG>(A, gd, k0, z, x, v, acc, comp, ext, live, local);
+  if constexpr (knock_out_kind<KIND>()) ko_apply<KIND, SC>(A, g, ext, acc);
+}
+
+// The 4 M normals of group g (M Philox blocks), words XORed with flip.
+template <int M>
+__device__ __forceinline__ void group_normals(int g, uint32_t p_lo, uint32_t p_hi, uint32_t seed_lo,
+                                              uint32_t seed_hi, uint32_t flip, float (&z)[4 * M]) {
+#pragma unroll
+  for (int bk = 0; bk < M; ++bk) {
+    const uint4 q = philox4x32_10((uint32_t)(g * M + bk), p_lo, p_hi, 0u, seed_lo, seed_hi);
+    z[4 * bk + 0] = normal_from_word(q.x ^ flip);
+    z[4 * bk + 1] = normal_from_word(q.y ^ flip);
+    z[4 * bk + 2] = normal_from_word(q.z ^ flip);
+    z[4 * bk + 3] = normal_from_word(q.w ^ flip);
   }
+}
+
+// Fast kinds that get a loop of their own for a uniform prefix of groups.
+template <int KIND, int MODEL>
+constexpr bool uniform_loop_built() {
+  if constexpr (MODEL == MK_BASKET) return KIND == G_NONE;
+  return KIND == G_NONE || KIND == G_KO_GE || KIND == G_KO_LE || KIND == G_AVG || KIND == G_RMAX;
+}
+
+// Groups 0 .. n-1, all of kind KIND on gd's slot, asset and level: the same
+// per-point code as the dispatching loop, in a loop of its own so that the
+// monitor's accumulators stay in fixed registers across groups.  Knock-out
+// groups of one level share their thresholds, so the extreme runs over the
+// whole prefix and is compared once at its end: a point crosses the level
+// exactly when the prefix's extreme does, and the flag only changes from
+// its initial value to 1, so the result equals a compare per group.
+template <int KIND, int MODEL, int K, int SC, bool DEBUG, class PA>
+__device__ __forceinline__ void uniform_groups(const PA& A, const KGroup gd, int n, uint32_t p_lo, uint32_t p_hi,
+                                               uint32_t flip, float (&x)[SC][K], float& v, float (&acc)[SC],
+                                               float (&comp)[SC], bool live, uint64_t local) {
+  constexpr int M = MODEL == MK_GBM ? 1 : (MODEL == MK_HESTON ? 2 : K);
+  float ext[SC];
+  if constexpr (knock_out_kind<KIND>()) ko_init<KIND, SC>(ext);
+  for (int g = 0; g < n; ++g) {
+    float z[4 * M];
+    group_normals<M>(g, p_lo, p_hi, A.seed_lo, A.seed_hi, flip, z);
+    fast_points<KIND, MODEL, K, SC, DEBUG>(A, gd, 4 * g, z, x, v, acc, comp, ext, live, local);
+  }
+  if constexpr (knock_out_kind<KIND>()) ko_apply<KIND, SC>(A, 0, ext, acc);
 }
 
 // One block of 256 paths: paths blk*256 .. blk*256+255 of the launch (global
@@ -173,21 +239,59 @@ __device__ __forceinline__ void path_block(const PA& A, uint32_t blk, float (&r)
     }
   };
 
-  for (int g = 0; g < A.n_groups; ++g) {
+  auto load_group = [&](int g) {
     KGroup gd;
     gd.kind = (int32_t)ld_u32(A.groups, (size_t)g * 4);
     gd.slot = (int32_t)ld_u32(A.groups, (size_t)g * 4 + 1);
     gd.asset = (int32_t)ld_u32(A.groups, (size_t)g * 4 + 2);
     gd.imm = __uint_as_float(ld_u32(A.groups, (size_t)g * 4 + 3));
-    float z[4 * M];
-#pragma unroll
-    for (int bk = 0; bk < M; ++bk) {
-      const uint4 q = philox4x32_10((uint32_t)(g * M + bk), p_lo, p_hi, 0u, A.seed_lo, A.seed_hi);
-      z[4 * bk + 0] = normal_from_word(q.x ^ flip);
-      z[4 * bk + 1] = normal_from_word(q.y ^ flip);
-      z[4 * bk + 2] = normal_from_word(q.z ^ flip);
-      z[4 * bk + 3] = normal_from_word(q.w ^ flip);
+    return gd;
+  };
+
+  // A uniform prefix of groups runs in a loop specialised to its kind (the
+  // same arithmetic in the same order as the dispatching loop below, which
+  // then continues from where the prefix ends with the monitor still cached).
+  int g_begin = 0;
+  if constexpr (!DEBUG) {
+    const int nu = min(A.n_uniform, A.n_groups);
+    if (nu > 0) {
+      const KGroup gu = load_group(0);
+      auto enter = [&]() {
+        if (gu.kind != G_NONE) {
+          const uint32_t s0i = (uint32_t)gu.slot & (NS - 1), s1i = (s0i + 1) & (NS - 1);
+          FOR_SC {
+            macc[sc] = get(sc, s0i);
+            mcomp[sc] = gu.kind == G_AVG ? get(sc, s1i) : 0.0f;
+          }
+          mslot = gu.slot;
+          mkind = gu.kind;
+        }
+        g_begin = nu;
+      };
+      switch (gu.kind) {
+#define KT_UNIFORM(KIND)                                                                              \
+  case KIND:                                                                                          \
+    if constexpr (uniform_loop_built<KIND, MODEL>()) {                                                \
+      enter();                                                                                        \
+      uniform_groups<KIND, MODEL, K, SC, DEBUG>(A, gu, nu, p_lo, p_hi, flip, x, v, macc, mcomp, live, \
+                                                local);                                               \
+    }                                                                                                 \
+    break;
+        KT_UNIFORM(G_NONE)
+        KT_UNIFORM(G_KO_GE)
+        KT_UNIFORM(G_KO_LE)
+        KT_UNIFORM(G_AVG)
+        KT_UNIFORM(G_RMAX)
+#undef KT_UNIFORM
+        default: break;
+      }
     }
+  }
+
+  for (int g = g_begin; g < A.n_groups; ++g) {
+    const KGroup gd = load_group(g);
+    float z[4 * M];
+    group_normals<M>(g, p_lo, p_hi, A.seed_lo, A.seed_hi, flip, z);
     const int k0 = 4 * g;
     if (gd.kind != G_GENERIC && gd.kind != G_NONE && gd.slot != mslot) {
       flush();

@@ -36,6 +36,7 @@ struct PathArgs {
   int n_init;
   int n_grid;
   int n_groups;
+  int n_uniform;  // leading groups of one fast kind (uniform_group_prefix)
   int tab_stride;
   int heston_full_truncation;
   uint64_t n_paths;    // paths in this launch

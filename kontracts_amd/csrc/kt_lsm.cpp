@@ -97,6 +97,7 @@ void price_american(kt_engine* E, const char* json, const double* dates, uint64_
   A.n_init = (int)prog.init.size();
   A.n_grid = (int)grid.size();
   A.n_groups = (int)prog.groups.size();
+  A.n_uniform = uniform_group_prefix(prog.groups.data(), A.n_groups);
   A.tab_stride = stride;
   fill_model_args(A, md, cfg->seed);
   A.err = err;

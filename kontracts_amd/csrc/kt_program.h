@@ -88,6 +88,19 @@ struct KGroup {
   float imm;
 };
 
+// Number of leading groups that repeat group 0's fast kind (and, for a
+// monitor, its slot, asset and level).  The path kernels run such a prefix
+// in a loop specialised to that kind, with no per-group dispatch; 0 when
+// group 0 is interpreted.
+inline int uniform_group_prefix(const KGroup* g, int n) {
+  if (n <= 0 || g[0].kind == G_GENERIC) return 0;
+  int i = 1;
+  while (i < n && g[i].kind == g[0].kind &&
+         (g[0].kind == G_NONE || (g[i].slot == g[0].slot && g[i].asset == g[0].asset && g[i].imm == g[0].imm)))
+    ++i;
+  return i;
+}
+
 // Result statistics of one output over a set of paths: count, mean and the
 // sum of squared deviations (combined with Chan et al.'s pairwise formula).
 struct KStats {

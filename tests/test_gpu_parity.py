@@ -133,6 +133,14 @@ PARITY_CASES = {
     "basket5": (BASKET_PAYOFF, BASKET5, 50, None, 100.0, 0),
     "average_window": (K.scale((K.average_over(K.spot("X"), 0.25, 0.75) - K.spot("X")).clip(0.0), K.one("USD"))
                        @ K.at(1.0), GBM_X, 20, None, 100.0, 0),
+    "gbm_down_and_out": (K.down_and_out_call("X", 100, 85, 1.0), GBM_X, 252, None, 100.0, 0.01),
+    "heston_up_and_out": (K.up_and_out_call("X", 100, 130, 1.0), HESTON_X, 365, None, 100.0, 0.01),
+    "heston_lookback": (K.scale((K.running_max(K.spot("X")) - 100.0).clip(0.0), K.one("USD")) @ K.at(1.0),
+                        HESTON_X, 120, None, 100.0, 0),
+    # a knock-out level of 130 up to t = 0.5, then 140 on another slot after it
+    "two_barrier_levels": (K.until(K.spot("X") >= 130.0, K.one("USD") @ K.at(0.5))
+                           + (K.until(K.spot("X") >= 140.0, K.european_call("X", 100, 1.0)) @ K.at(0.5)),
+                           GBM_X, 252, None, 100.0, 0.01),
     # AverageAt extension: monthly fixings on a daily grid, under Heston
     "heston_asian_monthly": (K.scale((K.average_at(K.spot("X"), [i / 12 for i in range(1, 13)]) - 100.0).clip(0.0),
                                      K.one("USD")) @ K.at(1.0), HESTON_X, 252, None, 100.0, 0),
@@ -157,6 +165,22 @@ def test_per_path_parity(eng, name):
     assert res.price == pytest.approx(float(np.mean(pv)), rel=1e-12, abs=1e-12)
     assert res.sample_std == pytest.approx(float(np.std(pv, ddof=1)), rel=1e-9, abs=1e-12)
     assert abs(res.price - cpu_res["price"]) <= 3 * cpu_res["std_error"] + 1e-9
+
+
+@pytest.mark.parametrize("name", sorted(PARITY_CASES))
+def test_pricing_kernel_equals_dispatching_kernel(eng, name):
+    """The pricing kernels run a leading run of groups of one fast kind (and
+    one slot, asset and level) in a loop of their own, with a knock-out's
+    compare deferred to the end of the run; the debug kernel, whose per-path
+    values are checked against the oracle above, takes every group through
+    the dispatching loop.  Same paths and arithmetic: the price and spread
+    agree to the order of the reduction."""
+    contract, model, spy, grid, _, _ = PARITY_CASES[name]
+    n = 4096
+    pv, _, _, _ = debug_paths(eng, contract, model, n, 42, spy, grid)
+    res = contract.price(model, n_paths=n, seed=42, steps_per_year=spy, grid=grid)
+    assert res.price == pytest.approx(float(np.mean(pv)), rel=1e-12, abs=1e-12)
+    assert res.sample_std == pytest.approx(float(np.std(pv, ddof=1)), rel=1e-9, abs=1e-12)
 
 
 def test_normals_follow_the_counter_layout(eng):
