@@ -312,11 +312,9 @@ __device__ __forceinline__ float heston_v_step(const PA& P, int k, float z1, flo
   const float t3 = ld_f32(P.tab, (size_t)kHestonStride * k + 3);
   const float t4 = ld_f32(P.tab, (size_t)kHestonStride * k + 4);
   const float t5 = ld_f32(P.tab, (size_t)kHestonStride * k + 5);
-  const float vp = P.heston_full_truncation ? fmaxf(v, 0.0f) : v;
+  const float vp = fmaxf(v, 0.0f);
   const float sv = __builtin_amdgcn_sqrtf(vp);
-  const float wv = fmaf(P.rho, z1, P.rho_perp * z2);
-  const float vn = fmaf(t3, vp, v) + fmaf(t5 * sv, wv, t4);
-  return P.heston_full_truncation ? vn : fmaxf(vn, 0.0f);
+  return heston_v_next(v, vp, sv, z1, z2, t3, t4, t5, P.rho, P.rho_perp, heston_floor(P));
 }
 
 // Heston in forward (tangent) mode: one pass carrying d x / d theta and
@@ -379,7 +377,7 @@ __device__ __forceinline__ void tangent_heston_block(const AA& A, uint32_t blk) 
           const float dt = ld_f32(A.rev, 8 * (size_t)k + 4);
           const float sq = ld_f32(A.rev, 8 * (size_t)k + 5);
           const float kdt = ld_f32(A.rev, 8 * (size_t)k + 6);
-          const float vp = ft ? fmaxf(v, 0.0f) : v;
+          const float vp = fmaxf(v, 0.0f);  // = v under the floor scheme
           const float sv = __builtin_amdgcn_sqrtf(vp);
           const float w = fmaf(rho, z1, rho_perp * z2);
           const bool pass = !ft || v > 0.0f;  // d vp / d v
@@ -606,7 +604,7 @@ __device__ __forceinline__ void adjoint_heston_block(const AA& A, uint32_t blk, 
           const float sq = ld_f32(A.rev, 8 * (size_t)p + 5);
           const float kdt = ld_f32(A.rev, 8 * (size_t)p + 6);
           const float vprev = vb[i], z1 = z1s[i], z2 = z2s[i];
-          const float vp = ft ? fmaxf(vprev, 0.0f) : vprev;
+          const float vp = fmaxf(vprev, 0.0f);  // = vprev under the floor scheme
           const float sv = __builtin_amdgcn_sqrtf(vp);
           const float w = fmaf(rho, z1, rho_perp * z2);
           // v_new = floor(v + e0 vp + d1 + d2 sv w); x_new = x + c0 + c1 vp + c2 sv z1

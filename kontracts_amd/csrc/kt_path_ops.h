@@ -137,6 +137,25 @@ __device__ __forceinline__ void run_ops(const PA& A, int k, int begin, int count
 
 // Advance every scenario of one path to grid point k with that point's
 // normals zk[0..M).
+// Heston variance floor applied after each Euler step: 0 for the reference's
+// floor scheme (src/simulator.rs:506-541, v = max(v', 0)), -inf for full
+// truncation (v' kept, max(v, 0) used inside the step).  Under the floor
+// scheme v >= 0 always, so max(v, 0) inside the step is v itself and one
+// expression serves both schemes.
+template <class PA>
+__device__ __forceinline__ float heston_floor(const PA& A) {
+  return A.heston_full_truncation ? -INFINITY : 0.0f;
+}
+
+// v' = max(v + e0 vp + d1 + d2 sqrt(vp) (rho z1 + rho' z2), floor), with
+// vp = max(v, 0) and sv = sqrt(vp).  The variance recompute of the adjoint
+// kernels calls this same expression, so recomputed paths are bit-identical.
+__device__ __forceinline__ float heston_v_next(float v, float vp, float sv, float z1, float z2, float e0, float d1,
+                                               float d2, float rho, float rho_perp, float lo) {
+  const float wv = fmaf(rho, z1, rho_perp * z2);
+  return fmaxf(fmaf(d2 * sv, wv, fmaf(e0, vp, v) + d1), lo);
+}
+
 template <int MODEL, int K, int SC, class PA>
 __device__ __forceinline__ void advance(const PA& A, int k, const float* zk, float (&x)[SC][K], float& v) {
   if constexpr (MODEL == MK_GBM) {
@@ -151,12 +170,10 @@ __device__ __forceinline__ void advance(const PA& A, int k, const float* zk, flo
 #pragma unroll
     for (int i = 0; i < 6; ++i) t[i] = ld_f32(A.tab, (size_t)kHestonStride * k + i);
     const float z1 = zk[0], z2 = zk[1];
-    const float vp = A.heston_full_truncation ? fmaxf(v, 0.0f) : v;
+    const float vp = fmaxf(v, 0.0f);
     const float sv = __builtin_amdgcn_sqrtf(vp);
-    x[0][0] = x[0][0] + fmaf(t[1], vp, t[0]) + t[2] * sv * z1;
-    const float wv = fmaf(A.rho, z1, A.rho_perp * z2);
-    const float vn = fmaf(t[3], vp, v) + fmaf(t[5] * sv, wv, t[4]);
-    v = A.heston_full_truncation ? vn : fmaxf(vn, 0.0f);
+    x[0][0] = fmaf(t[2] * sv, z1, fmaf(t[1], vp, x[0][0]) + t[0]);
+    v = heston_v_next(v, vp, sv, z1, z2, t[3], t[4], t[5], A.rho, A.rho_perp, heston_floor(A));
   } else {
 #pragma unroll
     for (int i = 0; i < K; ++i) {
