@@ -3,24 +3,31 @@
 Unit: lane-ops, one VALU instruction executed by one lane.  The peak is
 MI355X's published FP32 vector rate, 157.3 TFLOP/s = 78.6e12 FMA lane-ops/s
 (256 CUs x 128 FP32 lanes per clock x 2.4 GHz).  The kernels issue no packed
-FP32 instructions, so this is the ceiling for their instruction stream.
+FP32 instructions, so their instruction stream is also bounded by the
+non-packed issue rate, one wave64 VALU instruction per SIMD every 4 cycles
+(256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz = 39.3e12 lane-ops/s).
 
-The counts below are fixed from the algorithm (the minimal instruction
-sequence for each step), not from the compiler's output; the measured
-instruction count (rocprof SQ_INSTS_VALU) is reported beside them.
+The counts below are the minimal instruction sequence of each step on
+gfx950 (the sequence the kernels' straight-line loops execute), not the
+compiler's total; the measured instruction count (rocprof SQ_INSTS_VALU,
+which adds per-path set-up, the payoff and the FP64 reduction) is reported
+beside them.
 
   Philox4x32-10: 10 rounds x (2 mad_u64_u32 + 2 three-input xor) per 4 words
                  -> 10 per 32-bit word
-  normal from a word: 5 (cell -> x) + 1 (fma 1-x^2) + 1 (log2) + 1 (fma)
-                 + 8 (central polynomial) + 1 (scale), plus the tail branch
-                 (13 instructions) in the ~20% of wave-normals that need it
-                 -> 20
-  one normal: 30
-  GBM advance (exact log step) per log-spot state: 2
-  Heston advance: sqrt + 11 FMA/mul/max = 12
+  normal from a word: 3 (cell -> x: shift, convert, one exact FMA)
+                 + 1 (FMA 1 - x^2) + 1 (log2) + 1 (FMA to Giles' w - 2.5)
+                 + 8 (central polynomial) + 1 (tail test) + 1 (scale by x)
+                 = 16, plus the tail branch (12 instructions), which a wave
+                 runs when any of its 64 lanes has |x| >= 0.99663
+                 (P = 1 - (1 - 0.00337)^64 = 0.194) -> 18.3
+  one normal: 28.3
+  GBM advance (exact log step) per log-spot state: 2 (add, FMA)
+  Heston advance: max, sqrt, 4 for the log-spot, 7 for the variance = 13
   basket advance, asset i (0-based): (i+1) Cholesky FMAs + 2
-  knock-out monitor: running extreme 1 per point and state, plus compare and
-                 select once per 4 points and scenario (0.5 per point)
+  knock-out monitor: running extreme, 0.5 per point and state (one
+                 three-input max folds two points), compared with the level
+                 once per run of points (0 per point)
   spot read (exp2 + mul) 2; Kahan sum 4; running max/min 1
 """
 
@@ -29,7 +36,10 @@ PEAK_LANE_OPS = 256 * 128 * 2.4e9  # 78.6e12
 # the rate one VALU instruction per lane per cycle reaches.
 PEAK_NONPACKED_LANE_OPS = 256 * 64 * 2.4e9  # 39.3e12
 PEAK_FP32_TFLOPS = 157.3
-NORMAL = 30
+PHILOX_PER_WORD = 10.0
+TAIL_WAVE_PROBABILITY = 1.0 - (1.0 - 0.00337) ** 64
+INVERSE_NORMAL = 16.0 + 12.0 * TAIL_WAVE_PROBABILITY
+NORMAL = PHILOX_PER_WORD + INVERSE_NORMAL  # 28.3
 
 
 def gbm_european() -> float:
@@ -38,13 +48,14 @@ def gbm_european() -> float:
 
 def gbm_barrier(scenarios: int = 1, states: int | None = None) -> float:
     """One path-step of a knock-out contract under GBM with `scenarios` CRN
-    scenarios sharing the normal, `states` distinct log-spot paths."""
+    scenarios sharing the normal, `states` distinct log-spot paths (the
+    scenarios' compares with their levels happen once per path)."""
     states = scenarios if states is None else states
-    return NORMAL + 2 * states + 1 * states + 0.5 * scenarios
+    return NORMAL + 2 * states + 0.5 * states
 
 
 def heston_european() -> float:
-    return 2 * NORMAL + 12
+    return 2 * NORMAL + 13
 
 
 def basket_european(k: int) -> float:
