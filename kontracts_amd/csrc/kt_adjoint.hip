@@ -181,9 +181,12 @@ __device__ __forceinline__ void adjoint_gbm_block(const AA& A, uint32_t blk) {
   const int n = P.n_grid - 1;
   float r[1][NS];
   init_registers<NS>(P, r);
-  float x[1][K], js[K], jm[K];
+  // ws[a] = sum of sqrt(dt) w_a over the steps so far: d x_a / d sigma_a at
+  // time t is (ws[a] - sigma_a t) log2(e) and d x_a / d mu_a is t log2(e),
+  // so one FMA per asset and point carries both.
+  float x[1][K], ws[K];
 #pragma unroll
-  for (int a = 0; a < K; ++a) x[0][a] = js[a] = jm[a] = 0.0f;
+  for (int a = 0; a < K; ++a) x[0][a] = ws[a] = 0.0f;
   float W[kAdjMaxAcc][3];
   int argstep[kAdjMaxAcc];
 #pragma unroll
@@ -208,15 +211,13 @@ __device__ __forceinline__ void adjoint_gbm_block(const AA& A, uint32_t blk) {
       const int k = 4 * g + s;
       if (k <= n) {
         const float* zk = &z[s * M];
-        // d x_a / d sigma_a += (-sigma_a dt + sqrt(dt) w_a) log2(e); d x_a / d mu_a += dt log2(e)
-        const float dt = ld_f32(A.adj_tab, 2 * (size_t)k), sq = ld_f32(A.adj_tab, 2 * (size_t)k + 1);
+        const float sq = ld_f32(A.adj_tab, 2 * (size_t)k);
 #pragma unroll
         for (int a = 0; a < K; ++a) {
           float w = 0.0f;
 #pragma unroll
           for (int c = 0; c <= a; ++c) w = fmaf(K == 1 ? 1.0f : P.chol[a * K + c], zk[c], w);
-          js[a] = fmaf(fmaf(-A.sigma[a], dt, sq * w), kLog2ef, js[a]);
-          jm[a] = fmaf(dt, kLog2ef, jm[a]);
+          ws[a] = fmaf(sq, w, ws[a]);
         }
         advance<MODEL, K, 1>(P, k, zk, x, v);
         const uint32_t fm = ld_u32(A.feed, (size_t)k);
@@ -229,15 +230,17 @@ __device__ __forceinline__ void adjoint_gbm_block(const AA& A, uint32_t blk) {
           for (int q = 0; q < kAdjMaxAcc; ++q) {
             if ((fm >> q) & 1u) {
               const int aq = A.acc_asset[q];
-              float xa = x[0][0], ja = js[0], jma = jm[0], s0a = P.s0[0];
+              float xa = x[0][0], wa = ws[0], sga = A.sigma[0], s0a = P.s0[0];
 #pragma unroll
               for (int c = 1; c < K; ++c)
                 if (aq == c) {
                   xa = x[0][c];
-                  ja = js[c];
-                  jma = jm[c];
+                  wa = ws[c];
+                  sga = A.sigma[c];
                   s0a = P.s0[c];
                 }
+              const float jma = ld_f32(A.adj_tab, 2 * (size_t)k + 1);  // t log2(e)
+              const float ja = fmaf(wa, kLog2ef, -sga * jma);
               // S / s0 = 2^x: no division on the per-point path
               const float g = exp2_fast(xa), S = s0a * g;
               const float d0 = S * kLn2f * ja, d1 = S * kLn2f * jma, d2 = g;
@@ -257,7 +260,10 @@ __device__ __forceinline__ void adjoint_gbm_block(const AA& A, uint32_t blk) {
     }
   }
   // payoff derivatives at the flow date
-  float S[K];
+  const float tl_n = ld_f32(A.adj_tab, 2 * (size_t)n + 1);
+  float S[K], js[K];
+#pragma unroll
+  for (int a = 0; a < K; ++a) js[a] = fmaf(ws[a], kLog2ef, -A.sigma[a] * tl_n);
 #pragma unroll
   for (int a = 0; a < K; ++a) S[a] = P.s0[a] * exp2_fast(x[0][a]);
   float gr[NP];
@@ -271,7 +277,7 @@ __device__ __forceinline__ void adjoint_gbm_block(const AA& A, uint32_t blk) {
       const float ds = dual_point<NS, K>(A, n, begin, count, S, r, a);
       gr[a] += ds * S[a] / P.s0[a];
       gr[K + a] += ds * S[a] * kLn2f * js[a];
-      gr[2 * K + a] += ds * S[a] * kLn2f * jm[a];
+      gr[2 * K + a] += ds * S[a] * kLn2f * tl_n;
     }
     for (int q = 0; q < A.n_acc; ++q) {
       const float ab = dual_point<NS, K>(A, n, begin, count, S, r, K + q);

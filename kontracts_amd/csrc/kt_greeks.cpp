@@ -345,13 +345,19 @@ AdjJobHost adjoint_job_host(const Program& prog, const ModelDesc& md, const std:
   H.tab = model_table(md, grid, &H.stride);
   H.df.resize(grid.size());
   for (size_t k = 0; k < grid.size(); ++k) H.df[k] = std::exp(-rate * grid[k]);
+  // GBM / basket: {sqrt(dt), t log2(e)} per point (d x / d sigma is
+  // (W_t - sigma t) log2(e), W_t the sum of sqrt(dt) w; d x / d mu is t log2(e))
   H.adj.assign(2 * grid.size(), 0.0f);
+  std::vector<double> dts(grid.size(), 0.0);
+  double t_acc = 0.0;
   for (size_t k = 1; k < grid.size(); ++k) {
     const double dt = grid[k] - grid[k - 1];
     if (dt > 0.0) {
-      H.adj[2 * k] = (float)dt;
-      H.adj[2 * k + 1] = (float)std::sqrt(dt);
+      dts[k] = dt;
+      t_acc += dt;
+      H.adj[2 * k] = (float)std::sqrt(dt);
     }
+    H.adj[2 * k + 1] = (float)(t_acc * kLog2e);
   }
   H.ops = prog.ops;
   if (H.ops.empty()) H.ops.push_back(KOp{OP_NOP, 0, 0, 0, 0.0f});
@@ -367,9 +373,9 @@ AdjJobHost adjoint_job_host(const Program& prog, const ModelDesc& md, const std:
       r[1] = t[2];
       r[2] = t[3];
       r[3] = t[5];
-      r[4] = H.adj[2 * k];
-      r[5] = H.adj[2 * k + 1];
-      r[6] = (float)md.kappa * H.adj[2 * k];
+      r[4] = (float)dts[k];
+      r[5] = (float)std::sqrt(dts[k]);
+      r[6] = (float)md.kappa * (float)dts[k];
     }
   }
   return H;

@@ -77,7 +77,7 @@ constexpr int kAdjMaxAcc = 4;
 constexpr int kAdjMaxOut = 1 + 3 * kMaxAssets;
 struct AdjArgs {
   PathArgs p;               // program, model tables, RNG (n_grid points, last = flow date)
-  const float* adj_tab;     // [n_grid][2] {dt, sqrt(dt)} (zero at point 0)
+  const float* adj_tab;     // [n_grid][2] {sqrt(dt), t log2(e)} (GBM and basket; sqrt(dt) zero at point 0)
   const uint32_t* feed;     // [n_grid] bit q: accumulator q is fed at this point
   const float* rev;         // Heston: [n_grid][8] {t1, t2, t3, t5, dt, sqrt(dt), kappa dt, 0} for the reverse sweep
   float* ck;                // Heston: [n_ck][2][ck_stride] {log2(S/s0), v} before point seg*c
