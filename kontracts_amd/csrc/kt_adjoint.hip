@@ -360,52 +360,57 @@ __device__ __forceinline__ void tangent_heston_block(const AA& A, uint32_t blk) 
   for (int q = 0; q < (FEEDS ? kAdjMaxAcc : 1); ++q)
 #pragma unroll
     for (int j = 0; j <= NT; ++j) W[q][j] = 0.0f;
-  for (int g = 0; g < P.n_groups; ++g) {
-    float z[8];
+  // one Euler step at point k >= 1 with its tangents
+  auto step = [&](int k, const float* zk) {
+    const float z1 = zk[0], z2 = zk[1];
+    if (k >= 1) {
+      const float t1 = ld_f32(A.rev, 8 * (size_t)k + 0);
+      const float t2 = ld_f32(A.rev, 8 * (size_t)k + 1);
+      const float t3 = ld_f32(A.rev, 8 * (size_t)k + 2);
+      const float t5 = ld_f32(A.rev, 8 * (size_t)k + 3);
+      const float dt = ld_f32(A.rev, 8 * (size_t)k + 4);
+      const float sq = ld_f32(A.rev, 8 * (size_t)k + 5);
+      const float kdt = ld_f32(A.rev, 8 * (size_t)k + 6);
+      const float vp = fmaxf(v, 0.0f);  // = v under the floor scheme
+      const float sv = __builtin_amdgcn_sqrtf(vp);
+      const float w = fmaf(rho, z1, rho_perp * z2);
+      const bool pass = !ft || v > 0.0f;  // d vp / d v
+      const float hs = sv > 0.0f ? 0.5f * __builtin_amdgcn_rsqf(vp) : 0.0f;  // d sqrt(vp) / d vp
+      const float cx = t2 * z1, cv = t5 * w;
+      float dvn[NT];
 #pragma unroll
-    for (int bk = 0; bk < 2; ++bk) {
-      const uint4 q = philox4x32_10((uint32_t)(2 * g + bk), p_lo, p_hi, 0u, P.seed_lo, P.seed_hi);
-      z[4 * bk + 0] = normal_from_word(q.x);
-      z[4 * bk + 1] = normal_from_word(q.y);
-      z[4 * bk + 2] = normal_from_word(q.z);
-      z[4 * bk + 3] = normal_from_word(q.w);
+      for (int j = 0; j < NT; ++j) {
+        const float dvp = pass ? dv[j] : 0.0f;
+        const float dsv = hs * dvp;
+        dx[j] = fmaf(cx, dsv, fmaf(t1, dvp, dx[j]));
+        dvn[j] = fmaf(cv, dsv, fmaf(t3, dvp, dv[j]));
+      }
+      dvn[1] = fmaf(dt, theta - vp, dvn[1]);                  // kappa
+      dvn[2] += kdt;                                          // theta
+      dvn[3] = fmaf(sq * sv, w, dvn[3]);                      // sigma_v
+      dvn[4] = fmaf(t5 * sv, fmaf(-rho_ratio, z2, z1), dvn[4]);  // rho
+      advance<MK_HESTON, 1, 1>(P, k, zk, x, v);
+      const bool keep = ft || v > 0.0f;  // the floor cuts the derivative
+#pragma unroll
+      for (int j = 0; j < NT; ++j) dv[j] = keep ? dvn[j] : 0.0f;
     }
+  };
+  // leading groups whose points run no ops and feed nothing (P.n_uniform)
+  const int n_quiet = min(P.n_uniform, P.n_groups);
+  for (int g = 0; g < n_quiet; ++g) {
+    float z[8];
+    group_normals<2>(g, p_lo, p_hi, P.seed_lo, P.seed_hi, 0u, z);
+#pragma unroll
+    for (int s = 0; s < 4; ++s) step(4 * g + s, &z[2 * s]);
+  }
+  for (int g = n_quiet; g < P.n_groups; ++g) {
+    float z[8];
+    group_normals<2>(g, p_lo, p_hi, P.seed_lo, P.seed_hi, 0u, z);
 #pragma unroll
     for (int s = 0; s < 4; ++s) {
       const int k = 4 * g + s;
       if (k <= n) {
-        const float z1 = z[2 * s], z2 = z[2 * s + 1];
-        if (k >= 1) {
-          const float t1 = ld_f32(A.rev, 8 * (size_t)k + 0);
-          const float t2 = ld_f32(A.rev, 8 * (size_t)k + 1);
-          const float t3 = ld_f32(A.rev, 8 * (size_t)k + 2);
-          const float t5 = ld_f32(A.rev, 8 * (size_t)k + 3);
-          const float dt = ld_f32(A.rev, 8 * (size_t)k + 4);
-          const float sq = ld_f32(A.rev, 8 * (size_t)k + 5);
-          const float kdt = ld_f32(A.rev, 8 * (size_t)k + 6);
-          const float vp = fmaxf(v, 0.0f);  // = v under the floor scheme
-          const float sv = __builtin_amdgcn_sqrtf(vp);
-          const float w = fmaf(rho, z1, rho_perp * z2);
-          const bool pass = !ft || v > 0.0f;  // d vp / d v
-          const float hs = sv > 0.0f ? 0.5f * __builtin_amdgcn_rsqf(vp) : 0.0f;  // d sqrt(vp) / d vp
-          const float cx = t2 * z1, cv = t5 * w;
-          float dvn[NT];
-#pragma unroll
-          for (int j = 0; j < NT; ++j) {
-            const float dvp = pass ? dv[j] : 0.0f;
-            const float dsv = hs * dvp;
-            dx[j] = fmaf(cx, dsv, fmaf(t1, dvp, dx[j]));
-            dvn[j] = fmaf(cv, dsv, fmaf(t3, dvp, dv[j]));
-          }
-          dvn[1] = fmaf(dt, theta - vp, dvn[1]);                  // kappa
-          dvn[2] += kdt;                                          // theta
-          dvn[3] = fmaf(sq * sv, w, dvn[3]);                      // sigma_v
-          dvn[4] = fmaf(t5 * sv, fmaf(-rho_ratio, z2, z1), dvn[4]);  // rho
-          advance<MK_HESTON, 1, 1>(P, k, &z[2 * s], x, v);
-          const bool keep = ft || v > 0.0f;  // the floor cuts the derivative
-#pragma unroll
-          for (int j = 0; j < NT; ++j) dv[j] = keep ? dvn[j] : 0.0f;
-        }
+        step(k, &z[2 * s]);
         if constexpr (FEEDS) {
           const uint32_t fm = ld_u32(A.feed, (size_t)k);
           int before[kAdjMaxAcc];
@@ -493,25 +498,32 @@ __device__ __forceinline__ void adjoint_heston_block(const AA& A, uint32_t blk, 
 #pragma unroll
   for (int q = 0; q < kAdjMaxAcc; ++q) argstep[q] = 0;
   double pv[1][1] = {{0.0}};
-  for (int g = 0; g < P.n_groups; ++g) {
-    float z[8];
-#pragma unroll
-    for (int bk = 0; bk < 2; ++bk) {
-      const uint4 q = philox4x32_10((uint32_t)(2 * g + bk), p_lo, p_hi, 0u, P.seed_lo, P.seed_hi);
-      z[4 * bk + 0] = normal_from_word(q.x);
-      z[4 * bk + 1] = normal_from_word(q.y);
-      z[4 * bk + 2] = normal_from_word(q.z);
-      z[4 * bk + 3] = normal_from_word(q.w);
+  auto checkpoint = [&](int k) {
+    if ((k & (SEG - 1)) == 0 && live) {
+      const size_t c = (size_t)(k / SEG);
+      if constexpr (FEEDS) ck[(2 * c) * cs + lane] = x[0][0];
+      ck[(2 * c + 1) * cs + lane] = v;
     }
+  };
+  // leading groups without ops (P.n_uniform): advance and checkpoint only
+  const int n_quiet = min(P.n_uniform, P.n_groups);
+  for (int g = 0; g < n_quiet; ++g) {
+    float z[8];
+    group_normals<2>(g, p_lo, p_hi, P.seed_lo, P.seed_hi, 0u, z);
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      checkpoint(4 * g + s);
+      advance<MK_HESTON, 1, 1>(P, 4 * g + s, &z[2 * s], x, v);
+    }
+  }
+  for (int g = n_quiet; g < P.n_groups; ++g) {
+    float z[8];
+    group_normals<2>(g, p_lo, p_hi, P.seed_lo, P.seed_hi, 0u, z);
 #pragma unroll
     for (int s = 0; s < 4; ++s) {
       const int k = 4 * g + s;
       if (k <= n) {
-        if ((k & (SEG - 1)) == 0 && live) {
-          const size_t c = (size_t)(k / SEG);
-          if constexpr (FEEDS) ck[(2 * c) * cs + lane] = x[0][0];
-          ck[(2 * c + 1) * cs + lane] = v;
-        }
+        checkpoint(k);
         advance<MK_HESTON, 1, 1>(P, k, &z[2 * s], x, v);
         run_point_ops<NS, 1>(A, k, r, pv, x, live, FEEDS ? ld_u32(A.feed, (size_t)k) : 0u, argstep);
       }
@@ -557,7 +569,9 @@ __device__ __forceinline__ void adjoint_heston_block(const AA& A, uint32_t blk, 
       if constexpr (FEEDS) x_ck = live ? ck[(2 * (size_t)(c - 1)) * cs + lane] : 0.0f;
       v_ck = live ? ck[(2 * (size_t)(c - 1) + 1) * cs + lane] : 0.0f;
     }
-    float vb[SEG], z1s[SEG], z2s[SEG];
+    // per point: sqrt(max(v, 0)) before the step (the sweep needs it and its
+    // square, not v), and the two normals
+    float svs[SEG], z1s[SEG], z2s[SEG];
 #pragma unroll
     for (int gq = 0; gq < SEG / 4; ++gq) {
       const int g = (SEG / 4) * c + gq;
@@ -573,7 +587,7 @@ __device__ __forceinline__ void adjoint_heston_block(const AA& A, uint32_t blk, 
 #pragma unroll
       for (int s = 0; s < 4; ++s) {
         const int i = 4 * gq + s;
-        vb[i] = vs;
+        svs[i] = __builtin_amdgcn_sqrtf(fmaxf(vs, 0.0f));  // the step's own sqrt (same expression)
         z1s[i] = z[2 * s];
         z2s[i] = z[2 * s + 1];
         if (4 * g + s <= n) {
@@ -583,7 +597,7 @@ __device__ __forceinline__ void adjoint_heston_block(const AA& A, uint32_t blk, 
       }
     }
     // reverse sweep through the segment; xs, vs hold the state after its last point
-    float vnext = vs;
+    float sv_next = __builtin_amdgcn_sqrtf(fmaxf(vs, 0.0f));  // > 0 exactly when v after the step is
 #pragma unroll
     for (int i = SEG - 1; i >= 0; --i) {
       const int p = SEG * c + i;
@@ -609,27 +623,26 @@ __device__ __forceinline__ void adjoint_heston_block(const AA& A, uint32_t blk, 
           const float dt = ld_f32(A.rev, 8 * (size_t)p + 4);
           const float sq = ld_f32(A.rev, 8 * (size_t)p + 5);
           const float kdt = ld_f32(A.rev, 8 * (size_t)p + 6);
-          const float vprev = vb[i], z1 = z1s[i], z2 = z2s[i];
-          const float vp = fmaxf(vprev, 0.0f);  // = vprev under the floor scheme
-          const float sv = __builtin_amdgcn_sqrtf(vp);
+          const float sv = svs[i], z1 = z1s[i], z2 = z2s[i];
+          const float vp = sv * sv;  // max(v, 0) to rounding
           const float w = fmaf(rho, z1, rho_perp * z2);
           // v_new = floor(v + e0 vp + d1 + d2 sv w); x_new = x + c0 + c1 vp + c2 sv z1
-          const float vnb = (ft || vnext > 0.0f) ? vbar : 0.0f;
+          const float vnb = (ft || sv_next > 0.0f) ? vbar : 0.0f;
           float vpb = fmaf(xb, t1, vnb * t3);
           const float svb = fmaf(xb * t2, z1, vnb * t5 * w);
           g_k = fmaf(vnb, dt * (theta - vp), g_k);
           g_th = fmaf(vnb, kdt, g_th);
           g_sv = fmaf(vnb, sq * sv * w, g_sv);
           g_rho = fmaf(vnb * t5 * sv, fmaf(-rho_ratio, z2, z1), g_rho);
-          // d sqrt(v)/dv = 0.5 / sqrt(v) through the hardware reciprocal square
-          // root (one instruction instead of an IEEE division)
-          vpb = sv > 0.0f ? fmaf(svb, 0.5f * __builtin_amdgcn_rsqf(vp), vpb) : vpb;
-          vbar = vnb + ((!ft || vprev > 0.0f) ? vpb : 0.0f);
+          // d sqrt(v)/dv = 0.5 / sqrt(v) through the hardware reciprocal (one
+          // instruction instead of an IEEE division)
+          vpb = sv > 0.0f ? fmaf(svb, 0.5f * __builtin_amdgcn_rcpf(sv), vpb) : vpb;
+          vbar = vnb + ((!ft || sv > 0.0f) ? vpb : 0.0f);
           if constexpr (FEEDS) {
             const float t0 = ld_f32(P.tab, (size_t)kHestonStride * p + 0);
             xs[0][0] = xs[0][0] - (fmaf(t1, vp, t0) + t2 * sv * z1);
           }
-          vnext = vprev;
+          sv_next = sv;
         }
       }
     }

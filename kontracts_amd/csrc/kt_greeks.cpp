@@ -407,6 +407,11 @@ void fill_adjoint_args(AdjArgs& A, const AdjJobHost& H, const Program& prog, con
   P.n_init = (int)prog.init.size();
   P.n_grid = (int)grid.size();
   P.n_groups = (int)((grid.size() + 3) / 4);
+  // leading groups whose points run no ops: the Heston kernels advance them
+  // with no interpretation
+  P.n_uniform = !prog.groups.empty() && prog.groups[0].kind == G_NONE
+                    ? std::min(uniform_group_prefix(prog.groups.data(), (int)prog.groups.size()), P.n_groups)
+                    : 0;
   P.tab_stride = H.stride;
   fill_model_args(P, md, seed);
   A.adj_tab = (const float*)dev[7];

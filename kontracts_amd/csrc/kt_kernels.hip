@@ -128,20 +128,6 @@ __device__ __forceinline__ void fast_group(const PA& A, const KGroup gd, int g, 
   if constexpr (knock_out_kind<KIND>()) ko_apply<KIND, SC>(A, g, ext, acc);
 }
 
-// The 4 M normals of group g (M Philox blocks), words XORed with flip.
-template <int M>
-__device__ __forceinline__ void group_normals(int g, uint32_t p_lo, uint32_t p_hi, uint32_t seed_lo,
-                                              uint32_t seed_hi, uint32_t flip, float (&z)[4 * M]) {
-#pragma unroll
-  for (int bk = 0; bk < M; ++bk) {
-    const uint4 q = philox4x32_10((uint32_t)(g * M + bk), p_lo, p_hi, 0u, seed_lo, seed_hi);
-    z[4 * bk + 0] = normal_from_word(q.x ^ flip);
-    z[4 * bk + 1] = normal_from_word(q.y ^ flip);
-    z[4 * bk + 2] = normal_from_word(q.z ^ flip);
-    z[4 * bk + 3] = normal_from_word(q.w ^ flip);
-  }
-}
-
 // Fast kinds that get a loop of their own for a uniform prefix of groups.
 template <int KIND, int MODEL>
 constexpr bool uniform_loop_built() {
@@ -406,6 +392,18 @@ __global__ __launch_bounds__(kBlock) void path_kernel(const PathArgs A) {
   path_block<MODEL, K, NS, NOUT, SC, NSTAT, DEBUG, ANTI>(A, blockIdx.x, r);
 }
 
+// The seven-scenario CRN kernel allocates 123 VGPRs (4 waves per SIMD) for
+// its payoff and reduction epilogue; capped at 102 (5 waves) the epilogue
+// spills a few values and the path loop, which needs fewer, runs 3% faster
+// (measured 1.006 against 1.038 ms per launch at 4,194,304 x 252; 6 waves
+// cost 9% and 8 waves 29% through spills in the loop itself).
+template <>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))) void
+path_kernel<MK_GBM, 1, 16, 1, kScenarios, kScenarioStats, false, false>(const PathArgs A) {
+  float r[kScenarios][16];
+  path_block<MK_GBM, 1, 16, 1, kScenarios, kScenarioStats, false, false>(A, blockIdx.x, r);
+}
+
 // Block j reduces output j.  err (nullable): block 0 copies the launch's
 // error word to *err_copy and clears it for the next launch.
 __global__ __launch_bounds__(256) void finalize_kernel(const KStats* __restrict__ partials, int n_blocks, int nout,
@@ -483,7 +481,7 @@ __global__ __launch_bounds__(256) void portfolio_finalize_kernel(const Portfolio
 // chunk (identity on the last partial one), so it only changes which
 // workgroup computes a block, never a result.
 template <int NS>
-__global__ __launch_bounds__(kBlock) void surface_kernel(const PathArgs* __restrict__ cells, uint32_t n_blocks,
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))) void surface_kernel(const PathArgs* __restrict__ cells, uint32_t n_blocks,
                                                          uint32_t total, uint32_t n_cu, uint32_t per_cu) {
   const uint32_t L = blockIdx.x;
   const uint32_t chunk = n_cu * per_cu;

@@ -137,6 +137,20 @@ __device__ __forceinline__ void run_ops(const PA& A, int k, int begin, int count
 
 // Advance every scenario of one path to grid point k with that point's
 // normals zk[0..M).
+// The 4 M normals of group g (M Philox blocks), words XORed with flip.
+template <int M>
+__device__ __forceinline__ void group_normals(int g, uint32_t p_lo, uint32_t p_hi, uint32_t seed_lo,
+                                              uint32_t seed_hi, uint32_t flip, float (&z)[4 * M]) {
+#pragma unroll
+  for (int bk = 0; bk < M; ++bk) {
+    const uint4 q = philox4x32_10((uint32_t)(g * M + bk), p_lo, p_hi, 0u, seed_lo, seed_hi);
+    z[4 * bk + 0] = normal_from_word(q.x ^ flip);
+    z[4 * bk + 1] = normal_from_word(q.y ^ flip);
+    z[4 * bk + 2] = normal_from_word(q.z ^ flip);
+    z[4 * bk + 3] = normal_from_word(q.w ^ flip);
+  }
+}
+
 // Heston variance floor applied after each Euler step: 0 for the reference's
 // floor scheme (src/simulator.rs:506-541, v = max(v', 0)), -inf for full
 // truncation (v' kept, max(v, 0) used inside the step).  Under the floor
