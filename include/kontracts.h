@@ -290,6 +290,13 @@ kt_status kt_debug_normals(kt_engine* engine, const uint32_t* words, uint64_t n,
  * copy kernel, repeated `reps` times; *ms receives the average kernel time.
  * Calibrates the profiler's HBM byte counters against a known size. */
 kt_status kt_debug_copy(kt_engine* engine, uint64_t bytes, int32_t reps, double* ms);
+/* VALU issue calibration: 2^21 lanes each run `iters` (a multiple of 8)
+ * iterations of eight independent FP32 FMAs (eight VALU instructions per
+ * iteration, the loop unrolled by 8), repeated
+ * `reps` times after one warm launch; *ms receives the average kernel time
+ * and *lane_ops the FMA lane-ops per second.  Measures the vector issue rate
+ * the path kernels' roofline is compared with. */
+kt_status kt_debug_valu(kt_engine* engine, int32_t iters, int32_t reps, double* ms, double* lane_ops);
 /* kt_price that also returns every path's present value (pv_out[n_paths]) and,
  * when normals_out != NULL, the FP32 normals the GPU drew, laid out
  * [path][grid point][component].  grid_out receives the grid (capacity

@@ -431,6 +431,7 @@ std::vector<KStats> run_program_scen(kt_engine* E, const Program& prog, const st
   A.n_grid = (int)grid.size();
   A.n_groups = (int)groups.size();
   A.n_uniform = uniform_group_prefix(groups.data(), A.n_groups);
+  A.tab_uniform = uniform_table_rows(tab, grid.size(), (size_t)stride);
   A.tab_stride = stride;
   fill_model_args(A, md, seed);
   for (int sc = 0; sc < SC && sc < kScenarios; ++sc) {
@@ -811,6 +812,30 @@ kt_status kt_debug_copy(kt_engine* E, uint64_t bytes, int32_t reps, double* ms) 
   (void)hipEventDestroy(a);
   (void)hipEventDestroy(b);
   if (ms) *ms = t / reps;
+  return KT_OK;
+  KT_GUARD_END(E)
+}
+
+kt_status kt_debug_valu(kt_engine* E, int32_t iters, int32_t reps, double* ms, double* lane_ops) {
+  KT_GUARD_BEGIN
+  if (!E || iters < 8 || iters % 8 || reps < 1) raise(KT_ERR_INVALID_ARGUMENT, "bad VALU probe arguments");
+  check_hip(hipSetDevice(E->device), "hipSetDevice");
+  float* out = (float*)E->ensure(E->aux, (size_t)kValuProbeBlocks * 256 * sizeof(float));
+  check_hip(launch_debug_valu(out, iters, E->stream), "valu launch");
+  hipEvent_t a, b;
+  check_hip(hipEventCreate(&a), "event");
+  check_hip(hipEventCreate(&b), "event");
+  check_hip(hipEventRecord(a, E->stream), "record");
+  for (int i = 0; i < reps; ++i) check_hip(launch_debug_valu(out, iters, E->stream), "valu launch");
+  check_hip(hipEventRecord(b, E->stream), "record");
+  check_hip(hipEventSynchronize(b), "sync");
+  float t = 0.0f;
+  check_hip(hipEventElapsedTime(&t, a, b), "elapsed");
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  const double avg_ms = (double)t / reps;
+  if (ms) *ms = avg_ms;
+  if (lane_ops) *lane_ops = (double)kValuProbeBlocks * 256 * (double)iters * 8.0 / (avg_ms * 1e-3);
   return KT_OK;
   KT_GUARD_END(E)
 }

@@ -3,6 +3,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <cstring>
 #include <memory>
 #include <string>
 #include <utility>
@@ -91,6 +92,16 @@ struct PreparedContract {
 
 extern const double kZ95;
 extern const double kLog2e;
+
+// 1 when rows 1 .. n_rows - 1 of a per-point table (stride floats a row) are
+// bitwise identical: a grid of equal steps, whose coefficients the path
+// kernels then hold in registers.
+inline int uniform_table_rows(const std::vector<float>& tab, size_t n_rows, size_t stride) {
+  if (n_rows < 3 || tab.size() < n_rows * stride) return 0;
+  for (size_t k = 2; k < n_rows; ++k)
+    if (std::memcmp(&tab[k * stride], &tab[stride], stride * sizeof(float)) != 0) return 0;
+  return 1;
+}
 
 ModelDesc model_from_abi(const kt_model* m);
 void validate_model(const ModelDesc& d);

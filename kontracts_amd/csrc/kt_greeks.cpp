@@ -234,6 +234,7 @@ void greek_surface(kt_engine* E, const char* json, const char* asset, const doub
       A.n_grid = (int)grid.size();
       A.n_groups = (int)T[i].groups.size();
       A.n_uniform = uniform_group_prefix(T[i].groups.data(), A.n_groups);
+      A.tab_uniform = uniform_table_rows(T[i].tab, grid.size(), (size_t)T[i].stride);
       A.tab_stride = T[i].stride;
       fill_model_args(A, sc_models[0], cfg->seed);
       for (int sc = 0; sc < kScenarios; ++sc) {

@@ -82,11 +82,13 @@ __device__ __forceinline__ void ko_apply(const PA& A, int g, const float (&ext)[
 }
 
 // The four points of a fast group: advance, and feed the monitor (knock-out
-// kinds update ext only).
-template <int KIND, int MODEL, int K, int SC, bool DEBUG, class PA>
+// kinds update ext only).  ROW: the step coefficients come from *R (VGPRs)
+// instead of the table.
+template <int KIND, int MODEL, int K, int SC, bool DEBUG, bool ROW = false, class PA>
 __device__ __forceinline__ void fast_points(const PA& A, const KGroup gd, int k0, const float* z, float (&x)[SC][K],
                                             float& v, float (&acc)[SC], float (&comp)[SC], float (&ext)[SC],
-                                            bool live, uint64_t local) {
+                                            bool live, uint64_t local,
+                                            const StepRow<MODEL, K, SC>* R = nullptr) {
   constexpr int M = MODEL == MK_GBM ? 1 : (MODEL == MK_HESTON ? 2 : K);
   constexpr bool KO = knock_out_kind<KIND>();
   constexpr bool UP = KIND == G_KO_GE || KIND == G_KO_GT;
@@ -95,7 +97,8 @@ __device__ __forceinline__ void fast_points(const PA& A, const KGroup gd, int k0
   FOR_SC s0v[sc] = scen_s0<K, SC>(A, sc, asset);
 #pragma unroll
   for (int s = 0; s < 4; ++s) {
-    advance<MODEL, K, SC>(A, k0 + s, &z[s * M], x, v);
+    if constexpr (ROW) advance_row<MODEL, K, SC>(*R, &z[s * M], x, v);
+    else advance<MODEL, K, SC>(A, k0 + s, &z[s * M], x, v);
     store_normals<MODEL, K, SC, DEBUG>(A, k0 + s, &z[s * M], live, local);
     if constexpr (KIND != G_NONE) {
       FOR_SC {
@@ -141,7 +144,12 @@ constexpr bool uniform_loop_built() {
 // groups of one level share their thresholds, so the extreme runs over the
 // whole prefix and is compared once at its end: a point crosses the level
 // exactly when the prefix's extreme does, and the flag only changes from
-// its initial value to 1, so the result equals a compare per group.
+// its initial value to 1, so the result equals a compare per group.  Under
+// GBM on a grid of equal steps (tab_uniform) groups 1 .. n-1 take the step
+// coefficients from VGPR copies of table row 1 (group 0 holds the zero row
+// of point 0 and reads the table): 5% on the seven-scenario kernel.  Heston
+// and basket steps measured 1-4% slower that way (more VGPRs), so they keep
+// reading the table.
 template <int KIND, int MODEL, int K, int SC, bool DEBUG, class PA>
 __device__ __forceinline__ void uniform_groups(const PA& A, const KGroup gd, int n, uint32_t p_lo, uint32_t p_hi,
                                                uint32_t flip, float (&x)[SC][K], float& v, float (&acc)[SC],
@@ -149,7 +157,24 @@ __device__ __forceinline__ void uniform_groups(const PA& A, const KGroup gd, int
   constexpr int M = MODEL == MK_GBM ? 1 : (MODEL == MK_HESTON ? 2 : K);
   float ext[SC];
   if constexpr (knock_out_kind<KIND>()) ko_init<KIND, SC>(ext);
-  for (int g = 0; g < n; ++g) {
+  int g_table = 0;  // groups before g_table have run
+  if constexpr (MODEL == MK_GBM) {
+    if (A.tab_uniform) {
+      {
+        float z[4 * M];
+        group_normals<M>(0, p_lo, p_hi, A.seed_lo, A.seed_hi, flip, z);
+        fast_points<KIND, MODEL, K, SC, DEBUG>(A, gd, 0, z, x, v, acc, comp, ext, live, local);
+      }
+      const StepRow<MODEL, K, SC> R = vgpr_row<MODEL, K, SC>(A, 1);
+      for (int g = 1; g < n; ++g) {
+        float z[4 * M];
+        group_normals<M>(g, p_lo, p_hi, A.seed_lo, A.seed_hi, flip, z);
+        fast_points<KIND, MODEL, K, SC, DEBUG, true>(A, gd, 4 * g, z, x, v, acc, comp, ext, live, local, &R);
+      }
+      g_table = n;
+    }
+  }
+  for (int g = g_table; g < n; ++g) {
     float z[4 * M];
     group_normals<M>(g, p_lo, p_hi, A.seed_lo, A.seed_hi, flip, z);
     fast_points<KIND, MODEL, K, SC, DEBUG>(A, gd, 4 * g, z, x, v, acc, comp, ext, live, local);
@@ -539,6 +564,25 @@ __global__ __launch_bounds__(256) void debug_copy_kernel(const uint4* __restrict
                                                          uint64_t n16) {
   for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n16; i += (uint64_t)gridDim.x * 256)
     dst[i] = src[i];
+}
+
+// VALU issue calibration: eight independent FMA chains per lane, so each
+// iteration issues eight VALU instructions and no chain waits on another;
+// unrolled by 8 so that the scalar loop counter and branch come once per 64
+// FMAs (iters is a multiple of 8).
+__global__ __launch_bounds__(256) void debug_valu_kernel(float* __restrict__ out, int iters, float b) {
+  float a[8];
+#pragma unroll
+  for (int i = 0; i < 8; ++i) a[i] = (float)(threadIdx.x + i) * 1e-3f;
+#pragma unroll 8
+  for (int it = 0; it < iters; ++it) {
+#pragma unroll
+    for (int i = 0; i < 8; ++i) a[i] = fmaf(a[i], b, 1e-7f);
+  }
+  float s = 0.0f;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) s += a[i];
+  out[(size_t)blockIdx.x * 256 + threadIdx.x] = s;
 }
 
 __global__ void debug_normals_kernel(const uint32_t* __restrict__ w, uint64_t n, float* __restrict__ out) {
@@ -1027,6 +1071,11 @@ hipError_t launch_debug_copy(const void* src, void* dst, uint64_t bytes, hipStre
   const uint64_t n16 = bytes / 16;
   if (n16 == 0) return hipSuccess;
   hipLaunchKernelGGL(debug_copy_kernel, dim3(8192), dim3(256), 0, s, (const uint4*)src, (uint4*)dst, n16);
+  return hipGetLastError();
+}
+
+hipError_t launch_debug_valu(float* out, int iters, hipStream_t s) {
+  hipLaunchKernelGGL(debug_valu_kernel, dim3(kValuProbeBlocks), dim3(256), 0, s, out, iters, 1.0000001f);
   return hipGetLastError();
 }
 

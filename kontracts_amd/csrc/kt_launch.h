@@ -37,6 +37,7 @@ struct PathArgs {
   int n_grid;
   int n_groups;
   int n_uniform;  // leading groups of one fast kind (uniform_group_prefix)
+  int tab_uniform;  // rows 1 .. n_grid - 1 of tab are identical (a grid of equal steps)
   int tab_stride;
   int heston_full_truncation;
   uint64_t n_paths;    // paths in this launch
@@ -187,6 +188,8 @@ int surface_occupancy_blocks_per_cu();
 hipError_t launch_debug_philox(const uint32_t* ctr, uint32_t k0, uint32_t k1, uint64_t n, uint32_t* out,
                                hipStream_t s);
 hipError_t launch_debug_copy(const void* src, void* dst, uint64_t bytes, hipStream_t s);
+constexpr int kValuProbeBlocks = 8192;
+hipError_t launch_debug_valu(float* out, int iters, hipStream_t s);
 hipError_t launch_debug_normals(const uint32_t* words, uint64_t n, float* out, hipStream_t s);
 
 }  // namespace kt

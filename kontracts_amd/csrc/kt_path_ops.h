@@ -170,6 +170,75 @@ __device__ __forceinline__ float heston_v_next(float v, float vp, float sv, floa
   return fmaxf(fmaf(d2 * sv, wv, fmaf(e0, vp, v) + d1), lo);
 }
 
+// A wave-uniform value copied into a VGPR.  On gfx950 an FP32 VALU
+// instruction that reads an SGPR operand issues at about half the rate of
+// one whose operands are all VGPRs or literals (a loop of independent FMAs
+// measured 3.8e13 against 6.7e13 lane-ops/s), so loop-invariant model
+// coefficients are held in VGPRs for the hot loops.
+__device__ __forceinline__ float vgpr_copy(float x) {
+  float r;
+  asm("v_mov_b32 %0, %1" : "=v"(r) : "s"(x));
+  return r;
+}
+
+// One grid point's step coefficients (a table row, the Cholesky factor,
+// rho, rho' and the variance floor) held in VGPRs.
+template <int MODEL, int K, int SC>
+struct StepRow {
+  static constexpr int N = MODEL == MK_HESTON ? 6 : 2 * (MODEL == MK_GBM ? SC : K);
+  static constexpr int NC = MODEL == MK_BASKET ? K * (K + 1) / 2 : 1;
+  float c[N];
+  float chol[NC];
+  float rho, rho_perp, lo;
+};
+
+template <int MODEL, int K, int SC, class PA>
+__device__ __forceinline__ StepRow<MODEL, K, SC> vgpr_row(const PA& A, int k) {
+  using R = StepRow<MODEL, K, SC>;
+  R r;
+  const size_t stride = MODEL == MK_HESTON ? kHestonStride : R::N;
+#pragma unroll
+  for (int i = 0; i < R::N; ++i) r.c[i] = vgpr_copy(ld_f32(A.tab, (size_t)k * stride + i));
+  if constexpr (MODEL == MK_BASKET) {
+#pragma unroll
+    for (int i = 0; i < K; ++i)
+#pragma unroll
+      for (int c = 0; c <= i; ++c) r.chol[i * (i + 1) / 2 + c] = vgpr_copy(A.chol[i * K + c]);
+  }
+  if constexpr (MODEL == MK_HESTON) {
+    r.rho = vgpr_copy(A.rho);
+    r.rho_perp = vgpr_copy(A.rho_perp);
+    r.lo = vgpr_copy(heston_floor(A));
+  }
+  return r;
+}
+
+// advance() with the coefficients from a StepRow: the same expressions, so
+// the same results.
+template <int MODEL, int K, int SC>
+__device__ __forceinline__ void advance_row(const StepRow<MODEL, K, SC>& R, const float* zk, float (&x)[SC][K],
+                                            float& v) {
+  if constexpr (MODEL == MK_GBM) {
+    FOR_SC {
+      if (xsrc<SC>(sc) == sc) x[sc][0] = fmaf(R.c[2 * sc + 1], zk[0], x[sc][0] + R.c[2 * sc]);
+    }
+  } else if constexpr (MODEL == MK_HESTON) {
+    const float z1 = zk[0], z2 = zk[1];
+    const float vp = fmaxf(v, 0.0f);
+    const float sv = __builtin_amdgcn_sqrtf(vp);
+    x[0][0] = fmaf(R.c[2] * sv, z1, fmaf(R.c[1], vp, x[0][0]) + R.c[0]);
+    v = heston_v_next(v, vp, sv, z1, z2, R.c[3], R.c[4], R.c[5], R.rho, R.rho_perp, R.lo);
+  } else {
+#pragma unroll
+    for (int i = 0; i < K; ++i) {
+      float wi = 0.0f;
+#pragma unroll
+      for (int c = 0; c <= i; ++c) wi = fmaf(R.chol[i * (i + 1) / 2 + c], zk[c], wi);
+      x[0][i] = fmaf(R.c[2 * i + 1], wi, x[0][i] + R.c[2 * i]);
+    }
+  }
+}
+
 template <int MODEL, int K, int SC, class PA>
 __device__ __forceinline__ void advance(const PA& A, int k, const float* zk, float (&x)[SC][K], float& v) {
   if constexpr (MODEL == MK_GBM) {

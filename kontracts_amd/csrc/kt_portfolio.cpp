@@ -180,6 +180,7 @@ void price_portfolio(kt_engine* E, const kt_job* jobs, uint64_t n_jobs, kt_price
       A.n_grid = (int)P.grid.size();
       A.n_groups = (int)P.groups.size();
       A.n_uniform = uniform_group_prefix(P.groups.data(), A.n_groups);
+      A.tab_uniform = uniform_table_rows(P.tab, P.grid.size(), (size_t)P.stride);
       A.tab_stride = P.stride;
       A.n_paths = J.cfg.n_paths;
       A.path_base = 0;

@@ -3,9 +3,12 @@
 Unit: lane-ops, one VALU instruction executed by one lane.  The peak is
 MI355X's published FP32 vector rate, 157.3 TFLOP/s = 78.6e12 FMA lane-ops/s
 (256 CUs x 128 FP32 lanes per clock x 2.4 GHz).  The kernels issue no packed
-FP32 instructions, so their instruction stream is also bounded by the
-non-packed issue rate, one wave64 VALU instruction per SIMD every 4 cycles
-(256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz = 39.3e12 lane-ops/s).
+FP32 instructions; a plain loop of independent non-packed FMAs
+(kt_debug_valu, `bench.py --valu-calibration`) measures the issue rate they
+can reach, and the bench reports the path kernel against both.  Not every
+instruction issues at the FMA rate: the 64-bit integer multiply-add of Philox
+and the transcendentals (log2, exp2, sqrt, reciprocal) take several issue
+cycles each, which the lane-op count does not weight.
 
 The counts below are the minimal instruction sequence of each step on
 gfx950 (the sequence the kernels' straight-line loops execute), not the
@@ -32,9 +35,6 @@ beside them.
 """
 
 PEAK_LANE_OPS = 256 * 128 * 2.4e9  # 78.6e12
-# Issue ceiling without packed FP32 (256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz):
-# the rate one VALU instruction per lane per cycle reaches.
-PEAK_NONPACKED_LANE_OPS = 256 * 64 * 2.4e9  # 39.3e12
 PEAK_FP32_TFLOPS = 157.3
 PHILOX_PER_WORD = 10.0
 TAIL_WAVE_PROBABILITY = 1.0 - (1.0 - 0.00337) ** 64
