@@ -52,7 +52,7 @@ struct kt_engine {
   DevBuf errp;
   bool errp_zero = false;
   int occupancy_portfolio = 0;  // cached occupancies (blocks per CU)
-  int occupancy_surface = 0;
+  int occupancy_surface[2] = {0, 0};  // Greek surface kernel with 8 and 16 registers
   int occupancy_adjoint[kt::APV_COUNT] = {0, 0, 0, 0, 0, 0};
   bool pathwise_tangent = false;  // Heston pathwise Greeks: forward tangents instead of the reverse sweep
   bool profiling = false;

@@ -151,7 +151,7 @@ void greek_surface(kt_engine* E, const char* json, const char* asset, const doub
   const uint64_t nb = (N + kBlock - 1) / kBlock;
   int ns = 0, nout = 0;
   const bool fast = !cfg->antithetic && !cfg->control_variate && N > 0 && nb <= (1u << 20) &&
-                    path_variant_exists(MK_GBM, 1, prog.n_slots, 1, false, kScenarios, &ns, &nout) && ns == 16;
+                    path_variant_exists(MK_GBM, 1, prog.n_slots, 1, false, kScenarios, &ns, &nout);
   if (!fast) {
     for (uint64_t c = 0; c < n_cells; ++c)
       greeks_gbm_crn(E, json, asset, spots[c / n_vols], vols[c % n_vols], cfg, h, &out[c]);
@@ -254,8 +254,9 @@ void greek_surface(kt_engine* E, const char* json, const char* asset, const doub
     E->prof_begin();
     int n_cu = 0;
     check_hip(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, E->device), "device attribute");
-    if (E->occupancy_surface <= 0) E->occupancy_surface = surface_occupancy_blocks_per_cu();
-    check_hip(launch_surface((const PathArgs*)dev, (int)nc, (int)nb, partials, stats, n_cu, E->occupancy_surface,
+    int& occ = E->occupancy_surface[ns == 8 ? 0 : 1];
+    if (occ <= 0) occ = surface_occupancy_blocks_per_cu(ns);
+    check_hip(launch_surface(ns, (const PathArgs*)dev, (int)nc, (int)nb, partials, stats, n_cu, occ,
                              E->stream),
               "surface launch");
     E->prof_end((double)N * (double)(grid.size() - 1) * kScenarios * (double)nc);

@@ -421,12 +421,22 @@ __global__ __launch_bounds__(kBlock) void path_kernel(const PathArgs A) {
 // its payoff and reduction epilogue; capped at 102 (5 waves) the epilogue
 // spills a few values and the path loop, which needs fewer, runs 3% faster
 // (measured 1.006 against 1.038 ms per launch at 4,194,304 x 252; 6 waves
-// cost 9% and 8 waves 29% through spills in the loop itself).
+// cost 9% and 8 waves 29% through spills in the loop itself).  Programs of
+// at most 8 registers use an 8-register file: the seven scenarios' files
+// live in scratch, and halving them cut the kernel by 4% (0.87 against
+// 0.907 ms on the same workload).
 template <>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))) void
 path_kernel<MK_GBM, 1, 16, 1, kScenarios, kScenarioStats, false, false>(const PathArgs A) {
   float r[kScenarios][16];
   path_block<MK_GBM, 1, 16, 1, kScenarios, kScenarioStats, false, false>(A, blockIdx.x, r);
+}
+
+template <>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))) void
+path_kernel<MK_GBM, 1, 8, 1, kScenarios, kScenarioStats, false, false>(const PathArgs A) {
+  float r[kScenarios][8];
+  path_block<MK_GBM, 1, 8, 1, kScenarios, kScenarioStats, false, false>(A, blockIdx.x, r);
 }
 
 // Block j reduces output j.  err (nullable): block 0 copies the launch's
@@ -569,15 +579,17 @@ __global__ __launch_bounds__(256) void debug_copy_kernel(const uint4* __restrict
 // VALU issue calibration: eight independent FMA chains per lane, so each
 // iteration issues eight VALU instructions and no chain waits on another;
 // unrolled by 8 so that the scalar loop counter and branch come once per 64
-// FMAs (iters is a multiple of 8).
+// FMAs (iters is a multiple of 8).  The multiplier is held in a VGPR: with
+// an SGPR operand the same loop issues at about half the rate.
 __global__ __launch_bounds__(256) void debug_valu_kernel(float* __restrict__ out, int iters, float b) {
   float a[8];
 #pragma unroll
   for (int i = 0; i < 8; ++i) a[i] = (float)(threadIdx.x + i) * 1e-3f;
+  const float bv = vgpr_copy(b);
 #pragma unroll 8
   for (int it = 0; it < iters; ++it) {
 #pragma unroll
-    for (int i = 0; i < 8; ++i) a[i] = fmaf(a[i], b, 1e-7f);
+    for (int i = 0; i < 8; ++i) a[i] = fmaf(a[i], bv, 1e-7f);
   }
   float s = 0.0f;
 #pragma unroll
@@ -916,9 +928,9 @@ bool path_variant_exists(int model, int k_assets, int n_slots, int n_outputs, bo
   if (model == MK_BASKET ? (k_assets < 2 || k_assets > kMaxAssets) : k_assets != 1) return false;
   if (anti && (model != MK_GBM || n_scen != 1 || debug)) return false;
   if (n_scen != 1) {
-    // CRN scenario kernel: one GBM contract, kScenarios scenarios, 16 registers.
+    // CRN scenario kernel: one GBM contract, kScenarios scenarios, 8 or 16 registers.
     if (model != MK_GBM || n_scen != kScenarios || n_outputs != 1 || n_slots > 16 || debug) return false;
-    *ns_chosen = 16;
+    *ns_chosen = n_slots <= 8 ? 8 : 16;
     *nout_chosen = 1;
     return true;
   }
@@ -942,6 +954,8 @@ hipError_t launch_path(int model, int k, int ns, int nout, bool debug, int n_sce
     if (anti) return hipErrorInvalidValue;
     if (model == MK_GBM && n_scen == kScenarios && ns == 16 && nout == 1 && !debug)
       return go<MK_GBM, 1, 16, 1, kScenarios, kScenarioStats, false>(a, n_blocks, s);
+    if (model == MK_GBM && n_scen == kScenarios && ns == 8 && nout == 1 && !debug)
+      return go<MK_GBM, 1, 8, 1, kScenarios, kScenarioStats, false>(a, n_blocks, s);
     return hipErrorInvalidValue;
   }
   switch (model) {
@@ -1036,19 +1050,25 @@ hipError_t launch_portfolio(const PortfolioArgs& q, const int* n_blocks, int n_j
   return hipGetLastError();
 }
 
-int surface_occupancy_blocks_per_cu() {
+int surface_occupancy_blocks_per_cu(int ns) {
   int n = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, surface_kernel<16>, kBlock, 0) != hipSuccess) return 4;
+  const hipError_t e = ns == 8 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, surface_kernel<8>, kBlock, 0)
+                               : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, surface_kernel<16>, kBlock, 0);
+  if (e != hipSuccess) return 4;
   return n > 0 ? n : 1;
 }
 
-hipError_t launch_surface(const PathArgs* cells, int n_cells, int n_blocks, const KStats* partials, KStats* out,
-                          int n_cu, int per_cu, hipStream_t s) {
-  if (n_cells <= 0 || n_blocks <= 0 || n_cu <= 0 || per_cu <= 0) return hipErrorInvalidValue;
+hipError_t launch_surface(int ns, const PathArgs* cells, int n_cells, int n_blocks, const KStats* partials,
+                          KStats* out, int n_cu, int per_cu, hipStream_t s) {
+  if (n_cells <= 0 || n_blocks <= 0 || n_cu <= 0 || per_cu <= 0 || (ns != 8 && ns != 16)) return hipErrorInvalidValue;
   const uint64_t total = (uint64_t)n_cells * (uint64_t)n_blocks;
   if (total > 0x7fffffffull || (uint64_t)n_cu * (uint64_t)per_cu > 0xffffffffull) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(surface_kernel<16>, dim3((unsigned)total), dim3(kBlock), 0, s, cells, (uint32_t)n_blocks,
-                     (uint32_t)total, (uint32_t)n_cu, (uint32_t)per_cu);
+  if (ns == 8)
+    hipLaunchKernelGGL(surface_kernel<8>, dim3((unsigned)total), dim3(kBlock), 0, s, cells, (uint32_t)n_blocks,
+                       (uint32_t)total, (uint32_t)n_cu, (uint32_t)per_cu);
+  else
+    hipLaunchKernelGGL(surface_kernel<16>, dim3((unsigned)total), dim3(kBlock), 0, s, cells, (uint32_t)n_blocks,
+                       (uint32_t)total, (uint32_t)n_cu, (uint32_t)per_cu);
   hipLaunchKernelGGL(surface_finalize_kernel, dim3(n_cells * kScenarioStats), dim3(256), 0, s, partials, n_blocks,
                      out);
   return hipGetLastError();

@@ -178,13 +178,13 @@ hipError_t launch_diff_stats(const double* a, const double* b, double scale, uin
 hipError_t launch_finalize(const KStats* partials, int n_blocks, int nout, int n_used, KStats* out,
                            hipStream_t s, unsigned* err = nullptr, unsigned* err_copy = nullptr);
 // Greek surface: cells[c] (device) is the seven-scenario launch of cell c
-// (16 registers), its partials [n_blocks][kScenarioStats] at
+// (ns = 8 or 16 registers), its partials [n_blocks][kScenarioStats] at
 // partials + c * n_blocks * kScenarioStats; out[c * kScenarioStats + j].
 // n_cu, per_cu: compute units and resident blocks per CU, for the
 // co-residency-aware workgroup map (never changes results).
-hipError_t launch_surface(const PathArgs* cells, int n_cells, int n_blocks, const KStats* partials, KStats* out,
-                          int n_cu, int per_cu, hipStream_t s);
-int surface_occupancy_blocks_per_cu();
+hipError_t launch_surface(int ns, const PathArgs* cells, int n_cells, int n_blocks, const KStats* partials,
+                          KStats* out, int n_cu, int per_cu, hipStream_t s);
+int surface_occupancy_blocks_per_cu(int ns);
 hipError_t launch_debug_philox(const uint32_t* ctr, uint32_t k0, uint32_t k1, uint64_t n, uint32_t* out,
                                hipStream_t s);
 hipError_t launch_debug_copy(const void* src, void* dst, uint64_t bytes, hipStream_t s);

@@ -301,8 +301,16 @@ def test_basket_perfect_correlation_is_single_asset():
 # ---------------------------------------------------------------------------
 # Greeks (src/greeks.rs): fused CRN scenario kernel
 # ---------------------------------------------------------------------------
-def test_greeks_scenario_kernel_equals_separate_pricings():
-    c = K.up_and_out_call("X", 100, 200, 1.0)
+def _lookback_knock_out():
+    # 13 registers: the scenario kernel's 16-register file (the plain up-and-out uses the 8-register one)
+    S = K.spot("X")
+    e = (K.running_max(S) - 95.0).clip(0.0) + (105.0 - K.running_min(S)).clip(0.0)
+    return K.up_and_out_call("X", 100.0, 200.0, 1.0) + K.until(S >= 230.0, K.scale(e, K.one("USD")) @ K.at(1.0))
+
+
+@pytest.mark.parametrize("which", ["up_and_out_8_registers", "lookback_knock_out_16_registers"])
+def test_greeks_scenario_kernel_equals_separate_pricings(which):
+    c = K.up_and_out_call("X", 100, 200, 1.0) if which.startswith("up_and_out") else _lookback_knock_out()
     s0, sg, r, n, spy = 180.0, 0.25, 0.05, 65536, 252
     hs, hv, hr = 1.8, 0.01, 1e-4
     g = c.greeks(K.GBM(s0, sg, r, asset="X"), n_paths=n, seed=7, steps_per_year=spy, bumps=(hs, hv, hr))

@@ -89,3 +89,13 @@ def test_surface_errors():
     assert e.value.kind == "UnknownAsset"
     z = K.greek_surface(CALL, "AAPL", [100.0, 110.0], [0.2], n_paths=0)
     assert np.all(z.price == 0.0)
+
+
+def test_surface_sixteen_register_programs():
+    # 13 registers: the surface kernel's 16-register file (the contracts above use the 8-register one)
+    S = K.spot("X")
+    e = (K.running_max(S) - 95.0).clip(0.0) + (105.0 - K.running_min(S)).clip(0.0)
+    c = K.up_and_out_call("X", 100.0, 200.0, 1.0) + K.until(S >= 230.0, K.scale(e, K.one("USD")) @ K.at(1.0))
+    surf = K.greek_surface(c, "X", [95.0, 110.0], [0.2, 0.3], n_paths=8192, seed=9, steps_per_year=52, rate=0.03,
+                           bumps=(1.0, 0.01, 1e-4))
+    assert_cells_equal_greeks(surf, c, "X", 8192, 9, 52, 0.03, (1.0, 0.01, 1e-4))
