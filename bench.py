@@ -161,34 +161,46 @@ def main():
 
     extras = {}
     if args.extras:
+
+        def timed(fn, reps=5):
+            """Median wall time and median path-kernel time of reps calls (after one warm call)."""
+            out = fn()
+            walls, kms = [], []
+            for _ in range(reps):
+                eng.set_profiling(True)
+                eng.kernel_time(reset=True)
+                t = time.perf_counter()
+                out = fn()
+                walls.append(time.perf_counter() - t)
+                kms.append(eng.kernel_time(reset=True)[0])
+                eng.set_profiling(False)
+            return out, statistics.median(walls), statistics.median(kms)
+
+        extras["timing"] = "single pricings and Greeks: median of 5 calls each, after a warm call"
         grid = [i / 252 for i in range(253)]
         c1 = K.european_call("X", 100.0, 1.0)
         m1 = K.GBM(100.0, 0.2, 0.05, asset="X")
-        c1.price(m1, n_paths=1_000_000, grid=grid)
-        t = time.perf_counter()
-        r1 = c1.price(m1, n_paths=1_000_000, grid=grid)
-        extras["config1_european_gbm_path_steps_per_s"] = 1_000_000 * 252 / (time.perf_counter() - t)
+        r1, t1, k1 = timed(lambda: c1.price(m1, n_paths=1_000_000, grid=grid))
+        extras["config1_european_gbm_path_steps_per_s"] = 1_000_000 * 252 / t1
+        extras["config1_european_gbm_kernel_path_steps_per_s"] = 1_000_000 * 252 / (k1 / 1e3)
         extras["config1_price"] = r1.price
         h = K.heston(100, 0.04, 2.0, 0.04, 0.3, -0.5, 0.05, asset="X")
-        c1.price(h, n_paths=2_097_152, steps_per_year=365)
-        t = time.perf_counter()
-        r3 = c1.price(h, n_paths=2_097_152, steps_per_year=365)
-        extras["config3_heston_forward_path_steps_per_s"] = 2_097_152 * 365 / (time.perf_counter() - t)
+        r3, t3, k3 = timed(lambda: c1.price(h, n_paths=2_097_152, steps_per_year=365))
+        extras["config3_heston_forward_path_steps_per_s"] = 2_097_152 * 365 / t3
+        extras["config3_heston_forward_kernel_ms"] = k3
         extras["config3_price"] = r3.price
         fs = [K.GbmFactor(f"A{i}", 100.0, 0.05, 0.15 + 0.05 * i) for i in range(5)]
         b5 = K.correlated_gbm(fs, [[1.0 if i == j else 0.3 for j in range(5)] for i in range(5)], 0.05)
         avg5 = (K.spot("A0") + K.spot("A1") + K.spot("A2") + K.spot("A3") + K.spot("A4")) / 5.0
         cb = K.scale((avg5 - 100.0).clip(0.0), K.one("USD")) @ K.at(1.0)
-        cb.price(b5, n_paths=1 << 20, grid=grid)
-        t = time.perf_counter()
-        rb = cb.price(b5, n_paths=1 << 20, grid=grid)
-        extras["basket5_path_steps_per_s"] = (1 << 20) * 252 / (time.perf_counter() - t)
+        rb, tb, kb = timed(lambda: cb.price(b5, n_paths=1 << 20, grid=grid))
+        extras["basket5_path_steps_per_s"] = (1 << 20) * 252 / tb
+        extras["basket5_kernel_ms"] = kb
         extras["basket5_price"] = rb.price
         ca = K.scale((K.average(K.spot("X")) - 100.0).clip(0.0), K.one("USD")) @ K.at(1.0)
-        ca.price(m1, n_paths=1 << 20, steps_per_year=252)
-        t = time.perf_counter()
-        ra = ca.price(m1, n_paths=1 << 20, steps_per_year=252)
-        extras["asian_daily_path_steps_per_s"] = (1 << 20) * 252 / (time.perf_counter() - t)
+        ra, ta, ka = timed(lambda: ca.price(m1, n_paths=1 << 20, steps_per_year=252))
+        extras["asian_daily_path_steps_per_s"] = (1 << 20) * 252 / ta
+        extras["asian_daily_kernel_ms"] = ka
         extras["asian_price"] = ra.price
         # American put by LSM: s0=36, K=40, sigma=0.2, r=0.06, T=1, 50 dates,
         # 2^20 paths stored in HBM.  Algorithmic HBM bytes: the paths kernel
@@ -222,25 +234,10 @@ def main():
         n3 = 2_097_152
         hb = [1.0, 1e-3, 1e-2, 1e-3, 1e-2, 1e-2]
 
-        def timed(fn, reps=5):
-            """Median wall time and median path-kernel time of reps calls (after one warm call)."""
-            out = fn()
-            walls, kms = [], []
-            for _ in range(reps):
-                eng.set_profiling(True)
-                eng.kernel_time(reset=True)
-                t = time.perf_counter()
-                out = fn()
-                walls.append(time.perf_counter() - t)
-                kms.append(eng.kernel_time(reset=True)[0])
-                eng.set_profiling(False)
-            return out, statistics.median(walls), statistics.median(kms)
-
         gA, tA, kA_ms = timed(lambda: c1.pathwise_greeks(hm, n_paths=n3, steps_per_year=365))
         _, tB, kB_ms = timed(lambda: c1.bump_greeks(hm, bumps=hb, n_paths=n3, steps_per_year=365, std_errors=False))
         gB = c1.bump_greeks(hm, bumps=hb, n_paths=n3, steps_per_year=365)
         eq = {nm: (tB / tA) * (sb / sa) ** 2 for nm, sa, sb in zip(gA.names, gA.std_errors, gB.std_errors)}
-        extras["config3_timing"] = "median of 5 calls each, after a warm call"
         extras["config3_pathwise_ms"] = tA * 1e3
         extras["config3_pathwise_kernel_ms"] = kA_ms
         extras["config3_bump13_ms"] = tB * 1e3
