@@ -21,6 +21,8 @@
 // * GBM and correlated GBM: d x_{p+1} / d x_p = 1, so the adjoint sweep is a
 //   running sum and collapses into the forward pass: it carries d x/d sigma and
 //   d x/d mu per asset (two FMAs per step) and each accumulator's derivative.
+#include <type_traits>
+
 #include "kt_path_ops.h"
 
 namespace kt {
@@ -561,92 +563,120 @@ __device__ __forceinline__ void adjoint_heston_block(const AA& A, uint32_t blk, 
   // that its latency hides behind the current segment's work
   float x_ck = FEEDS && live ? ck[(2 * (size_t)(n_seg - 1)) * cs + lane] : 0.0f;
   float v_ck = live ? ck[(2 * (size_t)(n_seg - 1) + 1) * cs + lane] : 0.0f;
-  for (int c = n_seg - 1; c >= 0; --c) {
-    // recompute segment c (points SEG*c .. SEG*c + SEG - 1) from its checkpoint
-    float xs[1][1] = {{x_ck}};
-    float vs = v_ck;
-    if (c > 0) {
-      if constexpr (FEEDS) x_ck = live ? ck[(2 * (size_t)(c - 1)) * cs + lane] : 0.0f;
-      v_ck = live ? ck[(2 * (size_t)(c - 1) + 1) * cs + lane] : 0.0f;
+  // The recompute and the sweep, in two instances: U (a grid of equal steps
+  // and no running accumulators) takes the step's table row and the reverse
+  // row, shared by points 1 .. n, from VGPR copies (FP32 instructions with
+  // SGPR operands issue at about half rate: 5% on config 3), the other reads
+  // them per point.  Both evaluate the same expressions on the same values.
+  auto sweep = [&](auto urow) {
+    constexpr bool U = decltype(urow)::value && !FEEDS;
+    float u_rev[7], u_e0 = 0.0f, u_d1 = 0.0f, u_d2 = 0.0f, u_rho = 0.0f, u_rhop = 0.0f, u_lo = 0.0f;
+    if constexpr (U) {
+#pragma unroll
+      for (int q = 0; q < 7; ++q) u_rev[q] = vgpr_copy(ld_f32(A.rev, 8 + q));
+      u_e0 = vgpr_copy(ld_f32(P.tab, kHestonStride + 3));
+      u_d1 = vgpr_copy(ld_f32(P.tab, kHestonStride + 4));
+      u_d2 = vgpr_copy(ld_f32(P.tab, kHestonStride + 5));
+      u_rho = vgpr_copy(P.rho);
+      u_rhop = vgpr_copy(P.rho_perp);
+      u_lo = vgpr_copy(heston_floor(P));
     }
-    // per point: sqrt(max(v, 0)) before the step (the sweep needs it and its
-    // square, not v), and the two normals
-    float svs[SEG], z1s[SEG], z2s[SEG];
-#pragma unroll
-    for (int gq = 0; gq < SEG / 4; ++gq) {
-      const int g = (SEG / 4) * c + gq;
-      float z[8];
-#pragma unroll
-      for (int bk = 0; bk < 2; ++bk) {
-        const uint4 q = philox4x32_10((uint32_t)(2 * g + bk), p_lo, p_hi, 0u, P.seed_lo, P.seed_hi);
-        z[4 * bk + 0] = normal_from_word(q.x);
-        z[4 * bk + 1] = normal_from_word(q.y);
-        z[4 * bk + 2] = normal_from_word(q.z);
-        z[4 * bk + 3] = normal_from_word(q.w);
+    for (int c = n_seg - 1; c >= 0; --c) {
+      // recompute segment c (points SEG*c .. SEG*c + SEG - 1) from its checkpoint
+      float xs[1][1] = {{x_ck}};
+      float vs = v_ck;
+      if (c > 0) {
+        if constexpr (FEEDS) x_ck = live ? ck[(2 * (size_t)(c - 1)) * cs + lane] : 0.0f;
+        v_ck = live ? ck[(2 * (size_t)(c - 1) + 1) * cs + lane] : 0.0f;
       }
+      // per point: sqrt(max(v, 0)) before the step (the sweep needs it and its
+      // square, not v), and the two normals
+      float svs[SEG], z1s[SEG], z2s[SEG];
 #pragma unroll
-      for (int s = 0; s < 4; ++s) {
-        const int i = 4 * gq + s;
-        svs[i] = __builtin_amdgcn_sqrtf(fmaxf(vs, 0.0f));  // the step's own sqrt (same expression)
-        z1s[i] = z[2 * s];
-        z2s[i] = z[2 * s + 1];
-        if (4 * g + s <= n) {
-          if constexpr (FEEDS) advance<MK_HESTON, 1, 1>(P, 4 * g + s, &z[2 * s], xs, vs);
-          else vs = heston_v_step(P, 4 * g + s, z[2 * s], z[2 * s + 1], vs);
+      for (int gq = 0; gq < SEG / 4; ++gq) {
+        const int g = (SEG / 4) * c + gq;
+        float z[8];
+#pragma unroll
+        for (int bk = 0; bk < 2; ++bk) {
+          const uint4 q = philox4x32_10((uint32_t)(2 * g + bk), p_lo, p_hi, 0u, P.seed_lo, P.seed_hi);
+          z[4 * bk + 0] = normal_from_word(q.x);
+          z[4 * bk + 1] = normal_from_word(q.y);
+          z[4 * bk + 2] = normal_from_word(q.z);
+          z[4 * bk + 3] = normal_from_word(q.w);
         }
-      }
-    }
-    // reverse sweep through the segment; xs, vs hold the state after its last point
-    float sv_next = __builtin_amdgcn_sqrtf(fmaxf(vs, 0.0f));  // > 0 exactly when v after the step is
 #pragma unroll
-    for (int i = SEG - 1; i >= 0; --i) {
-      const int p = SEG * c + i;
-      if (p <= n) {
-        if constexpr (FEEDS) {
-          float seed = p == n ? ds_n : 0.0f;
-          const uint32_t fm = ld_u32(A.feed, (size_t)p);
-          if (fm) {
-#pragma unroll
-            for (int q = 0; q < kAdjMaxAcc; ++q)
-              if ((fm >> q) & 1u) seed += (A.acc_kind[q] == OP_AVG || argstep[q] == p) ? abar[q] : 0.0f;
+        for (int s = 0; s < 4; ++s) {
+          const int i = 4 * gq + s;
+          svs[i] = __builtin_amdgcn_sqrtf(fmaxf(vs, 0.0f));  // the step's own sqrt (same expression)
+          z1s[i] = z[2 * s];
+          z2s[i] = z[2 * s + 1];
+          if (4 * g + s <= n) {
+            if constexpr (FEEDS) advance<MK_HESTON, 1, 1>(P, 4 * g + s, &z[2 * s], xs, vs);
+            else if constexpr (U) {
+              // point 0 has the zero row (no step); the others share row 1
+              if (4 * g + s == 0) continue;
+              const float vp0 = fmaxf(vs, 0.0f);
+              vs = heston_v_next(vs, vp0, __builtin_amdgcn_sqrtf(vp0), z[2 * s], z[2 * s + 1], u_e0, u_d1, u_d2, u_rho,
+                                 u_rhop, u_lo);
+            } else vs = heston_v_step(P, 4 * g + s, z[2 * s], z[2 * s + 1], vs);
           }
-          const float g = exp2_fast(xs[0][0]), S = s0 * g;  // S / s0 = 2^x
-          xb = fmaf(seed * S, kLn2f, xb);
-          g_s0 = fmaf(seed, g, g_s0);
         }
-        if (p >= 1) {
-          // one 32-byte row per point: a single scalar load
-          const float t1 = ld_f32(A.rev, 8 * (size_t)p + 0);
-          const float t2 = ld_f32(A.rev, 8 * (size_t)p + 1);
-          const float t3 = ld_f32(A.rev, 8 * (size_t)p + 2);
-          const float t5 = ld_f32(A.rev, 8 * (size_t)p + 3);
-          const float dt = ld_f32(A.rev, 8 * (size_t)p + 4);
-          const float sq = ld_f32(A.rev, 8 * (size_t)p + 5);
-          const float kdt = ld_f32(A.rev, 8 * (size_t)p + 6);
-          const float sv = svs[i], z1 = z1s[i], z2 = z2s[i];
-          const float vp = sv * sv;  // max(v, 0) to rounding
-          const float w = fmaf(rho, z1, rho_perp * z2);
-          // v_new = floor(v + e0 vp + d1 + d2 sv w); x_new = x + c0 + c1 vp + c2 sv z1
-          const float vnb = (ft || sv_next > 0.0f) ? vbar : 0.0f;
-          float vpb = fmaf(xb, t1, vnb * t3);
-          const float svb = fmaf(xb * t2, z1, vnb * t5 * w);
-          g_k = fmaf(vnb, dt * (theta - vp), g_k);
-          g_th = fmaf(vnb, kdt, g_th);
-          g_sv = fmaf(vnb, sq * sv * w, g_sv);
-          g_rho = fmaf(vnb * t5 * sv, fmaf(-rho_ratio, z2, z1), g_rho);
-          // d sqrt(v)/dv = 0.5 / sqrt(v) through the hardware reciprocal (one
-          // instruction instead of an IEEE division)
-          vpb = sv > 0.0f ? fmaf(svb, 0.5f * __builtin_amdgcn_rcpf(sv), vpb) : vpb;
-          vbar = vnb + ((!ft || sv > 0.0f) ? vpb : 0.0f);
+      }
+      // reverse sweep through the segment; xs, vs hold the state after its last point
+      float sv_next = __builtin_amdgcn_sqrtf(fmaxf(vs, 0.0f));  // > 0 exactly when v after the step is
+#pragma unroll
+      for (int i = SEG - 1; i >= 0; --i) {
+        const int p = SEG * c + i;
+        if (p <= n) {
           if constexpr (FEEDS) {
-            const float t0 = ld_f32(P.tab, (size_t)kHestonStride * p + 0);
-            xs[0][0] = xs[0][0] - (fmaf(t1, vp, t0) + t2 * sv * z1);
+            float seed = p == n ? ds_n : 0.0f;
+            const uint32_t fm = ld_u32(A.feed, (size_t)p);
+            if (fm) {
+#pragma unroll
+              for (int q = 0; q < kAdjMaxAcc; ++q)
+                if ((fm >> q) & 1u) seed += (A.acc_kind[q] == OP_AVG || argstep[q] == p) ? abar[q] : 0.0f;
+            }
+            const float g = exp2_fast(xs[0][0]), S = s0 * g;  // S / s0 = 2^x
+            xb = fmaf(seed * S, kLn2f, xb);
+            g_s0 = fmaf(seed, g, g_s0);
           }
-          sv_next = sv;
+          if (p >= 1) {
+            // one 32-byte row per point: a single scalar load
+            const float t1 = U ? u_rev[0] : ld_f32(A.rev, 8 * (size_t)p + 0);
+            const float t2 = U ? u_rev[1] : ld_f32(A.rev, 8 * (size_t)p + 1);
+            const float t3 = U ? u_rev[2] : ld_f32(A.rev, 8 * (size_t)p + 2);
+            const float t5 = U ? u_rev[3] : ld_f32(A.rev, 8 * (size_t)p + 3);
+            const float dt = U ? u_rev[4] : ld_f32(A.rev, 8 * (size_t)p + 4);
+            const float sq = U ? u_rev[5] : ld_f32(A.rev, 8 * (size_t)p + 5);
+            const float kdt = U ? u_rev[6] : ld_f32(A.rev, 8 * (size_t)p + 6);
+            const float sv = svs[i], z1 = z1s[i], z2 = z2s[i];
+            const float vp = sv * sv;  // max(v, 0) to rounding
+            const float w = fmaf(rho, z1, rho_perp * z2);
+            // v_new = floor(v + e0 vp + d1 + d2 sv w); x_new = x + c0 + c1 vp + c2 sv z1
+            const float vnb = (ft || sv_next > 0.0f) ? vbar : 0.0f;
+            float vpb = fmaf(xb, t1, vnb * t3);
+            const float svb = fmaf(xb * t2, z1, vnb * t5 * w);
+            g_k = fmaf(vnb, dt * (theta - vp), g_k);
+            g_th = fmaf(vnb, kdt, g_th);
+            g_sv = fmaf(vnb, sq * sv * w, g_sv);
+            g_rho = fmaf(vnb * t5 * sv, fmaf(-rho_ratio, z2, z1), g_rho);
+            // d sqrt(v)/dv = 0.5 / sqrt(v) through the hardware reciprocal (one
+            // instruction instead of an IEEE division)
+            vpb = sv > 0.0f ? fmaf(svb, 0.5f * __builtin_amdgcn_rcpf(sv), vpb) : vpb;
+            vbar = vnb + ((!ft || sv > 0.0f) ? vpb : 0.0f);
+            if constexpr (FEEDS) {
+              const float t0 = ld_f32(P.tab, (size_t)kHestonStride * p + 0);
+              xs[0][0] = xs[0][0] - (fmaf(t1, vp, t0) + t2 * sv * z1);
+            }
+            sv_next = sv;
+          }
         }
       }
     }
-  }
+  };
+  if (P.tab_uniform) sweep(std::integral_constant<bool, true>{});
+  else sweep(std::integral_constant<bool, false>{});
+
   double vals[NOUT] = {pv[0][0], (double)g_s0, (double)vbar, (double)g_k, (double)g_th, (double)g_sv, (double)g_rho};
   block_stats_out<NOUT>(vals, live, P.partials, blk);
 }

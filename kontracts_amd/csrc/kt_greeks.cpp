@@ -415,6 +415,9 @@ void fill_adjoint_args(AdjArgs& A, const AdjJobHost& H, const Program& prog, con
                     ? std::min(uniform_group_prefix(prog.groups.data(), (int)prog.groups.size()), P.n_groups)
                     : 0;
   P.tab_stride = H.stride;
+  // the Heston sweep keeps the shared rows in registers when every row is equal
+  P.tab_uniform = uniform_table_rows(H.tab, grid.size(), (size_t)H.stride) &&
+                  uniform_table_rows(H.rev, grid.size(), 8);
   fill_model_args(P, md, seed);
   A.adj_tab = (const float*)dev[7];
   A.feed = (const uint32_t*)dev[8];

@@ -208,14 +208,19 @@ void price_portfolio(kt_engine* E, const kt_job* jobs, uint64_t n_jobs, kt_price
     Q.jobs = (const PortfolioJob*)dev;
     Q.items = (const uint32_t*)(dev + jobs_bytes);
     Q.n_items = (uint32_t)n_items;
-    Q.chunk = std::max(1, E->pf_chunk);
     Q.counter = counter;
     const int* nblk_dev = (const int*)(dev + jobs_bytes + items_bytes);
     int n_cu = 0;
     check_hip(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, E->device), "device attribute");
     if (E->occupancy_portfolio <= 0) E->occupancy_portfolio = portfolio_occupancy_blocks_per_cu();
     const int bpc = E->pf_blocks_per_cu > 0 ? E->pf_blocks_per_cu : E->occupancy_portfolio;
-    const int grid = (int)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)n_cu * bpc, n_items));
+    const uint64_t slots = (uint64_t)n_cu * bpc;
+    // A queue chunk of c items saves atomics only where items outnumber the
+    // resident workgroups c times over; on a smaller book it would serialise
+    // items on a few workgroups.  Which workgroup takes an item never changes
+    // a result.
+    Q.chunk = (int)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)std::max(1, E->pf_chunk), n_items / slots));
+    const int grid = (int)std::max<uint64_t>(1, std::min<uint64_t>(slots, (n_items + Q.chunk - 1) / Q.chunk));
 
     // Replay the captured launch sequence while the book keeps its shape.
     const std::vector<uint64_t> key = {(uint64_t)dev, (uint64_t)partials, (uint64_t)stats, (uint64_t)errw,

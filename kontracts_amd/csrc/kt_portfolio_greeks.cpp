@@ -164,13 +164,16 @@ void price_portfolio_greeks(kt_engine* E, const kt_job* jobs, uint64_t n_jobs, k
     // Heston checkpoint slots: one per resident workgroup of the larger launch.
     int n_cu = 0;
     check_hip(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, E->device), "device attribute");
-    int grid_v[APV_COUNT] = {0, 0, 0, 0, 0, 0};
+    int grid_v[APV_COUNT] = {0, 0, 0, 0, 0, 0}, chunk_v[APV_COUNT] = {1, 1, 1, 1, 1, 1};
     for (int v = 0; v < APV_COUNT; ++v) {
       if (vitems[v].empty()) continue;
       if (E->occupancy_adjoint[v] <= 0) E->occupancy_adjoint[v] = portfolio_adjoint_occupancy(v);
       const int bpc = E->pf_blocks_per_cu > 0 ? std::min(E->pf_blocks_per_cu, E->occupancy_adjoint[v])
                                               : E->occupancy_adjoint[v];
-      grid_v[v] = (int)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)n_cu * bpc, vitems[v].size() / 2));
+      // as for prices: no chunk larger than the items per resident workgroup
+      const uint64_t slots = (uint64_t)n_cu * bpc, n_it = vitems[v].size() / 2;
+      chunk_v[v] = (int)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)std::max(1, E->pf_chunk), n_it / slots));
+      grid_v[v] = (int)std::max<uint64_t>(1, std::min<uint64_t>(slots, (n_it + chunk_v[v] - 1) / chunk_v[v]));
     }
     uint64_t ck_floats = 0;
     for (const AdjJob& a : adj)
@@ -218,7 +221,7 @@ void price_portfolio_greeks(kt_engine* E, const kt_job* jobs, uint64_t n_jobs, k
       Q.jobs = (const AdjArgs*)dev;
       Q.items = (const uint32_t*)(dev + items_off[v]);
       Q.n_items = (uint32_t)(vitems[v].size() / 2);
-      Q.chunk = std::max(1, E->pf_chunk);
+      Q.chunk = chunk_v[v];
       Q.counter = counters + v;
       Q.ck = (v == APV_HESTON || v == APV_HESTON_FEEDS) ? ck : nullptr;
       Q.ck_floats = ck_floats;
