@@ -68,6 +68,21 @@ def test_inverse_normal_error_bound(eng):
     np.testing.assert_allclose(zm, -z[:100000], atol=1e-6)
 
 
+def test_valu_issue_calibration(eng):
+    # the FMA loop the roofline is compared with: a positive rate below the
+    # published 78.6e12 FMA lane-ops/s, and iteration counts it cannot unroll
+    # by 8 are rejected
+    ms, ops = C.c_double(), C.c_double()
+    N.check(N.lib().kt_debug_valu(eng.handle, 512, 3, C.byref(ms), C.byref(ops)), eng.handle)
+    assert ms.value > 0.0
+    assert 1e12 < ops.value < 78.7e12
+    assert ops.value == pytest.approx(8192 * 256 * 512 * 8 / (ms.value * 1e-3), rel=1e-9)
+    for bad in ((12, 1), (0, 1), (64, 0)):
+        with pytest.raises(N.KontractError) as ei:
+            N.check(N.lib().kt_debug_valu(eng.handle, bad[0], bad[1], C.byref(ms), C.byref(ops)), eng.handle)
+        assert ei.value.kind == "InvalidArgument"
+
+
 def debug_paths(eng, contract, model, n_paths, seed, spy=50, grid=None):
     m = _model_struct(model)
     cfg = _cfg(n_paths, seed, spy, model.rate, grid)
