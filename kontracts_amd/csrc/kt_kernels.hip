@@ -3,8 +3,9 @@
 //
 // Work layout: one path per lane, 256 lanes (4 waves) per workgroup, paths
 // contiguous across the grid.  A path's state (log-spot per asset, Heston
-// variance, the op program's register file, the present values) never leaves
-// VGPRs.  Normals come from Philox4x32-10 keyed by the seed with counter
+// variance, the present values) stays in VGPRs; the op program's register
+// file does too for one scenario, and lives in per-lane scratch for the seven
+// CRN scenarios.  Normals come from Philox4x32-10 keyed by the seed with counter
 // (block, path_lo, path_hi, 0): normal j = k*M + c of grid point k, component
 // c, is word j%4 of block j/4, so the stream does not depend on the launch
 // shape.  Each group of 4 grid points consumes exactly M Philox blocks.
@@ -12,8 +13,11 @@
 // Dispatch is per group of 4 grid points (KGroup): groups whose points all
 // run one monitor (knock-out compare, running sum, running max/min) or none
 // execute as unrolled straight-line code; the other groups interpret each
-// point's op list.  All tables are read through the constant address space,
-// so their loads are scalar (s_load) and their values wave-uniform.
+// point's op list.  The leading run of groups of one such kind (n_uniform)
+// runs in a loop of its own before the dispatching loop takes over.  All
+// tables are read through the constant address space, so their loads are
+// scalar (s_load) and their values wave-uniform; the GBM and basket runs on
+// grids of equal steps copy the shared step row into VGPRs instead.
 //
 // Scenarios (SC > 1, GBM only): SC repricings of one contract with different
 // model parameters share every normal of a path (common random numbers); each
