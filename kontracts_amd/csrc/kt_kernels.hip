@@ -4,8 +4,9 @@
 // Work layout: one path per lane, 256 lanes (4 waves) per workgroup, paths
 // contiguous across the grid.  A path's state (log-spot per asset, Heston
 // variance, the present values) stays in VGPRs; the op program's register
-// file does too for one scenario, and lives in per-lane scratch for the seven
-// CRN scenarios.  Normals come from Philox4x32-10 keyed by the seed with counter
+// file does too for one scenario and 16 registers, and lives (partly) in
+// per-lane scratch for 32 registers and for the seven CRN scenarios.  Normals
+// come from Philox4x32-10 keyed by the seed with counter
 // (block, path_lo, path_hi, 0): normal j = k*M + c of grid point k, component
 // c, is word j%4 of block j/4, so the stream does not depend on the launch
 // shape.  Each group of 4 grid points consumes exactly M Philox blocks.
