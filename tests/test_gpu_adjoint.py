@@ -236,3 +236,28 @@ def test_heston_tangent_mode_equals_reverse_sweep():
         eng.set_option("pathwise_mode", 0)
     with pytest.raises(N.KontractError):
         eng.set_option("pathwise_mode", 2)
+
+
+def test_heston_sweep_on_equal_and_unequal_steps():
+    """The reverse sweep reads its step rows from registers on a grid of equal
+    steps and from the tables otherwise; on either grid it must agree with the
+    forward-tangent mode (same paths, FP32 rounding only), and on the equal
+    grid the two paths give the same price as the plain pricer."""
+    eng = N.engine(0)
+    m = K.heston(100.0, 0.04, 2.0, 0.04, 0.3, -0.5, 0.05, asset="X")
+    c = call("X", 100.0)
+    equal = [i / 96 for i in range(97)]
+    unequal = sorted(set([i / 96 for i in range(97)] + [0.013, 0.377, 0.5011, 0.9]))
+    try:
+        for grid in (equal, unequal):
+            eng.set_option("pathwise_mode", 0)
+            a = c.pathwise_greeks(m, n_paths=1 << 15, seed=13, grid=grid)
+            eng.set_option("pathwise_mode", 1)
+            b = c.pathwise_greeks(m, n_paths=1 << 15, seed=13, grid=grid)
+            assert a.price == b.price
+            for name, x, y, se in zip(a.names, a.values, b.values, a.std_errors):
+                assert abs(x - y) <= 1e-5 * abs(x) + 1e-3 * se, (len(grid), name, x, y)
+            p = c.price(m, n_paths=1 << 15, seed=13, grid=grid)
+            assert a.price == pytest.approx(p.price, rel=1e-6)
+    finally:
+        eng.set_option("pathwise_mode", 0)
