@@ -145,7 +145,8 @@ def main():
             "kernel": "path_kernel<GBM, 7 CRN scenarios>", "avg_launch_ms": round(avg_launch_s * 1e3, 4),
             "launches": k_launches, "lane_ops_per_path_step": lane_ops,
             "path_steps_per_launch": path_steps_per_launch,
-            "peak_basis": "FP32 vector 157.3 TFLOP/s = 78.6e12 FMA lane-ops/s (packed); no MFMA: no step is a "
+            "peak_basis": "FP32 vector 157.3 TFLOP/s = 78.6e12 FMA lane-ops/s; lane-ops are the minimal gfx950 "
+                          "instruction sequence per path-step (kontracts_amd/roofline.py); no MFMA: no step is a "
                           "dense contraction, the work is Philox integer ops, inverse normals and FMAs",
             "fma_issue_measured": round(fma["fma_lane_ops_per_s"] / 1e12, 3),
             "frac_of_measured_fma_issue": round(achieved * 1e12 / fma["fma_lane_ops_per_s"], 4)}
