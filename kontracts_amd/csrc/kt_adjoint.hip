@@ -674,8 +674,13 @@ __device__ __forceinline__ void adjoint_heston_block(const AA& A, uint32_t blk, 
       }
     }
   };
-  if (P.tab_uniform) sweep(std::integral_constant<bool, true>{});
-  else sweep(std::integral_constant<bool, false>{});
+  // (one instance only with running accumulators: U needs none)
+  if constexpr (!FEEDS) {
+    if (P.tab_uniform) sweep(std::integral_constant<bool, true>{});
+    else sweep(std::integral_constant<bool, false>{});
+  } else {
+    sweep(std::integral_constant<bool, false>{});
+  }
 
   double vals[NOUT] = {pv[0][0], (double)g_s0, (double)vbar, (double)g_k, (double)g_th, (double)g_sv, (double)g_rho};
   block_stats_out<NOUT>(vals, live, P.partials, blk);
