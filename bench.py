@@ -65,6 +65,17 @@ def valu_probe(N, eng, reps=20, iters=4096):
             "fma_lane_ops_per_s": ops.value}
 
 
+def rng_probe(N, eng, normals, reps=20, iters=256):
+    """Rate of the normal generator alone: Philox blocks with the path kernels'
+    counter layout, folded as raw words or as their FP32 inverse normals."""
+    import ctypes as C
+    ms, words = C.c_double(), C.c_double()
+    N.check(N.lib().kt_debug_rng_rate(eng.handle, 1 if normals else 0, iters, reps, C.byref(ms), C.byref(words)),
+            eng.handle)
+    return {"kernel": f"debug_rng_rate_kernel<{'true' if normals else 'false'}>", "iters": iters, "reps": reps,
+            "avg_ms": ms.value, ("normals_per_s" if normals else "words_per_s"): words.value}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -85,6 +96,8 @@ def main():
         from kontracts_amd import _native as N
         eng = N.engine(0)
         print(json.dumps(valu_probe(N, eng, reps=50)))
+        print(json.dumps(rng_probe(N, eng, False, reps=50)))
+        print(json.dumps(rng_probe(N, eng, True, reps=50)))
         return
     if args.copy_calibration:
         import ctypes as C
@@ -125,6 +138,7 @@ def main():
     k_ms, k_launches, k_path_steps = eng.kernel_time(reset=True)
     eng.set_profiling(False)
     fma = valu_probe(N, eng)  # right after the timed steps, at the same clocks
+    gen = rng_probe(N, eng, True)
 
     wall = t1 - t0
     ms_per_step = 1e3 * wall / args.steps
@@ -150,6 +164,15 @@ def main():
                           "dense contraction, the work is Philox integer ops, inverse normals and FMAs",
             "fma_issue_measured": round(fma["fma_lane_ops_per_s"] / 1e12, 3),
             "frac_of_measured_fma_issue": round(achieved * 1e12 / fma["fma_lane_ops_per_s"], 4)}
+    # Mix-limited ceiling: one normal at the generator's measured rate plus the
+    # rest of the path-step at the measured FMA rate (Philox's 64-bit multiplies
+    # and the transcendentals take several issue cycles, which lane-ops do not
+    # weight).
+    rate = path_steps_per_launch / avg_launch_s if avg_launch_s > 0 else 0.0
+    ceiling = 1.0 / (1.0 / gen["normals_per_s"] + (lane_ops - R.NORMAL) / fma["fma_lane_ops_per_s"])
+    roof.update({"normals_measured": round(gen["normals_per_s"] / 1e12, 4),
+                 "mix_ceiling_path_steps_per_s": round(ceiling, 1),
+                 "frac_of_mix_ceiling": round(rate / ceiling, 4)})
 
     cpu = None
     if not args.no_cpu:

@@ -297,6 +297,14 @@ kt_status kt_debug_copy(kt_engine* engine, uint64_t bytes, int32_t reps, double*
  * and *lane_ops the FMA lane-ops per second.  Measures the vector issue rate
  * the path kernels' roofline is compared with. */
 kt_status kt_debug_valu(kt_engine* engine, int32_t iters, int32_t reps, double* ms, double* lane_ops);
+/* Normal-generator throughput: 2^21 lanes each draw `iters` Philox4x32-10
+ * blocks with the path kernels' counter layout and fold the four words
+ * (normals == 0) or their four FP32 inverse normals (normals != 0) into one
+ * accumulator, repeated `reps` times after one warm launch; *ms receives the
+ * average kernel time and *words the words (or normals) per second.  Gives
+ * the issue cost of the generator the path kernels run once per normal. */
+kt_status kt_debug_rng_rate(kt_engine* engine, int32_t normals, int32_t iters, int32_t reps, double* ms,
+                            double* words);
 /* kt_price that also returns every path's present value (pv_out[n_paths]) and,
  * when normals_out != NULL, the FP32 normals the GPU drew, laid out
  * [path][grid point][component].  grid_out receives the grid (capacity

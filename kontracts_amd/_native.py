@@ -166,6 +166,7 @@ _SIGNATURES = {
     "kt_debug_normals": (C.c_int, [C.c_void_p, _P(C.c_uint32), C.c_uint64, _P(C.c_float)]),
     "kt_debug_copy": (C.c_int, [C.c_void_p, C.c_uint64, C.c_int32, _P(C.c_double)]),
     "kt_debug_valu": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, _P(C.c_double), _P(C.c_double)]),
+    "kt_debug_rng_rate": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, _P(C.c_double), _P(C.c_double)]),
     "kt_debug_paths": (
         C.c_int,
         [C.c_void_p, C.c_char_p, _P(KtModel), _P(KtMcConfig), _P(C.c_double), _P(C.c_float), _P(C.c_double),

@@ -534,6 +534,26 @@ int set_error(kt_engine* E, int code, const std::string& msg) {
 }
 }  // namespace kt
 
+namespace {
+// Average time of `reps` launches after one warm launch, on the engine's stream.
+template <class F>
+double time_probe(kt_engine* E, int reps, F launch) {
+  check_hip(launch(), "probe launch");
+  hipEvent_t a, b;
+  check_hip(hipEventCreate(&a), "event");
+  check_hip(hipEventCreate(&b), "event");
+  check_hip(hipEventRecord(a, E->stream), "record");
+  for (int i = 0; i < reps; ++i) check_hip(launch(), "probe launch");
+  check_hip(hipEventRecord(b, E->stream), "record");
+  check_hip(hipEventSynchronize(b), "sync");
+  float t = 0.0f;
+  check_hip(hipEventElapsedTime(&t, a, b), "elapsed");
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  return (double)t / reps;
+}
+}  // namespace
+
 extern "C" {
 
 kt_status kt_engine_create(int device, kt_engine** out) {
@@ -821,21 +841,23 @@ kt_status kt_debug_valu(kt_engine* E, int32_t iters, int32_t reps, double* ms, d
   if (!E || iters < 8 || iters % 8 || reps < 1) raise(KT_ERR_INVALID_ARGUMENT, "bad VALU probe arguments");
   check_hip(hipSetDevice(E->device), "hipSetDevice");
   float* out = (float*)E->ensure(E->aux, (size_t)kValuProbeBlocks * 256 * sizeof(float));
-  check_hip(launch_debug_valu(out, iters, E->stream), "valu launch");
-  hipEvent_t a, b;
-  check_hip(hipEventCreate(&a), "event");
-  check_hip(hipEventCreate(&b), "event");
-  check_hip(hipEventRecord(a, E->stream), "record");
-  for (int i = 0; i < reps; ++i) check_hip(launch_debug_valu(out, iters, E->stream), "valu launch");
-  check_hip(hipEventRecord(b, E->stream), "record");
-  check_hip(hipEventSynchronize(b), "sync");
-  float t = 0.0f;
-  check_hip(hipEventElapsedTime(&t, a, b), "elapsed");
-  (void)hipEventDestroy(a);
-  (void)hipEventDestroy(b);
-  const double avg_ms = (double)t / reps;
+  const double avg_ms = time_probe(E, reps, [&] { return launch_debug_valu(out, iters, E->stream); });
   if (ms) *ms = avg_ms;
   if (lane_ops) *lane_ops = (double)kValuProbeBlocks * 256 * (double)iters * 8.0 / (avg_ms * 1e-3);
+  return KT_OK;
+  KT_GUARD_END(E)
+}
+
+kt_status kt_debug_rng_rate(kt_engine* E, int32_t normals, int32_t iters, int32_t reps, double* ms,
+                            double* words) {
+  KT_GUARD_BEGIN
+  if (!E || iters < 1 || reps < 1) raise(KT_ERR_INVALID_ARGUMENT, "bad RNG probe arguments");
+  check_hip(hipSetDevice(E->device), "hipSetDevice");
+  float* out = (float*)E->ensure(E->aux, (size_t)kValuProbeBlocks * 256 * sizeof(float));
+  const double avg_ms =
+      time_probe(E, reps, [&] { return launch_debug_rng_rate(normals != 0, out, iters, E->stream); });
+  if (ms) *ms = avg_ms;
+  if (words) *words = (double)kValuProbeBlocks * 256 * (double)iters * 4.0 / (avg_ms * 1e-3);
   return KT_OK;
   KT_GUARD_END(E)
 }

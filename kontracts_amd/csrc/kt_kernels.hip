@@ -603,6 +603,29 @@ __global__ __launch_bounds__(256) void debug_valu_kernel(float* __restrict__ out
   out[(size_t)blockIdx.x * 256 + threadIdx.x] = s;
 }
 
+// Normal-generator throughput: each lane draws `iters` Philox blocks with the
+// path kernels' counter layout (block index, path, 0, 0; the key in SGPRs)
+// and folds the four words (NORMALS false: one three-input xor and one xor
+// per block) or their four inverse normals (NORMALS true: four adds per
+// block) into one accumulator.  Gives the issue cost of the generator the
+// path kernels run once per normal.
+template <bool NORMALS>
+__global__ __launch_bounds__(256) void debug_rng_rate_kernel(float* __restrict__ out, int iters, uint32_t k0,
+                                                             uint32_t k1) {
+  const uint32_t p = blockIdx.x * 256 + threadIdx.x;
+  uint32_t h = 0;
+  float s = 0.0f;
+#pragma unroll 2
+  for (int it = 0; it < iters; ++it) {
+    const uint4 q = philox4x32_10((uint32_t)it, p, 0u, 0u, k0, k1);
+    if constexpr (NORMALS)
+      s += (normal_from_word(q.x) + normal_from_word(q.y)) + (normal_from_word(q.z) + normal_from_word(q.w));
+    else
+      h = xor3(h, q.x ^ q.y, xor3(q.z, q.w, 0u));
+  }
+  out[p] = NORMALS ? s : __uint_as_float(h);
+}
+
 __global__ void debug_normals_kernel(const uint32_t* __restrict__ w, uint64_t n, float* __restrict__ out) {
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) out[i] = normal_from_word(w[i]);
@@ -1102,6 +1125,14 @@ hipError_t launch_debug_copy(const void* src, void* dst, uint64_t bytes, hipStre
 
 hipError_t launch_debug_valu(float* out, int iters, hipStream_t s) {
   hipLaunchKernelGGL(debug_valu_kernel, dim3(kValuProbeBlocks), dim3(256), 0, s, out, iters, 1.0000001f);
+  return hipGetLastError();
+}
+
+hipError_t launch_debug_rng_rate(bool normals, float* out, int iters, hipStream_t s) {
+  if (normals)
+    hipLaunchKernelGGL(debug_rng_rate_kernel<true>, dim3(kValuProbeBlocks), dim3(256), 0, s, out, iters, 42u, 7u);
+  else
+    hipLaunchKernelGGL(debug_rng_rate_kernel<false>, dim3(kValuProbeBlocks), dim3(256), 0, s, out, iters, 42u, 7u);
   return hipGetLastError();
 }
 

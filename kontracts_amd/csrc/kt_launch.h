@@ -190,6 +190,9 @@ hipError_t launch_debug_philox(const uint32_t* ctr, uint32_t k0, uint32_t k1, ui
 hipError_t launch_debug_copy(const void* src, void* dst, uint64_t bytes, hipStream_t s);
 constexpr int kValuProbeBlocks = 8192;
 hipError_t launch_debug_valu(float* out, int iters, hipStream_t s);
+// kValuProbeBlocks x 256 lanes, `iters` Philox blocks each (normals: their
+// four inverse normals too); out holds one float per lane.
+hipError_t launch_debug_rng_rate(bool normals, float* out, int iters, hipStream_t s);
 hipError_t launch_debug_normals(const uint32_t* words, uint64_t n, float* out, hipStream_t s);
 
 }  // namespace kt

@@ -83,6 +83,28 @@ def test_valu_issue_calibration(eng):
         assert ei.value.kind == "InvalidArgument"
 
 
+def test_rng_rate_probe(eng):
+    # the generator alone: raw Philox words and their inverse normals, each
+    # below what the published 78.6e12 lane-ops/s allows for the minimal
+    # instruction count (10 per word, 28.3 per normal); bad arguments rejected
+    from kontracts_amd import roofline as R
+    ms, rate = C.c_double(), C.c_double()
+    got = {}
+    for normals in (0, 1):
+        N.check(N.lib().kt_debug_rng_rate(eng.handle, normals, 64, 3, C.byref(ms), C.byref(rate)), eng.handle)
+        assert ms.value > 0.0
+        assert rate.value == pytest.approx(8192 * 256 * 64 * 4 / (ms.value * 1e-3), rel=1e-9)
+        got[normals] = rate.value
+    assert 1e11 < got[0] < R.PEAK_LANE_OPS / R.PHILOX_PER_WORD
+    assert 1e11 < got[1] < R.PEAK_LANE_OPS / R.NORMAL
+    assert got[1] < got[0]
+    for bad in ((0, 1), (64, 0)):
+        with pytest.raises(N.KontractError) as ei:
+            N.check(N.lib().kt_debug_rng_rate(eng.handle, 1, bad[0], bad[1], C.byref(ms), C.byref(rate)),
+                    eng.handle)
+        assert ei.value.kind == "InvalidArgument"
+
+
 def debug_paths(eng, contract, model, n_paths, seed, spy=50, grid=None):
     m = _model_struct(model)
     cfg = _cfg(n_paths, seed, spy, model.rate, grid)
