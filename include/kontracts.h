@@ -287,7 +287,8 @@ kt_status kt_debug_philox(kt_engine* engine, const uint32_t* ctr, const uint32_t
 /* FP32 inverse normal on the GPU applied to raw 32-bit words. */
 kt_status kt_debug_normals(kt_engine* engine, const uint32_t* words, uint64_t n, float* out);
 /* Streaming device-to-device copy of `bytes` (multiple of 16) through a plain
- * copy kernel, repeated `reps` times; *ms receives the average kernel time.
+ * copy kernel, repeated `reps` times after one warm launch; *ms receives the
+ * average kernel time.
  * Calibrates the profiler's HBM byte counters against a known size. */
 kt_status kt_debug_copy(kt_engine* engine, uint64_t bytes, int32_t reps, double* ms);
 /* VALU issue calibration: 2^21 lanes each run `iters` (a multiple of 8)

@@ -536,20 +536,26 @@ int set_error(kt_engine* E, int code, const std::string& msg) {
 
 namespace {
 // Average time of `reps` launches after one warm launch, on the engine's stream.
+struct ProbeEvents {
+  hipEvent_t a = nullptr, b = nullptr;
+  ~ProbeEvents() {
+    if (a) (void)hipEventDestroy(a);
+    if (b) (void)hipEventDestroy(b);
+  }
+};
+
 template <class F>
 double time_probe(kt_engine* E, int reps, F launch) {
   check_hip(launch(), "probe launch");
-  hipEvent_t a, b;
-  check_hip(hipEventCreate(&a), "event");
-  check_hip(hipEventCreate(&b), "event");
-  check_hip(hipEventRecord(a, E->stream), "record");
+  ProbeEvents ev;  // destroyed on every exit, the error paths included
+  check_hip(hipEventCreate(&ev.a), "event");
+  check_hip(hipEventCreate(&ev.b), "event");
+  check_hip(hipEventRecord(ev.a, E->stream), "record");
   for (int i = 0; i < reps; ++i) check_hip(launch(), "probe launch");
-  check_hip(hipEventRecord(b, E->stream), "record");
-  check_hip(hipEventSynchronize(b), "sync");
+  check_hip(hipEventRecord(ev.b, E->stream), "record");
+  check_hip(hipEventSynchronize(ev.b), "sync");
   float t = 0.0f;
-  check_hip(hipEventElapsedTime(&t, a, b), "elapsed");
-  (void)hipEventDestroy(a);
-  (void)hipEventDestroy(b);
+  check_hip(hipEventElapsedTime(&t, ev.a, ev.b), "elapsed");
   return (double)t / reps;
 }
 }  // namespace
@@ -820,18 +826,8 @@ kt_status kt_debug_copy(kt_engine* E, uint64_t bytes, int32_t reps, double* ms) 
   check_hip(hipSetDevice(E->device), "hipSetDevice");
   char* buf = (char*)E->ensure(E->aux, 2 * bytes);
   check_hip(hipMemsetAsync(buf, 1, bytes, E->stream), "memset");
-  hipEvent_t a, b;
-  check_hip(hipEventCreate(&a), "event");
-  check_hip(hipEventCreate(&b), "event");
-  check_hip(hipEventRecord(a, E->stream), "record");
-  for (int i = 0; i < reps; ++i) check_hip(launch_debug_copy(buf, buf + bytes, bytes, E->stream), "copy launch");
-  check_hip(hipEventRecord(b, E->stream), "record");
-  check_hip(hipEventSynchronize(b), "sync");
-  float t = 0.0f;
-  check_hip(hipEventElapsedTime(&t, a, b), "elapsed");
-  (void)hipEventDestroy(a);
-  (void)hipEventDestroy(b);
-  if (ms) *ms = t / reps;
+  const double avg_ms = time_probe(E, reps, [&] { return launch_debug_copy(buf, buf + bytes, bytes, E->stream); });
+  if (ms) *ms = avg_ms;
   return KT_OK;
   KT_GUARD_END(E)
 }
