@@ -204,6 +204,30 @@ def test_per_path_parity(eng, name):
     assert abs(res.price - cpu_res["price"]) <= 3 * cpu_res["std_error"] + 1e-9
 
 
+@pytest.mark.parametrize("n", [1, 2, 63, 255, 257, 1000])
+@pytest.mark.parametrize("name", ["gbm_up_and_out", "heston_up_and_out", "basket5"])
+def test_ragged_path_counts(eng, name, n):
+    """Path counts that leave a wave or a block partly empty: per-path values
+    match the oracle fed the same normals, and the pricing kernel's
+    statistics are those of the live paths only (one path: zero spread, as
+    summarize gives for N <= 1, src/pricer.rs:299-328)."""
+    contract, model, spy, grid, notional, flip_budget = PARITY_CASES[name]
+    pv, z, g, res = debug_paths(eng, contract, model, n, 7, spy, grid)
+    assert pv.shape == (n,) and z.shape[0] == n
+    cpu_res, pv_cpu, _ = O.price(contract.to_json(), oracle_model(model), O.cfg(n, 7, spy, model.rate, grid=grid),
+                                 normals=z, want_pv=True)
+    bad = np.abs(pv - pv_cpu) > 1e-5 * (np.abs(pv_cpu) + notional)
+    # a barrier path may sit within FP32 rounding of its level
+    assert int(bad.sum()) <= (max(1, int(flip_budget * n)) if flip_budget else 0), (name, n, int(bad.sum()))
+    priced = contract.price(model, n_paths=n, seed=7, steps_per_year=spy, grid=grid)
+    assert priced.n_paths == n
+    assert priced.price == pytest.approx(float(np.mean(pv)), rel=1e-12, abs=1e-12)
+    if n == 1:
+        assert priced.sample_std == 0.0 and priced.std_error == 0.0
+    else:
+        assert priced.sample_std == pytest.approx(float(np.std(pv, ddof=1)), rel=1e-9, abs=1e-12)
+
+
 @pytest.mark.parametrize("name", sorted(PARITY_CASES))
 def test_pricing_kernel_equals_dispatching_kernel(eng, name):
     """The pricing kernels run a leading run of groups of one fast kind (and
@@ -345,10 +369,11 @@ def _lookback_knock_out():
     return K.up_and_out_call("X", 100.0, 200.0, 1.0) + K.until(S >= 230.0, K.scale(e, K.one("USD")) @ K.at(1.0))
 
 
+@pytest.mark.parametrize("n", [65536, 1000])
 @pytest.mark.parametrize("which", ["up_and_out_8_registers", "lookback_knock_out_16_registers"])
-def test_greeks_scenario_kernel_equals_separate_pricings(which):
+def test_greeks_scenario_kernel_equals_separate_pricings(which, n):
     c = K.up_and_out_call("X", 100, 200, 1.0) if which.startswith("up_and_out") else _lookback_knock_out()
-    s0, sg, r, n, spy = 180.0, 0.25, 0.05, 65536, 252
+    s0, sg, r, spy = 180.0, 0.25, 0.05, 252
     hs, hv, hr = 1.8, 0.01, 1e-4
     g = c.greeks(K.GBM(s0, sg, r, asset="X"), n_paths=n, seed=7, steps_per_year=spy, bumps=(hs, hv, hr))
 
