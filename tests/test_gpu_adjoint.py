@@ -261,3 +261,34 @@ def test_heston_sweep_on_equal_and_unequal_steps():
             assert a.price == pytest.approx(p.price, rel=1e-6)
     finally:
         eng.set_option("pathwise_mode", 0)
+
+
+@pytest.mark.parametrize("n", [1, 257, 1000])
+def test_ragged_path_counts(n):
+    """Path counts that leave a block partly empty: the pathwise launch's
+    price is the pricer's on the same paths, one path gives zero standard
+    errors, and the Heston reverse sweep (checkpoints indexed by global path)
+    agrees with the forward-tangent mode."""
+    eng = N.engine(0)
+    gbm = K.GBM(100.0, 0.25, 0.03, asset="X")
+    hes = K.heston(100.0, 0.04, 2.0, 0.04, 0.3, -0.5, 0.05, asset="X")
+    fac = [K.GbmFactor(nm, 100.0, 0.03, 0.2 + 0.05 * i) for i, nm in enumerate("ABC")]
+    bsk = K.correlated_gbm(fac, [[1.0 if i == j else 0.3 for j in range(3)] for i in range(3)], 0.03)
+    bc = K.scale(((K.spot("A") + K.spot("B") + K.spot("C")) / 3.0 - 100.0).clip(0.0), K.one("USD")) @ K.at(1.0)
+    for c, m, spy in ((call("X", 95.0), gbm, 52), (asian_call("X", 100.0), hes, 120), (bc, bsk, 50)):
+        g = c.pathwise_greeks(m, n_paths=n, seed=3, steps_per_year=spy)
+        p = c.price(m, n_paths=n, seed=3, steps_per_year=spy)
+        assert g.price == pytest.approx(p.price, rel=1e-6, abs=1e-9)
+        assert all(np.isfinite(g.values))
+        if n == 1:
+            assert g.price_std_error == 0.0 and all(se == 0.0 for se in g.std_errors)
+    try:
+        eng.set_option("pathwise_mode", 0)
+        a = asian_call("X", 100.0).pathwise_greeks(hes, n_paths=n, seed=3, steps_per_year=120)
+        eng.set_option("pathwise_mode", 1)
+        b = asian_call("X", 100.0).pathwise_greeks(hes, n_paths=n, seed=3, steps_per_year=120)
+        assert a.price == b.price
+        for name, x, y in zip(a.names, a.values, b.values):
+            assert abs(x - y) <= 1e-4 * max(1.0, abs(x)), (n, name, x, y)
+    finally:
+        eng.set_option("pathwise_mode", 0)
