@@ -415,6 +415,9 @@ __device__ __forceinline__ void path_block(const PA& A, uint32_t blk, float (&r)
     for (int wv = 1; wv < kBlock / 64; ++wv) acc = stats_combine(acc, sh[threadIdx.x][wv]);
     A.partials[(size_t)blk * NSTAT + threadIdx.x] = acc;
   }
+  // sh is reused by the next block a persistent caller runs: no wave may
+  // write it again before the reads above are done.
+  __syncthreads();
 }
 
 template <int MODEL, int K, int NS, int NOUT, int SC, int NSTAT, bool DEBUG, bool ANTI>
