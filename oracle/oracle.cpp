@@ -1242,6 +1242,28 @@ int orc_price(const char* json, const orc_model* m, const orc_cfg* cfg, const fl
   }
 }
 
+// Simulator::simulate_paths (src/simulator.rs:70-178; GBM :215-253, Heston
+// :506-541, CorrelatedGbmN src/multi_asset.rs:130-179) on an explicit grid:
+// out[(path * n_assets + asset) * n_grid + k] = spot of `asset` at grid point
+// k, drawn from the same Philox stream as orc_price.
+int orc_simulate(const orc_model* m, const double* grid, uint64_t n_grid, uint64_t n_paths, uint64_t seed,
+                 int threads, double* out, char* err, int errcap) {
+  try {
+    Model md = make_model(m);
+    validate_model(md);
+    std::vector<double> g(grid, grid + n_grid);
+    validate_grid(g);
+    const size_t K = md.names.size(), n = g.size();
+    Normals nz{seed, nullptr, n, md.m, 0u};
+    parallel_paths(n_paths, threads, [&](uint64_t b, uint64_t e) {
+      for (uint64_t i = b; i < e; ++i) simulate_path(md, g, i, nz, out + i * K * n);
+    });
+    return 0;
+  } catch (KErr& e) {
+    return report(e, err, errcap);
+  }
+}
+
 // price_batch_gbm (src/pricer.rs:250-294): merged plan, one simulation.
 int orc_price_batch(const char* const* jsons, uint64_t n_contracts, const orc_model* m, const orc_cfg* cfg,
                     orc_result* out, char* err, int errcap) {

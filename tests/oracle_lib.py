@@ -312,3 +312,16 @@ def summarize(pvs):
     res = OrcResult()
     L.orc_summarize(a.ctypes.data_as(C.POINTER(C.c_double)), C.c_uint64(len(a)), C.byref(res))
     return res.as_dict()
+
+
+def simulate(model, grid, n_paths, seed, threads=0):
+    """Simulated spots [n_paths][n_assets][n_grid] (Simulator::simulate_paths)."""
+    L = load()
+    K = model.s.n_assets
+    out = np.zeros((n_paths, K, len(grid)), dtype=np.float64)
+    e = _err()
+    rc = L.orc_simulate(C.byref(model.s), _dbl(grid), C.c_uint64(len(grid)), C.c_uint64(n_paths), C.c_uint64(seed),
+                        threads, out.ctypes.data_as(C.POINTER(C.c_double)), e, 512)
+    if rc:
+        raise OracleError(rc, e.value.decode())
+    return out

@@ -185,3 +185,72 @@ def test_portfolio_greeks_errors_and_tiles():
     finally:
         eng.set_option("portfolio_blocks_per_cu", 0)
         eng.set_option("portfolio_chunk", 1)
+
+
+# ---------------------------------------------------------------------------
+# The 1,000-contract book itself (1.63e11 path-steps)
+# ---------------------------------------------------------------------------
+FULL_TILES = ((4, 4), (0, 1), (1, 16))  # the autotuner's pick for this book, the default, a long chunk
+
+
+def _set_tile(eng, bpc, chunk):
+    eng.set_option("portfolio_blocks_per_cu", bpc)
+    eng.set_option("portfolio_chunk", chunk)
+
+
+def test_full_book_equals_per_contract_pricing():
+    """Every job of the whole book, priced in one persistent launch under
+    each tile, is bit-identical to pricing it alone (the reference's loop of
+    price_gbm calls, src/pricer.rs:97-115).  The book has ~1.6e6 (job, block)
+    items, so a chunk of 4 or 16 items per queue fetch takes effect on the
+    1,024 resident workgroups (the chunk is capped at items per workgroup)."""
+    eng = N.engine(0)
+    jobs = B.book(1000)
+    assert len(jobs) == 1000
+    n_items = sum((job[2] + 255) // 256 for job in jobs)
+    assert n_items // (256 * 4) >= 16
+    want = price_each(jobs)
+    try:
+        for bpc, chunk in FULL_TILES:
+            _set_tile(eng, bpc, chunk)
+            assert_identical(K.price_portfolio(jobs), want)
+    finally:
+        _set_tile(eng, 0, 1)
+
+
+def test_full_book_greeks_equal_per_contract():
+    eng = N.engine(0)
+    jobs = B.book(1000)
+    want = per_contract_greeks(jobs)
+    try:
+        for bpc, chunk in FULL_TILES[:2]:
+            _set_tile(eng, bpc, chunk)
+            assert_greeks_identical(K.price_portfolio(jobs, greeks=True), want)
+    finally:
+        _set_tile(eng, 0, 1)
+
+
+def test_book_slice_vs_oracle():
+    """The first 16 jobs of the book (4 of each kind, GBM and Heston, monthly
+    fixings, 5-asset baskets) at 8,192 paths each, in one persistent launch
+    under the tuned tile, against the oracle on the same Philox stream:
+    continuous payoffs agree to FP32 path rounding, knock-outs within half a
+    standard error (paths on the level may flip)."""
+    import oracle_lib as O
+    from test_gpu_parity import oracle_model
+
+    eng = N.engine(0)
+    jobs = [(c, m, 8192, seed, spy, grid) for c, m, _, seed, spy, grid in B.book(16)]
+    try:
+        _set_tile(eng, 4, 4)
+        got = K.price_portfolio(jobs)
+    finally:
+        _set_tile(eng, 0, 1)
+    for i, (job, g) in enumerate(zip(jobs, got)):
+        c, m, n, seed, spy, grid = job
+        want, _, _ = O.price(c.to_json(), oracle_model(m), O.cfg(n, seed, spy, m.rate, grid=grid, threads=16))
+        smooth = B.KINDS[i % 4] != "barrier"
+        tol = 5e-5 * (abs(want["price"]) + 100.0) if smooth else 0.5 * want["std_error"]
+        print(f"job {i} {B.KINDS[i % 4]}: gpu {g.price:.8f} oracle {want['price']:.8f} se {want['std_error']:.2e}")
+        assert abs(g.price - want["price"]) <= tol, i
+        assert g.n_paths == n
