@@ -145,6 +145,21 @@ const char* kt_last_error_global(void);
 kt_status kt_price(kt_engine* engine, const char* contract_json, const kt_model* model,
                    const kt_mc_config* cfg, kt_price_result* out);
 
+/* kt_price without device allocation.  No counterpart in the reference: it is
+ * for callers that cannot call hipMalloc (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid allocation).  Same arguments, result
+ * and errors as kt_price (bit-identical results), except:
+ *  - the program tables, block partials, error word and statistics live in
+ *    one static 8 MiB device array per GPU (held until the process ends), so
+ *    n_paths is at most 2^26 (67,108,864) and the tables at most 2 MiB less
+ *    4 KiB; larger calls return KT_ERR_INVALID_ARGUMENT before any launch;
+ *  - variance-reduction flags return KT_ERR_UNSUPPORTED;
+ *  - every call on one GPU shares that array and leaves its contents there:
+ *    calls that overlap on one GPU (two engines, two host threads) race, and
+ *    callers must serialise them. */
+kt_status kt_price_noalloc(kt_engine* engine, const char* contract_json, const kt_model* model,
+                           const kt_mc_config* cfg, kt_price_result* out);
+
 /* price_batch_gbm (src/pricer.rs:250-294): one merged grid (union of dates,
  * max horizon, OR of fine flags), one simulation shared by all contracts, one
  * result per contract.  n_contracts == 0 returns KT_OK with nothing written. */

@@ -115,6 +115,7 @@ _SIGNATURES = {
     "kt_last_error": (C.c_char_p, [C.c_void_p]),
     "kt_last_error_global": (C.c_char_p, []),
     "kt_price": (C.c_int, [C.c_void_p, C.c_char_p, _P(KtModel), _P(KtMcConfig), _P(KtPriceResult)]),
+    "kt_price_noalloc": (C.c_int, [C.c_void_p, C.c_char_p, _P(KtModel), _P(KtMcConfig), _P(KtPriceResult)]),
     "kt_price_batch": (
         C.c_int,
         [C.c_void_p, _P(C.c_char_p), C.c_uint64, _P(KtModel), _P(KtMcConfig), _P(KtPriceResult)],

@@ -365,6 +365,44 @@ LaunchTables launch_tables(const Program& prog, const std::vector<ModelDesc>& mo
   return T;
 }
 
+std::vector<std::pair<const void*, size_t>> program_blobs(const Program& prog, const LaunchTables& T,
+                                                         std::vector<KOp>* ops) {
+  *ops = prog.ops;
+  if (ops->empty()) ops->push_back(KOp{OP_NOP, 0, 0, 0, 0.0f});
+  return {{ops->data(), ops->size() * sizeof(KOp)},
+          {prog.steps.data(), prog.steps.size() * sizeof(KStep)},
+          {T.groups.data(), T.groups.size() * sizeof(KGroup)},
+          {prog.init.data(), prog.init.size() * sizeof(KOp)},
+          {T.df.data(), T.df.size() * sizeof(double)},
+          {T.tab.data(), T.tab.size() * sizeof(float)},
+          {T.thr.data(), T.thr.size() * sizeof(float)}};
+}
+
+PathArgs program_path_args(const Program& prog, const std::vector<ModelDesc>& models, const LaunchTables& T,
+                           const std::vector<double>& grid, uint64_t seed, const std::vector<void*>& dev) {
+  const int SC = (int)models.size();
+  PathArgs A{};
+  A.ops = (const KOp*)dev[0];
+  A.steps = (const KStep*)dev[1];
+  A.groups = (const KGroup*)dev[2];
+  A.init = (const KOp*)dev[3];
+  A.df = (const double*)dev[4];
+  A.tab = (const float*)dev[5];
+  A.thr = (const float*)dev[6];
+  A.n_init = (int)prog.init.size();
+  A.n_grid = (int)grid.size();
+  A.n_groups = (int)T.groups.size();
+  A.n_uniform = uniform_group_prefix(T.groups.data(), A.n_groups);
+  A.tab_uniform = uniform_table_rows(T.tab, grid.size(), (size_t)T.stride);
+  A.tab_stride = T.stride;
+  fill_model_args(A, models[0], seed);
+  for (int sc = 0; sc < SC && sc < kScenarios; ++sc) {
+    A.s0_scen[sc] = (float)models[sc].s0[0];
+    A.df_index[sc] = T.df_index[sc];
+  }
+  return A;
+}
+
 std::vector<KStats> run_program_scen(kt_engine* E, const Program& prog, const std::vector<ModelDesc>& models,
                                      const std::vector<double>& rates, const std::vector<std::vector<double>>& stat_w,
                                      const std::vector<double>& grid, uint64_t n_paths, uint64_t seed,
@@ -372,12 +410,6 @@ std::vector<KStats> run_program_scen(kt_engine* E, const Program& prog, const st
   const int SC = (int)models.size();
   const ModelDesc& md = models[0];
   LaunchTables T = launch_tables(prog, models, rates, grid);
-  const int stride = T.stride;
-  const std::vector<float>& tab = T.tab;
-  const std::vector<double>& df = T.df;
-  const std::vector<int>& df_index = T.df_index;
-  const std::vector<KGroup>& groups = T.groups;
-  const std::vector<float>& thr = T.thr;
 
   int kind = model_kernel_kind(md);
   int K = (int)md.names.size();
@@ -392,15 +424,8 @@ std::vector<KStats> run_program_scen(kt_engine* E, const Program& prog, const st
   const int nstat = SC == 1 ? nout : kScenarioStats;
   const int n_used = SC == 1 ? prog.n_outputs : kScenarioStats;
 
-  std::vector<KOp> ops = prog.ops;
-  if (ops.empty()) ops.push_back(KOp{OP_NOP, 0, 0, 0, 0.0f});
-  auto dev = E->upload({{ops.data(), ops.size() * sizeof(KOp)},
-                        {prog.steps.data(), prog.steps.size() * sizeof(KStep)},
-                        {groups.data(), groups.size() * sizeof(KGroup)},
-                        {prog.init.data(), prog.init.size() * sizeof(KOp)},
-                        {df.data(), df.size() * sizeof(double)},
-                        {tab.data(), tab.size() * sizeof(float)},
-                        {thr.data(), thr.size() * sizeof(float)}});
+  std::vector<KOp> ops;
+  auto dev = E->upload(program_blobs(prog, T, &ops));
 
   const uint64_t kMaxBlocksPerLaunch = 1ull << 20;
   uint64_t total_blocks = (n_paths + kBlock - 1) / kBlock;
@@ -419,25 +444,7 @@ std::vector<KStats> run_program_scen(kt_engine* E, const Program& prog, const st
   else if (pv_host) pv_dev = (double*)E->ensure(E->pv, std::max<uint64_t>(n_paths, 1) * sizeof(double));
   if (z_host) z_dev = (float*)E->ensure(E->z, std::max<uint64_t>(n_paths * grid.size() * M, 1) * sizeof(float));
 
-  PathArgs A{};
-  A.ops = (const KOp*)dev[0];
-  A.steps = (const KStep*)dev[1];
-  A.groups = (const KGroup*)dev[2];
-  A.init = (const KOp*)dev[3];
-  A.df = (const double*)dev[4];
-  A.tab = (const float*)dev[5];
-  A.thr = (const float*)dev[6];
-  A.n_init = (int)prog.init.size();
-  A.n_grid = (int)grid.size();
-  A.n_groups = (int)groups.size();
-  A.n_uniform = uniform_group_prefix(groups.data(), A.n_groups);
-  A.tab_uniform = uniform_table_rows(tab, grid.size(), (size_t)stride);
-  A.tab_stride = stride;
-  fill_model_args(A, md, seed);
-  for (int sc = 0; sc < SC && sc < kScenarios; ++sc) {
-    A.s0_scen[sc] = (float)models[sc].s0[0];
-    A.df_index[sc] = df_index[sc];
-  }
+  PathArgs A = program_path_args(prog, models, T, grid, seed, dev);
   if (combine > 0) {
     for (int o = 0; o < combine; ++o) A.stat_w[0][o] = stat_w[0][o];
     A.combine = combine;
@@ -513,19 +520,6 @@ PreparedContract prepare(const char* json, const ModelDesc& md, const kt_mc_conf
 // ---------------------------------------------------------------------------
 // C ABI
 // ---------------------------------------------------------------------------
-#define KT_GUARD_BEGIN try {
-#define KT_GUARD_END(E)                                      \
-  }                                                          \
-  catch (const kt::Error& e) {                               \
-    return (kt_status)kt::set_error((E), e.code, e.msg);     \
-  }                                                          \
-  catch (const std::bad_alloc&) {                            \
-    return (kt_status)kt::set_error((E), KT_ERR_OUT_OF_MEMORY, "host allocation failed"); \
-  }                                                          \
-  catch (const std::exception& ex) {                         \
-    return (kt_status)kt::set_error((E), KT_ERR_DEVICE, ex.what()); \
-  }
-
 namespace kt {
 int set_error(kt_engine* E, int code, const std::string& msg) {
   if (E) E->err = msg;
@@ -572,6 +566,9 @@ kt_status kt_engine_create(int device, kt_engine** out) {
   if (hipSetDevice(device) != hipSuccess) return (kt_status)set_error(nullptr, KT_ERR_DEVICE, "hipSetDevice failed");
   kt_engine* E = new kt_engine();
   E->device = device;
+  // the allocation-free entry point's scratch address on this device (its
+  // failure is reported by that entry point, not here)
+  (void)kt::noalloc_scratch_init(device);
   if (hipStreamCreateWithFlags(&E->stream, hipStreamNonBlocking) != hipSuccess) {
     delete E;
     return (kt_status)set_error(nullptr, KT_ERR_DEVICE, "hipStreamCreate failed");

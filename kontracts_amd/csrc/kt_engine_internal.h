@@ -145,6 +145,17 @@ LaunchTables launch_tables(const Program& prog, const std::vector<ModelDesc>& mo
 // correlation split) and the Philox key.
 void fill_model_args(PathArgs& A, const ModelDesc& md, uint64_t seed);
 
+// The seven device tables of one path-kernel launch, in the order
+// program_path_args expects their device copies (ops receives the op list,
+// padded to one no-op when empty, and must outlive the blobs).
+std::vector<std::pair<const void*, size_t>> program_blobs(const Program& prog, const LaunchTables& T,
+                                                         std::vector<KOp>* ops);
+// Kernel arguments of one launch (scenario sc of models[sc]) whose tables
+// sit at dev[0..6]; partials, error word and path range are left to the
+// caller.
+PathArgs program_path_args(const Program& prog, const std::vector<ModelDesc>& models, const LaunchTables& T,
+                           const std::vector<double>& grid, uint64_t seed, const std::vector<void*>& dev);
+
 std::vector<KStats> run_program_scen(kt_engine* E, const Program& prog, const std::vector<ModelDesc>& models,
                                      const std::vector<double>& rates, const std::vector<std::vector<double>>& stat_w,
                                      const std::vector<double>& grid, uint64_t n_paths, uint64_t seed,
@@ -195,4 +206,24 @@ kt_price_result price_vr(kt_engine* E, const Con& con, const ModelDesc& md, cons
 // The reference's Black-Scholes call (Abramowitz-Stegun normal CDF).
 double bs_call_reference(double s, double k, double t, double r, double sigma);
 
+// Scratch array of the allocation-free entry point on `device` (set up
+// once per device by kt_engine_create, with the device current).
+hipError_t noalloc_scratch_init(int device);
+
 }  // namespace kt
+
+// C ABI guard: kt::Error, allocation failures and other exceptions become a
+// kt_status and the engine's (and the thread's) last error message.
+#define KT_GUARD_BEGIN try {
+#define KT_GUARD_END(E)                                                                   \
+  }                                                                                       \
+  catch (const kt::Error& e) {                                                            \
+    return (kt_status)kt::set_error((E), e.code, e.msg);                                  \
+  }                                                                                       \
+  catch (const std::bad_alloc&) {                                                         \
+    return (kt_status)kt::set_error((E), KT_ERR_OUT_OF_MEMORY, "host allocation failed"); \
+  }                                                                                       \
+  catch (const std::exception& ex) {                                                      \
+    return (kt_status)kt::set_error((E), KT_ERR_DEVICE, ex.what());                       \
+  }
+
