@@ -1,26 +1,36 @@
 #!/usr/bin/env python3
 """Benchmark of the GPU pricing engine on one MI355X.
 
-Workload (one "step"): the up-and-out barrier call ((S-100)^+ at T=1, knocked
-out when S >= 200 on any of 253 daily monitoring points) under GBM s0=180,
-sigma=0.25, r=0.05, 4,194,304 paths x 252 steps, priced with its Greeks by CRN
-bump-and-reprice exactly as greeks_gbm does (price, S+-1.8, sigma+-0.01,
-r+-1e-4: seven scenario pricings sharing one random stream).  The metric is
-scenario path-steps per second: 7 x paths x steps per step, divided by the
-wall time of the timed steps.  Inputs are a contract, a model and a seed, so
-nothing crosses PCIe inside the timed region except the program tables
-(a few KB) and the results.
+Headline workload (one "step"): the 1,000-contract book of
+kontracts_amd.book (250 each of European, monthly-fixing Asian, up-and-out
+barrier and 5-asset basket calls; maturities 0.25-3 y on daily grids; 2^16 to
+2^20 paths per contract; single-asset contracts half GBM, half Heston; 1.63e11
+path-steps), priced with the pathwise Greeks of its Lipschitz contracts
+(European, Asian, basket) in persistent launches over (contract, block)
+queues under the autotuned tile, and the barriers in the price-only
+persistent launch.  `value` = the book's path-steps per second of wall time
+over the timed steps, inputs (contracts, models, seeds) resident: nothing
+but the program tables crosses PCIe.
 
-Prints one JSON line.  The roofline object uses HIP events recorded by the
-engine on its own stream around every path-kernel launch; cpu_baseline times
-the oracle (the C++ f64 restatement of the reference pricer) on this host's
-cores on a bounded number of paths.
+The JSON line also carries, as sub-objects of "configs", the other
+workloads of the metric:
+  config1  European call, GBM, 1,000,000 paths x 252 steps
+  config2  up-and-out call CRN Greeks (7 scenario pricings), 4,194,304 x 252
+  config3  Heston call, 2,097,152 x 365: six pathwise Greeks vs 13 bump
+           pricings at equal standard error
+  config5  American put by Longstaff-Schwartz, 2^20 paths x 50 dates (HBM)
+each with its own roofline.  Roofline figures use device times from HIP
+events the engine records on its own stream around every path-kernel launch
+group; cpu_baseline objects time the oracle (the C++ FP64 restatement of the
+reference pricer) on this host's cores on a bounded sample.
 """
 from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
+import platform
 import statistics
 import sys
 import time
@@ -29,31 +39,228 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "path-steps/s on 1 MI355X vs host CPU; adjoint vs bump Greeks/s at equal stderr"
+PEAK_BASIS = ("FP32 vector 157.3 TFLOP/s = 78.6e12 FMA lane-ops/s; lane-ops are the minimal gfx950 instruction "
+              "sequence per path-step (kontracts_amd/roofline.py); no MFMA: no step is a dense contraction, the "
+              "work is Philox integer ops, inverse normals and FMAs")
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(n_paths: int, threads: int):
-    sys.path.insert(0, os.path.join(ROOT, "tests"))
-    import oracle_lib as O
-    import kontracts_amd as K
+def cpu_info():
+    model = platform.processor() or ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    affinity = len(os.sched_getaffinity(0))
+    # the box's CPU share for one GPU is OMP_NUM_THREADS (16); use all of it
+    share = int(os.environ.get("OMP_NUM_THREADS", affinity) or affinity)
+    return model, affinity, max(1, min(affinity, share))
 
-    c = K.up_and_out_call("X", 100.0, 200.0, 1.0)
-    cfg = O.cfg(n_paths, 42, 252, 0.05, threads=threads)
+
+def timed_steps(eng, fn, steps):
+    """Wall time and engine device time of `steps` calls (each call returns
+    after the engine's stream has drained)."""
+    eng.set_profiling(True)
+    eng.kernel_time(reset=True)
     t0 = time.perf_counter()
-    O.greeks_gbm(c.to_json(), "X", 180.0, 0.25, cfg, 1.8, 0.01, 1e-4)
-    dt = time.perf_counter() - t0
-    return 7 * n_paths * 252 / dt, dt
+    out = None
+    for _ in range(steps):
+        out = fn()
+    wall = time.perf_counter() - t0
+    dev_ms, launches, path_steps = eng.kernel_time(reset=True)
+    eng.set_profiling(False)
+    return out, wall, dev_ms, launches, path_steps
 
 
-def traffic_from_profiles():
-    path = os.path.join(ROOT, "profiles", "path_kernel_gbm_barrier_traffic.json")
+def median_call(eng, fn, reps=5):
+    """Median wall and device time of reps calls after a warm one."""
+    out = fn()
+    walls, devs = [], []
+    for _ in range(reps):
+        out, w, d, _, _ = timed_steps(eng, fn, 1)
+        walls.append(w)
+        devs.append(d)
+    return out, statistics.median(walls), statistics.median(devs)
+
+
+def roof(lane_ops_per_launch, avg_launch_ms, kernel, extra=None):
+    from kontracts_amd import roofline as R
+    achieved = lane_ops_per_launch / (avg_launch_ms / 1e3) / 1e12 if avg_launch_ms > 0 else 0.0
+    peak = R.PEAK_LANE_OPS / 1e12
+    out = {"bound": "valu", "achieved": round(achieved, 3), "peak": round(peak, 3), "unit": "Tlane-op/s",
+           "frac": round(achieved / peak, 4), "traffic": None, "kernel": kernel,
+           "avg_launch_ms": round(avg_launch_ms, 4), "lane_ops_per_launch": lane_ops_per_launch,
+           "peak_basis": PEAK_BASIS}
+    if extra:
+        out.update(extra)
+    return out
+
+
+def traffic_from_profiles(name):
+    path = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(path):
         return None
     with open(path) as f:
         return json.load(f).get("bytes_per_launch")
+
+
+# ---------------------------------------------------------------------------
+# Config 4: the 1,000-contract book (headline)
+# ---------------------------------------------------------------------------
+def bench_book(args, eng, K, B, R, T, threads):
+    jobs = B.book(1000)
+    profiles = [B.job_profile(i, j) for i, j in enumerate(jobs)]
+    path_steps = B.path_steps(jobs)
+    # algorithmic lane-ops of one step: each job's paths x steps x its kernel's count
+    lane_ops = sum(job[2] * (len(job[5]) - 1) * R.book_job(kind, model, n, nf, greeks=True)
+                   for job, (kind, model, n, nf) in zip(jobs, profiles))
+    tuned = T.autotune_portfolio(jobs, repeats=1, greeks=True)
+    log(f"config4: autotuned tile {tuned['best']} ({', '.join(f'{k}: {v:.1f} ms' for k, v in tuned['device_ms'].items())})")
+
+    def step():
+        return K.price_portfolio(jobs, greeks=True)
+
+    for _ in range(args.warmup):
+        step()
+    res, wall, dev_ms, launches, _ = timed_steps(eng, step, args.steps)
+    value = path_steps * args.steps / wall
+    dev_per_step = dev_ms / args.steps
+    log(f"config4: {1e3 * wall / args.steps:.1f} ms/step ({dev_per_step:.1f} ms device), {value:.4e} path-steps/s")
+    n_greeks = sum(1 for _, g in res if g is not None)
+
+    # the reference's way: one call per contract (price_gbm loop), pathwise
+    # Greeks per contract where the payoff admits them
+    def per_contract():
+        for i, job in enumerate(jobs):
+            kw = dict(n_paths=job[2], seed=job[3], steps_per_year=job[4], grid=job[5])
+            if profiles[i][0] == "barrier":
+                job[0].price(job[1], **kw)
+            else:
+                job[0].pathwise_greeks(job[1], **kw)
+
+    _, wall_pc, dev_pc, _, _ = timed_steps(eng, per_contract, 1)
+    # price only, the same book
+    _, wall_po, dev_po, _, _ = timed_steps(eng, lambda: K.price_portfolio(jobs), 2)
+    lane_ops_po = sum(job[2] * (len(job[5]) - 1) * R.book_job(kind, model, n, nf, greeks=False)
+                      for job, (kind, model, n, nf) in zip(jobs, profiles))
+    roofline = roof(lane_ops, dev_per_step,
+                    "portfolio_adjoint_kernel<GBM | Heston | Heston+feeds | basket5> + portfolio_kernel<16> "
+                    "(the launch sequence of one step, device time summed)",
+                    {"launch_groups_per_step": launches / args.steps,
+                     "lane_ops_per_path_step_mean": round(lane_ops / path_steps, 3),
+                     "traffic": traffic_from_profiles("portfolio_book_traffic.json")})
+    kinds = {}
+    for (kind, model, _, _) in profiles:
+        kinds[f"{kind}/{model}"] = kinds.get(f"{kind}/{model}", 0) + 1
+    info = {
+        "workload": "1,000-contract book (kontracts_amd.book, seed 42): 250 each of European, monthly-fixing Asian, "
+                    "up-and-out barrier and 5-asset basket (rho 0.3) calls; K 80-120% of 100, vol 0.15-0.45, T "
+                    "0.25-3 y on daily grids (63-756 steps), 2^16-2^20 paths, single-asset contracts GBM/Heston "
+                    "50/50; pathwise Greeks on the 750 Lipschitz contracts, barriers priced; persistent launches "
+                    "under the autotuned tile",
+        "contracts": len(jobs), "path_steps": path_steps, "kinds": kinds, "with_greeks": n_greeks,
+        "tile": {"best": list(tuned["best"]), "device_ms": tuned["device_ms"]},
+        "parallelism": "single GPU",
+    }
+    extra = {
+        "per_contract_s": wall_pc, "per_contract_device_ms": dev_pc,
+        "speedup_vs_per_contract": wall_pc / (wall / args.steps),
+        "price_only_ms": 1e3 * wall_po / 2, "price_only_device_ms": dev_po / 2,
+        "price_only_path_steps_per_s": path_steps * 2 / wall_po,
+        "price_only_roofline_frac": round(lane_ops_po / (dev_po / 2 / 1e3) / R.PEAK_LANE_OPS, 4),
+        "first_prices": [round(r.price, 6) for r, _ in res[:4]],
+    }
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_book(jobs, threads, args.cpu_paths_book)
+    return value, wall / args.steps, info, roofline, extra, cpu
+
+
+def cpu_book(jobs, threads, n_paths):
+    """The oracle (reference algorithm in f64: materialised paths, recursive
+    contract evaluation, two-pass summarize) on every 25th job of the book
+    (10 of each kind) at n_paths paths each."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as O
+    from test_gpu_parity import oracle_model
+
+    sample = jobs[::25]
+    ps = 0.0
+    t0 = time.perf_counter()
+    for c, m, _, seed, spy, grid in sample:
+        O.price(c.to_json(), oracle_model(m), O.cfg(n_paths, seed, spy, m.rate, grid=grid, threads=threads))
+        ps += n_paths * (len(grid) - 1)
+    dt = time.perf_counter() - t0
+    model, affinity, _ = cpu_info()
+    log(f"cpu baseline (book sample) {ps / dt:.4e} path-steps/s on {threads} threads ({dt:.1f} s)")
+    return {"value": round(ps / dt, 1), "unit": "path-steps/s", "cores": threads, "kind": "port",
+            "sample": f"oracle price_gbm restatement (f64 paths, recursive AST evaluation) on every 25th contract of "
+                      f"the book (40 contracts, 10 per kind) at {n_paths} paths each, {ps:.3e} path-steps in "
+                      f"{dt:.1f} s; prices only (the reference has no pathwise Greeks)",
+            "cpu_model": model, "affinity_cores": affinity,
+            "threads_note": "threads = the host's CPU share per GPU (OMP_NUM_THREADS), capped by the affinity set"}
+
+
+# ---------------------------------------------------------------------------
+# Config 2: up-and-out CRN Greeks
+# ---------------------------------------------------------------------------
+def bench_config2(args, eng, K, N, R, threads):
+    contract = K.up_and_out_call("X", 100.0, 200.0, 1.0)
+    model = K.GBM(180.0, 0.25, 0.05, asset="X")
+    n, steps = 4_194_304, 252
+
+    def step():
+        return contract.greeks(model, n_paths=n, seed=42, steps_per_year=steps, bumps=(1.8, 0.01, 1e-4))
+
+    # the engine clock ramps up over ~40 launches of this kernel from idle
+    for _ in range(40):
+        step()
+    g, wall, dev_ms, launches, _ = timed_steps(eng, step, 20)
+    fma = valu_probe(N, eng)
+    gen = rng_probe(N, eng, True)
+    avg_launch_ms = dev_ms / max(launches, 1)
+    lane_ops = R.gbm_barrier(scenarios=7, states=5)
+    per_launch = float(n) * steps
+    rate = per_launch / (avg_launch_ms / 1e3)
+    ceiling = 1.0 / (1.0 / gen["normals_per_s"] + (lane_ops - R.NORMAL) / fma["fma_lane_ops_per_s"])
+    rl = roof(per_launch * lane_ops, avg_launch_ms, "path_kernel<GBM, 7 CRN scenarios>",
+              {"lane_ops_per_path_step": lane_ops, "path_steps_per_launch": per_launch,
+               "traffic": traffic_from_profiles("path_kernel_gbm_barrier_traffic.json"),
+               "fma_issue_measured": round(fma["fma_lane_ops_per_s"] / 1e12, 3),
+               "frac_of_measured_fma_issue": round(per_launch * lane_ops / (avg_launch_ms / 1e3)
+                                                   / fma["fma_lane_ops_per_s"], 4),
+               "normals_measured": round(gen["normals_per_s"] / 1e12, 4),
+               "mix_ceiling_path_steps_per_s": round(ceiling, 1),
+               "frac_of_mix_ceiling": round(rate / ceiling, 4)})
+    value = 7.0 * n * steps * 20 / wall
+    log(f"config2: {1e3 * wall / 20:.3f} ms/step, {value:.4e} scenario path-steps/s, kernel {avg_launch_ms:.3f} ms, "
+        f"frac {rl['frac']}")
+    out = {"workload": "up-and-out call (S-100)^+ KO at S>=200, GBM s0=180 sigma=0.25 r=0.05, 4,194,304 paths x 252 "
+                       "steps, CRN bump Greeks (7 scenario pricings sharing the normals)",
+           "value": value, "unit": "scenario path-steps/s", "ms_per_step": 1e3 * wall / 20,
+           "pricing_path_steps_per_s": n * steps * 20 / wall, "steps": 20, "warmup": 40, "roofline": rl,
+           "greeks": {"price": g.price, "delta": g.delta, "gamma": g.gamma, "vega": g.vega, "rho": g.rho,
+                      "std_errors": list(g.std_errors)}}
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import oracle_lib as O
+        npaths = args.cpu_paths
+        t0 = time.perf_counter()
+        O.greeks_gbm(contract.to_json(), "X", 180.0, 0.25, O.cfg(npaths, 42, 252, 0.05, threads=threads),
+                     1.8, 0.01, 1e-4)
+        dt = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": round(7 * npaths * 252 / dt, 1), "unit": "scenario path-steps/s",
+                               "cores": threads, "kind": "port",
+                               "sample": f"oracle greeks_gbm (7 CRN pricings, f64) on {npaths} paths x 252 steps, "
+                                         f"{dt:.1f} s"}
+    return out
 
 
 def valu_probe(N, eng, reps=20, iters=4096):
@@ -66,8 +273,7 @@ def valu_probe(N, eng, reps=20, iters=4096):
 
 
 def rng_probe(N, eng, normals, reps=20, iters=256):
-    """Rate of the normal generator alone: Philox blocks with the path kernels'
-    counter layout, folded as raw words or as their FP32 inverse normals."""
+    """Rate of the normal generator alone."""
     import ctypes as C
     ms, words = C.c_double(), C.c_double()
     N.check(N.lib().kt_debug_rng_rate(eng.handle, 1 if normals else 0, iters, reps, C.byref(ms), C.byref(words)),
@@ -76,351 +282,138 @@ def rng_probe(N, eng, normals, reps=20, iters=256):
             "avg_ms": ms.value, ("normals_per_s" if normals else "words_per_s"): words.value}
 
 
+# ---------------------------------------------------------------------------
+# Config 3: Heston pathwise Greeks vs bump-and-reprice at equal stderr
+# ---------------------------------------------------------------------------
+def bench_config3(eng, K, R):
+    c = K.european_call("X", 100.0, 1.0)
+    hm = K.heston(100, 0.04, 2.0, 0.04, 0.3, -0.5, 0.05, asset="X")
+    n = 2_097_152
+    hb = [1.0, 1e-3, 1e-2, 1e-3, 1e-2, 1e-2]
+    rf, tf, kf = median_call(eng, lambda: c.price(hm, n_paths=n, steps_per_year=365))
+    gA, tA, kA = median_call(eng, lambda: c.pathwise_greeks(hm, n_paths=n, steps_per_year=365))
+    _, tB, kB = median_call(eng, lambda: c.bump_greeks(hm, bumps=hb, n_paths=n, steps_per_year=365,
+                                                       std_errors=False))
+    gB = c.bump_greeks(hm, bumps=hb, n_paths=n, steps_per_year=365)
+    eq = {nm: (tB / tA) * (sb / sa) ** 2 for nm, sa, sb in zip(gA.names, gA.std_errors, gB.std_errors)}
+    eng.set_option("pathwise_mode", 1)
+    try:
+        gT, tT, kT = median_call(eng, lambda: c.pathwise_greeks(hm, n_paths=n, steps_per_year=365))
+    finally:
+        eng.set_option("pathwise_mode", 0)
+    eqT = {nm: (tB / tT) * (sb / sa) ** 2 for nm, sa, sb in zip(gT.names, gT.std_errors, gB.std_errors)}
+    ps = float(n) * 365
+    log(f"config3: pathwise {tA * 1e3:.2f} ms vs 13 bumps {tB * 1e3:.2f} ms: {tB / tA:.2f}x raw, "
+        f"min {min(eq.values()):.2f}x at equal stderr")
+    return {
+        "workload": "Heston call s0=K=100 v0=theta=0.04 kappa=2 sigma_v=0.3 rho=-0.5 r=0.05 T=1, 2,097,152 paths x "
+                    "365 steps; d price / d {s0, v0, kappa, theta, sigma_v, rho}: pathwise reverse sweep with a "
+                    "checkpoint every 16 steps vs 13 CRN bump pricings",
+        "timing": "median of 5 calls after a warm one",
+        "forward_ms": tf * 1e3, "forward_kernel_ms": kf, "forward_path_steps_per_s": ps / tf,
+        "forward_roofline": roof(ps * R.heston_european(), kf, "path_kernel<Heston>"),
+        "pathwise_ms": tA * 1e3, "pathwise_kernel_ms": kA,
+        "pathwise_roofline": roof(ps * R.heston_pathwise(), kA, "adjoint_heston_kernel (forward, recompute, sweep)"),
+        "bump13_ms": tB * 1e3, "bump13_kernel_ms": kB,
+        "speedup_raw": tB / tA, "speedup_equal_stderr": eq, "speedup_equal_stderr_min": min(eq.values()),
+        "pathwise": dict(zip(gA.names, gA.values)), "pathwise_se": dict(zip(gA.names, gA.std_errors)),
+        "bump": dict(zip(gB.names, gB.values)), "bump_se": dict(zip(gB.names, gB.std_errors)),
+        "price": gA.price, "price_se": gA.price_std_error,
+        "tangent_ms": tT * 1e3, "tangent_kernel_ms": kT, "tangent_speedup_equal_stderr": eqT,
+        "greeks_per_s_pathwise": 6 / tA, "greeks_per_s_bump": 6 / tB,
+    }
+
+
+# ---------------------------------------------------------------------------
+# Config 1 and config 5
+# ---------------------------------------------------------------------------
+def bench_config1(eng, K, R):
+    grid = [i / 252 for i in range(253)]
+    c1 = K.european_call("X", 100.0, 1.0)
+    m1 = K.GBM(100.0, 0.2, 0.05, asset="X")
+    r1, t1, k1 = median_call(eng, lambda: c1.price(m1, n_paths=1_000_000, grid=grid))
+    ps = 1_000_000 * 252.0
+    return {"workload": "European call GBM s0=K=100 sigma=0.2 r=0.05 T=1, 1,000,000 paths x 252 steps (explicit "
+                        "grid), seed 42", "price": r1.price, "std_error": r1.std_error,
+            "black_scholes": 10.450583572185565, "ms": t1 * 1e3, "kernel_ms": k1, "path_steps_per_s": ps / t1,
+            "roofline": roof(ps * R.gbm_european(), k1, "path_kernel<GBM>")}
+
+
+def bench_config5(eng, K):
+    mL = K.GBM(36.0, 0.2, 0.06, asset="X")
+    put = K.scale((K.const_(40.0) - K.spot("X")).max(0.0), K.one("USD"))
+    dates = [i / 50 for i in range(1, 51)]
+    out = {"workload": "American put by LSM, GBM s0=36 K=40 sigma=0.2 r=0.06 T=1, 50 exercise dates, basis "
+                       "{1, S, S^2}, paths stored [date][path] in HBM", "timing": "median of 5 calls after a warm one"}
+    for label, nL in (("n_1048576", 1 << 20), ("n_8388608", 1 << 23)):
+        rL, tL, kL = median_call(eng, lambda: put.price_american(mL, dates, n_paths=nL, seed=42))
+        # algorithmic bytes: the paths kernel writes payoff + spot (8 B per
+        # path and date); each backward pass reads the payoff/spot pairs of two
+        # dates and reads + writes the f64 value (32 B per path and date)
+        nbytes = nL * len(dates) * (8 + 32)
+        out[label] = {"paths": nL, "price": rL.price, "std_error": rL.std_error, "ms": tL * 1e3, "device_ms": kL,
+                      "path_dates_per_s": nL * len(dates) / tL, "algorithmic_bytes": nbytes,
+                      "roofline": {"bound": "hbm", "achieved": round(nbytes / (kL / 1e3) / 1e9, 1),
+                                   "peak": 8000.0, "unit": "GB/s",
+                                   "frac": round(nbytes / (kL / 1e3) / 8e12, 4)}}
+    out["reference_ls2001"] = {"lsm": 4.472, "se": 0.010, "fd": 4.478}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
-    # the engine clock ramps up over the first ~40 launches from idle
-    # (1.3 ms -> 1.05 ms per launch of this kernel): warm up past that
-    ap.add_argument("--warmup", type=int, default=40)
-    ap.add_argument("--paths", type=int, default=4_194_304)
-    ap.add_argument("--cpu-paths", type=int, default=262_144)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--cpu-paths", type=int, default=262_144, help="config-2 CPU sample paths")
+    ap.add_argument("--cpu-paths-book", type=int, default=4096, help="paths per contract of the book's CPU sample")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--extras", action="store_true", help="also time configs 1 and 3 (single pricings)")
+    ap.add_argument("--only", choices=["book", "config2", "config3", "config1", "config5"], default=None,
+                    help="run one workload only (profiling runs)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run the plain copy kernel on MIB MiB (profiler byte-counter calibration)")
     ap.add_argument("--valu-calibration", action="store_true",
                     help="only run the FMA issue-rate kernel (eight independent FMA chains per lane)")
     args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU: this build has no multi-GPU path (paths would shard by index with one "
+              "f64 statistics reduction per contract); use --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    from kontracts_amd import _native as N
+    eng = N.engine(0)
     if args.valu_calibration:
-        from kontracts_amd import _native as N
-        eng = N.engine(0)
         print(json.dumps(valu_probe(N, eng, reps=50)))
         print(json.dumps(rng_probe(N, eng, False, reps=50)))
         print(json.dumps(rng_probe(N, eng, True, reps=50)))
         return
     if args.copy_calibration:
         import ctypes as C
-        from kontracts_amd import _native as N
-        eng = N.engine(0)
         ms = C.c_double()
         nbytes = args.copy_calibration << 20
         N.check(N.lib().kt_debug_copy(eng.handle, nbytes, 5, C.byref(ms)), eng.handle)
         print(json.dumps({"copy_bytes": nbytes, "reps": 5, "avg_ms": ms.value,
                           "GBps": 2 * nbytes / (ms.value / 1e3) / 1e9}))
         return
-    if args.gpus != 1:
-        print("bench.py runs on one GPU: this build has no multi-GPU path (paths would shard by index with one "
-              "f64 statistics reduction per contract); use --gpus 1", file=sys.stderr)
-        sys.exit(2)
 
     import kontracts_amd as K
-    from kontracts_amd import _native as N
+    from kontracts_amd import book as B
     from kontracts_amd import roofline as R
+    from kontracts_amd import tune as T
 
-    eng = N.engine(0)
-    contract = K.up_and_out_call("X", 100.0, 200.0, 1.0)
-    model = K.GBM(180.0, 0.25, 0.05, asset="X")
-    bumps = (1.8, 0.01, 1e-4)
-    n, steps = args.paths, 252
-
-    def one_step():
-        return contract.greeks(model, n_paths=n, seed=42, steps_per_year=steps, bumps=bumps)
-
-    for _ in range(args.warmup):
-        one_step()
-    eng.set_profiling(True)
-    eng.kernel_time(reset=True)
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        g = one_step()
-    t1 = time.perf_counter()
-    k_ms, k_launches, k_path_steps = eng.kernel_time(reset=True)
-    eng.set_profiling(False)
-    fma = valu_probe(N, eng)  # right after the timed steps, at the same clocks
-    gen = rng_probe(N, eng, True)
-
-    wall = t1 - t0
-    ms_per_step = 1e3 * wall / args.steps
-    scen_path_steps = 7.0 * n * steps
-    value = scen_path_steps * args.steps / wall
-    log(f"greeks: price {g.price:.6f} delta {g.delta:.6f} gamma {g.gamma:.6e} vega {g.vega:.6f} rho {g.rho:.6f}")
-    log(f"{ms_per_step:.3f} ms/step, {value:.4e} scenario path-steps/s; kernels {k_launches} launches "
-        f"{k_ms / max(k_launches, 1):.3f} ms avg")
-
-    # One launch per step: the 7-scenario CRN kernel (5 distinct log-spot paths).
-    avg_launch_s = (k_ms / max(k_launches, 1)) / 1e3
-    path_steps_per_launch = float(n) * steps
-    lane_ops = R.gbm_barrier(scenarios=7, states=5)
-    achieved = path_steps_per_launch * lane_ops / avg_launch_s / 1e12 if avg_launch_s > 0 else 0.0
-    peak = R.PEAK_LANE_OPS / 1e12
-    roof = {"bound": "valu", "achieved": round(achieved, 3), "peak": round(peak, 3), "unit": "Tlane-op/s",
-            "frac": round(achieved / peak, 4), "traffic": traffic_from_profiles(),
-            "kernel": "path_kernel<GBM, 7 CRN scenarios>", "avg_launch_ms": round(avg_launch_s * 1e3, 4),
-            "launches": k_launches, "lane_ops_per_path_step": lane_ops,
-            "path_steps_per_launch": path_steps_per_launch,
-            "peak_basis": "FP32 vector 157.3 TFLOP/s = 78.6e12 FMA lane-ops/s; lane-ops are the minimal gfx950 "
-                          "instruction sequence per path-step (kontracts_amd/roofline.py); no MFMA: no step is a "
-                          "dense contraction, the work is Philox integer ops, inverse normals and FMAs",
-            "fma_issue_measured": round(fma["fma_lane_ops_per_s"] / 1e12, 3),
-            "frac_of_measured_fma_issue": round(achieved * 1e12 / fma["fma_lane_ops_per_s"], 4)}
-    # Mix-limited ceiling: one normal at the generator's measured rate plus the
-    # rest of the path-step at the measured FMA rate (Philox's 64-bit multiplies
-    # and the transcendentals take several issue cycles, which lane-ops do not
-    # weight).
-    rate = path_steps_per_launch / avg_launch_s if avg_launch_s > 0 else 0.0
-    ceiling = 1.0 / (1.0 / gen["normals_per_s"] + (lane_ops - R.NORMAL) / fma["fma_lane_ops_per_s"])
-    roof.update({"normals_measured": round(gen["normals_per_s"] / 1e12, 4),
-                 "mix_ceiling_path_steps_per_s": round(ceiling, 1),
-                 "frac_of_mix_ceiling": round(rate / ceiling, 4)})
-
-    cpu = None
-    if not args.no_cpu:
-        threads = min(len(os.sched_getaffinity(0)), 16)
-        cv, dt = cpu_baseline(args.cpu_paths, threads)
-        cpu = {"value": round(cv, 1), "unit": "path-steps/s", "cores": threads, "kind": "port",
-               "sample": f"oracle greeks_gbm restatement (7 CRN pricings, f64, recursive AST evaluation) on "
-                         f"{args.cpu_paths} paths x 252 steps of the same contract, {dt:.1f} s"}
-        log(f"cpu baseline {cv:.4e} path-steps/s on {threads} threads ({dt:.1f} s)")
-
-    extras = {}
-    if args.extras:
-
-        def timed(fn, reps=5):
-            """Median wall time and median path-kernel time of reps calls (after one warm call)."""
-            out = fn()
-            walls, kms = [], []
-            for _ in range(reps):
-                eng.set_profiling(True)
-                eng.kernel_time(reset=True)
-                t = time.perf_counter()
-                out = fn()
-                walls.append(time.perf_counter() - t)
-                kms.append(eng.kernel_time(reset=True)[0])
-                eng.set_profiling(False)
-            return out, statistics.median(walls), statistics.median(kms)
-
-        extras["timing"] = "single pricings and Greeks: median of 5 calls each, after a warm call"
-        grid = [i / 252 for i in range(253)]
-        c1 = K.european_call("X", 100.0, 1.0)
-        m1 = K.GBM(100.0, 0.2, 0.05, asset="X")
-        r1, t1, k1 = timed(lambda: c1.price(m1, n_paths=1_000_000, grid=grid))
-        extras["config1_european_gbm_path_steps_per_s"] = 1_000_000 * 252 / t1
-        extras["config1_european_gbm_kernel_path_steps_per_s"] = 1_000_000 * 252 / (k1 / 1e3)
-        extras["config1_price"] = r1.price
-        h = K.heston(100, 0.04, 2.0, 0.04, 0.3, -0.5, 0.05, asset="X")
-        r3, t3, k3 = timed(lambda: c1.price(h, n_paths=2_097_152, steps_per_year=365))
-        extras["config3_heston_forward_path_steps_per_s"] = 2_097_152 * 365 / t3
-        extras["config3_heston_forward_kernel_ms"] = k3
-        extras["config3_price"] = r3.price
-        fs = [K.GbmFactor(f"A{i}", 100.0, 0.05, 0.15 + 0.05 * i) for i in range(5)]
-        b5 = K.correlated_gbm(fs, [[1.0 if i == j else 0.3 for j in range(5)] for i in range(5)], 0.05)
-        avg5 = (K.spot("A0") + K.spot("A1") + K.spot("A2") + K.spot("A3") + K.spot("A4")) / 5.0
-        cb = K.scale((avg5 - 100.0).clip(0.0), K.one("USD")) @ K.at(1.0)
-        rb, tb, kb = timed(lambda: cb.price(b5, n_paths=1 << 20, grid=grid))
-        extras["basket5_path_steps_per_s"] = (1 << 20) * 252 / tb
-        extras["basket5_kernel_ms"] = kb
-        extras["basket5_price"] = rb.price
-        ca = K.scale((K.average(K.spot("X")) - 100.0).clip(0.0), K.one("USD")) @ K.at(1.0)
-        ra, ta, ka = timed(lambda: ca.price(m1, n_paths=1 << 20, steps_per_year=252))
-        extras["asian_daily_path_steps_per_s"] = (1 << 20) * 252 / ta
-        extras["asian_daily_kernel_ms"] = ka
-        extras["asian_price"] = ra.price
-        # American put by LSM: s0=36, K=40, sigma=0.2, r=0.06, T=1, 50 dates,
-        # 2^20 paths stored in HBM.  Algorithmic HBM bytes: the paths kernel
-        # writes payoff + spot (8 B per path and date); each backward pass reads
-        # payoff/spot of two dates (16 B) and reads + writes the f64 value (16 B).
-        mL = K.GBM(36.0, 0.2, 0.06, asset="X")
-        put = K.scale((K.const_(40.0) - K.spot("X")).max(0.0), K.one("USD"))
-        dates = [i / 50 for i in range(1, 51)]
-        nL = 1 << 20
-        put.price_american(mL, dates, n_paths=nL, seed=42)
-        eng.set_profiling(True)
-        eng.kernel_time(reset=True)
-        t = time.perf_counter()
-        rL = put.price_american(mL, dates, n_paths=nL, seed=42)
-        wallL = time.perf_counter() - t
-        kL_ms, _, _ = eng.kernel_time(reset=True)
-        eng.set_profiling(False)
-        bytesL = nL * len(dates) * (8 + 32)
-        extras["config5_lsm_ms"] = wallL * 1e3
-        extras["config5_lsm_device_ms"] = kL_ms
-        extras["config5_lsm_path_steps_per_s"] = nL * len(dates) / wallL
-        extras["config5_lsm_algorithmic_GBps"] = bytesL / (kL_ms / 1e3) / 1e9 if kL_ms > 0 else None
-        extras["config5_price"] = rL.price
-        extras["config5_std_error"] = rL.std_error
-        # Config 3 Greeks: Heston 2^21 paths x 365 steps, d price / d {s0, v0,
-        # kappa, theta, sigma_v, rho}: pathwise (one forward + adjoint launch)
-        # vs CRN bump-and-reprice (13 pricings); then at equal standard error:
-        # paths needed scale with the variance, so the speedup per Greek is
-        # (t_bump / t_pathwise) * (se_bump / se_pathwise)^2.
-        hm = K.heston(100, 0.04, 2.0, 0.04, 0.3, -0.5, 0.05, asset="X")
-        n3 = 2_097_152
-        hb = [1.0, 1e-3, 1e-2, 1e-3, 1e-2, 1e-2]
-
-        gA, tA, kA_ms = timed(lambda: c1.pathwise_greeks(hm, n_paths=n3, steps_per_year=365))
-        _, tB, kB_ms = timed(lambda: c1.bump_greeks(hm, bumps=hb, n_paths=n3, steps_per_year=365, std_errors=False))
-        gB = c1.bump_greeks(hm, bumps=hb, n_paths=n3, steps_per_year=365)
-        eq = {nm: (tB / tA) * (sb / sa) ** 2 for nm, sa, sb in zip(gA.names, gA.std_errors, gB.std_errors)}
-        extras["config3_pathwise_ms"] = tA * 1e3
-        extras["config3_pathwise_kernel_ms"] = kA_ms
-        extras["config3_bump13_ms"] = tB * 1e3
-        extras["config3_bump13_kernel_ms"] = kB_ms
-        extras["config3_speedup_raw"] = tB / tA
-        extras["config3_speedup_equal_stderr"] = eq
-        extras["config3_pathwise"] = dict(zip(gA.names, gA.values))
-        extras["config3_pathwise_se"] = dict(zip(gA.names, gA.std_errors))
-        extras["config3_bump"] = dict(zip(gB.names, gB.values))
-        extras["config3_bump_se"] = dict(zip(gB.names, gB.std_errors))
-        # The same six Greeks with the variance-model derivatives carried
-        # forward as tangents (no checkpoints, no reverse sweep).
-        eng.set_option("pathwise_mode", 1)
-        try:
-            gT, tT, kT_ms = timed(lambda: c1.pathwise_greeks(hm, n_paths=n3, steps_per_year=365))
-        finally:
-            eng.set_option("pathwise_mode", 0)
-        extras["config3_tangent_ms"] = tT * 1e3
-        extras["config3_tangent_kernel_ms"] = kT_ms
-        extras["config3_tangent_speedup_raw"] = tB / tT
-        extras["config3_tangent_speedup_equal_stderr"] = {
-            nm: (tB / tT) * (sb / sa) ** 2 for nm, sa, sb in zip(gT.names, gT.std_errors, gB.std_errors)}
-        extras["config3_tangent_max_rel_diff"] = max(
-            abs(a - b) / max(abs(b), 1e-12) for a, b in zip(gT.values, gA.values))
-        # Config 4: the 1,000-contract book in one persistent launch vs one
-        # launch per contract (the reference's loop of price_gbm calls).
-        from kontracts_amd import book as B
-        from kontracts_amd import tune as T
-        jobs = B.book(1000)
-        tuned = T.autotune_portfolio(B.book(200), repeats=1)
-        extras["config4_autotune"] = {"best": list(tuned["best"]),
-                                      "device_ms": {f"{k[0]}x{k[1]}": v for k, v in tuned["device_ms"].items()}}
-        K.price_portfolio(jobs)
-        t = time.perf_counter()
-        rp = K.price_portfolio(jobs)
-        tp = time.perf_counter() - t
-        t = time.perf_counter()
-        rs = [job[0].price(job[1], n_paths=job[2], seed=job[3], steps_per_year=job[4], grid=job[5]) for job in jobs]
-        ts = time.perf_counter() - t
-        extras["config4_path_steps"] = B.path_steps(jobs)
-        extras["config4_portfolio_s"] = tp
-        extras["config4_per_contract_s"] = ts
-        extras["config4_speedup"] = ts / tp
-        extras["config4_portfolio_path_steps_per_s"] = B.path_steps(jobs) / tp
-        extras["config4_identical_to_per_contract"] = all(a.price == b.price and a.std_error == b.std_error
-                                                          for a, b in zip(rp, rs))
-        # the book with pathwise Greeks on its Lipschitz payoffs (European,
-        # Asian, basket): persistent Greeks launches + the price-only launch
-        # for the barriers, vs per contract pathwise Greeks / a price
-        K.price_portfolio(jobs, greeks=True)
-        t = time.perf_counter()
-        rg = K.price_portfolio(jobs, greeks=True)
-        tpg = time.perf_counter() - t
-        t = time.perf_counter()
-        sg = []
-        for i, job in enumerate(jobs):
-            kw = dict(n_paths=job[2], seed=job[3], steps_per_year=job[4], grid=job[5])
-            if B.KINDS[i % 4] == "barrier":
-                sg.append((job[0].price(job[1], **kw), None))
-            else:
-                sg.append((None, job[0].pathwise_greeks(job[1], **kw)))
-        tsg = time.perf_counter() - t
-        extras["config4_greeks_portfolio_s"] = tpg
-        extras["config4_greeks_per_contract_s"] = tsg
-        extras["config4_greeks_speedup"] = tsg / tpg
-        extras["config4_greeks_identical_to_per_contract"] = all(
-            (a[1] is None and a[0].price == b[0].price) or (a[1] is not None and a[1].values == b[1].values)
-            for a, b in zip(rg, sg))
-        # launch-bound regime: a book of the same kinds at 2^14 paths and 3-month maturities (63 steps)
-        small = B.book(1000, paths_exp=(14, 14), maturity=(0.25, 0.25))
-        K.price_portfolio(small)
-        t = time.perf_counter()
-        K.price_portfolio(small)
-        tps = time.perf_counter() - t
-        t = time.perf_counter()
-        for job in small:
-            job[0].price(job[1], n_paths=job[2], seed=job[3], steps_per_year=job[4], grid=job[5])
-        tss = time.perf_counter() - t
-        extras["config4_small_book_speedup"] = tss / tps
-        extras["config4_small_book_portfolio_ms"] = tps * 1e3
-        K.price_portfolio(small, greeks=True)
-        t = time.perf_counter()
-        K.price_portfolio(small, greeks=True)
-        tpsg = time.perf_counter() - t
-        t = time.perf_counter()
-        for i, job in enumerate(small):
-            kw = dict(n_paths=job[2], seed=job[3], steps_per_year=job[4], grid=job[5])
-            if B.KINDS[i % 4] == "barrier":
-                job[0].price(job[1], **kw)
-            else:
-                job[0].pathwise_greeks(job[1], **kw)
-        tssg = time.perf_counter() - t
-        extras["config4_small_book_greeks_speedup"] = tssg / tpsg
-        extras["config4_small_book_greeks_portfolio_ms"] = tpsg * 1e3
-        # Calibration-sized calls (1,000 paths x 50 steps): per-call latency of
-        # a single price, and one objective evaluation over 10 quotes as one
-        # portfolio call (direct launches, then graph replay) vs a loop.
-        ko = K.up_and_out_call("X", 100.0, 130.0, 1.0)
-        for _ in range(20):
-            ko.price(K.GBM(100.0, 0.2, 0.05, asset="X"), n_paths=1000, seed=1)
-        t = time.perf_counter()
-        for i in range(200):
-            ko.price(K.GBM(100.0, 0.2 + 1e-4 * i, 0.05, asset="X"), n_paths=1000, seed=1)
-        extras["small_call_price_us"] = (time.perf_counter() - t) / 200 * 1e6
-        quotes = [(ko, K.GBM(s, 0.2, 0.05, asset="X"), 1000, 1, 50) for s in range(90, 110, 2)]
-        from kontracts_amd import _native as NN
-        eng = NN.engine(0)
-        for graph in (0, 1):
-            eng.set_option("portfolio_graph", graph)
-            for _ in range(10):
-                K.price_portfolio(quotes)
-            t = time.perf_counter()
-            for _ in range(100):
-                K.price_portfolio(quotes)
-            extras[f"objective_10_quotes_portfolio_{'graph' if graph else 'direct'}_us"] = \
-                (time.perf_counter() - t) / 100 * 1e6
-        eng.set_option("portfolio_graph", 0)
-        t = time.perf_counter()
-        for _ in range(20):
-            for q in quotes:
-                q[0].price(q[1], n_paths=q[2], seed=q[3], steps_per_year=q[4])
-        extras["objective_10_quotes_loop_us"] = (time.perf_counter() - t) / 20 * 1e6
-        # Greek surface: 20 x 20 (spot, vol) cells of the config-2 barrier at
-        # 2^16 paths x 252 steps, one launch vs a loop of greeks calls.
-        spots = [150.0 + 2.5 * i for i in range(20)]
-        vols = [0.15 + 0.01 * j for j in range(20)]
-        bar = K.up_and_out_call("X", 100, 200, 1.0)
-        K.greek_surface(bar, "X", spots[:2], vols[:2], n_paths=1 << 16, seed=42, steps_per_year=252)
-        eng.set_profiling(True)
-        eng.kernel_time(reset=True)
-        t = time.perf_counter()
-        K.greek_surface(bar, "X", spots, vols, n_paths=1 << 16, seed=42, steps_per_year=252)
-        tsurf = time.perf_counter() - t
-        ksurf_ms = eng.kernel_time(reset=True)[0]
-        eng.set_profiling(False)
-        extras["surface_20x20_kernel_ms"] = ksurf_ms
-        t = time.perf_counter()
-        for s_ in spots[:5]:
-            for v_ in vols:
-                bar.greeks(K.GBM(s_, v_, 0.05, asset="X"), n_paths=1 << 16, seed=42, steps_per_year=252)
-        tloop = (time.perf_counter() - t) * 4  # 100 of the 400 cells, scaled
-        extras["surface_20x20_s"] = tsurf
-        extras["surface_20x20_loop_s"] = tloop
-        extras["surface_scenario_path_steps_per_s"] = 400 * 7 * (1 << 16) * 252 / tsurf
-        # Variance reduction on a daily-average Asian call (1e6 paths x 252 steps)
-        call = K.scale((K.average(K.spot("X")) - 100.0).clip(0.0), K.one("USD")) @ K.at(1.0)  # Asian call
-        g252 = [i / 252 for i in range(253)]
-        call.price(m1, n_paths=1_000_000, seed=42, grid=g252)
-        t = time.perf_counter()
-        rv = call.price(m1, n_paths=1_000_000, seed=42, grid=g252)
-        extras["vr_none_ms"] = (time.perf_counter() - t) * 1e3
-        extras["vr_none_std_error"] = rv.std_error
-        for label, anti, cv in (("antithetic", True, False), ("control_variate", False, True), ("both", True, True)):
-            call.price(m1, n_paths=1_000_000, seed=42, grid=g252, antithetic=anti, control_variate=cv)
-            t = time.perf_counter()
-            rv = call.price(m1, n_paths=1_000_000, seed=42, grid=g252, antithetic=anti, control_variate=cv)
-            extras[f"vr_{label}_ms"] = (time.perf_counter() - t) * 1e3
-            extras[f"vr_{label}_std_error"] = rv.std_error
-
+    model, affinity, threads = cpu_info()
+    configs = {}
+    if args.only in (None, "config2"):
+        configs["config2"] = bench_config2(args, eng, K, N, R, threads)
+    if args.only in (None, "config3"):
+        configs["config3"] = bench_config3(eng, K, R)
+    if args.only in (None, "config1"):
+        configs["config1"] = bench_config1(eng, K, R)
+    if args.only in (None, "config5"):
+        configs["config5"] = bench_config5(eng, K)
+    if args.only not in (None, "book"):
+        print(json.dumps({"configs": configs}))
+        return
+    value, step_s, info, roofline, extra, cpu = bench_book(args, eng, K, B, R, T, threads)
     line = {
         "metric": METRIC,
         "value": value,
@@ -428,22 +421,19 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
+        "ms_per_step": step_s * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32 paths, f64 payoff/stderr reduction",
-        "data": "synthetic (contract + model + seed; counter-based Philox normals generated on the GPU)",
-        "config": {"workload": "up-and-out call (S-100)^+ KO at S>=200, GBM s0=180 sigma=0.25 r=0.05, "
-                               f"{n} paths x 252 steps, CRN bump Greeks (7 scenario pricings)",
-                   "paths": n, "steps": steps, "scenarios": 7, "parallelism": "single GPU"},
-        "roofline": roof,
+        "data": "synthetic (seeded contract book, models and seeds; counter-based Philox normals generated on the GPU)",
+        "config": info,
+        "roofline": roofline,
         "cpu_baseline": cpu,
-        "greeks": {"price": g.price, "delta": g.delta, "gamma": g.gamma, "vega": g.vega, "rho": g.rho,
-                   "price_std_error": g.std_errors[0]},
+        "book": extra,
+        "configs": configs,
+        "host": {"cpu_model": model, "affinity_cores": affinity, "threads_used": threads},
     }
-    if extras:
-        line["extras"] = extras
     print(json.dumps(line))
 
 

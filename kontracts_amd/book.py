@@ -90,3 +90,30 @@ def book(n_contracts: int = 1000, seed: int = 42, paths_exp: tuple = (16, 20),
 def path_steps(jobs) -> float:
     """Total paths x grid steps of a book."""
     return float(sum(job[2] * (len(job[5]) - 1) for job in jobs))
+
+
+def job_profile(index: int, job) -> tuple:
+    """(kind, model, grid steps, fixings) of job `index` of a book: kind as in
+    KINDS, model "gbm", "heston" or "basket5", fixings the number of
+    AverageAt dates (0 for the other kinds)."""
+    import json
+
+    con, model = job[0], job[1]
+    kind = KINDS[index % 4]
+    if isinstance(model, dsl.GBM):
+        name = "gbm"
+    elif model.kind == 1:
+        name = "heston"
+    else:
+        name = "basket5"
+
+    def fixings(node):
+        if isinstance(node, dict):
+            if "AverageAt" in node:
+                return len(node["AverageAt"]["dates"])
+            return sum(fixings(v) for v in node.values())
+        if isinstance(node, list):
+            return sum(fixings(v) for v in node)
+        return 0
+
+    return kind, name, len(job[5]) - 1, fixings(json.loads(con.to_json()))

@@ -657,6 +657,7 @@ class Lowerer {
       raise(KT_ERR_UNSUPPORTED, "contract needs " + std::to_string(prog.n_slots) +
                                     " registers; the path kernel holds " + std::to_string(kMaxSlots));
     prog.time_only = time_only_;
+    prog.time_payload = time_payload_;
     return prog;
   }
 
@@ -670,6 +671,7 @@ class Lowerer {
   int temp_base_ = 0;
   int max_temp_ = 0;
   bool time_only_ = true;
+  double time_payload_ = __builtin_nan("");
   std::vector<Event> events_;
   std::vector<Temporal> temporals_;
   std::unordered_map<const Obs*, int> temporal_index_;
@@ -950,6 +952,7 @@ class Lowerer {
         for (int i = 0; i < n_ && !on_grid; ++i) on_grid = std::fabs(grid_[i] - d) <= 1e-12;
         if (!on_grid) {
           g.pending.push_back(KT_ERR_TIME_OUT_OF_RANGE);
+          if (std::isnan(time_payload_)) time_payload_ = d;
           return kSlotZero;
         }
         ++count;

@@ -72,6 +72,9 @@ struct Program {
   int n_outputs = 1;           // pv accumulators used
   bool time_only = true;       // no price-dependent condition anywhere
   int max_ops_per_step = 0;
+  // TimeOutOfRange's payload (KontractError::TimeOutOfRange(f64)): the first
+  // AverageAt fixing date found off the grid, NaN if none
+  double time_payload = __builtin_nan("");
   std::string describe() const;
 };
 

@@ -171,6 +171,16 @@ const char* error_text(int code) {
   }
 }
 
+std::string error_text(int code, const Program& prog) {
+  std::string s = error_text(code);
+  if (code == KT_ERR_TIME_OUT_OF_RANGE && !std::isnan(prog.time_payload)) {
+    char buf[64];
+    std::snprintf(buf, sizeof buf, ": %.17g", prog.time_payload);
+    s += buf;
+  }
+  return s;
+}
+
 }  // namespace kt
 
 // ---------------------------------------------------------------------------
@@ -484,7 +494,7 @@ std::vector<KStats> run_program_scen(kt_engine* E, const Program& prog, const st
   memcpy(&err_bits, rb + n_used * sizeof(KStats), sizeof(unsigned));
   if (err_bits) {
     int code = lowest_error_code(err_bits);
-    raise(code, error_text(code));
+    raise(code, error_text(code, prog));
   }
   return out;
 }

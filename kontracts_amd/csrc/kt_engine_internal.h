@@ -113,6 +113,9 @@ void check_hip(hipError_t e, const char* what);
 int set_error(kt_engine* E, int code, const std::string& msg);
 int lowest_error_code(unsigned bits);
 const char* error_text(int code);
+// error_text with the program's payload: TimeOutOfRange names the fixing
+// date ("...: <date>"), as KontractError::TimeOutOfRange(f64) carries it.
+std::string error_text(int code, const Program& prog);
 
 // allow_vr: the caller handles cfg's variance-reduction flags; otherwise
 // they are rejected.

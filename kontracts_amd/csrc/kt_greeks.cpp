@@ -268,7 +268,7 @@ void greek_surface(kt_engine* E, const char* json, const char* asset, const doub
     check_hip(hipStreamSynchronize(E->stream), "stream sync");
     if (err_bits) {
       int code = lowest_error_code(err_bits);
-      raise(code, error_text(code));
+      raise(code, error_text(code, prog));
     }
     for (uint64_t i = 0; i < nc; ++i) fill_greeks(res, i * kScenarioStats, &out[order[c0 + i]]);
   }

@@ -156,7 +156,7 @@ void price_american(kt_engine* E, const char* json, const double* dates, uint64_
   check_hip(hipStreamSynchronize(E->stream), "stream sync");
   if (err_bits) {
     int code = lowest_error_code(err_bits);
-    raise(code, error_text(code));
+    raise(code, error_text(code, prog));
   }
   for (int d = 0; d + 1 < D && coef_host; ++d) {
     for (int j = 0; j < m; ++j) coef_host[(size_t)d * (m + 1) + j] = c_host[(size_t)d * (kLsmBasisMax + 1) + j];

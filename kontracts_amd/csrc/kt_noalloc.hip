@@ -147,7 +147,7 @@ extern "C" kt_status kt_price_noalloc(kt_engine* E, const char* json, const kt_m
   std::memcpy(&err_bits, &rb.st[n_used], sizeof(unsigned));
   if (err_bits) {
     const int code = lowest_error_code(err_bits);
-    raise(code, error_text(code));
+    raise(code, error_text(code, prog));
   }
   *out = result_from_stats(rb.st[0]);
   E->err.clear();

@@ -270,7 +270,8 @@ void price_portfolio(kt_engine* E, const kt_job* jobs, uint64_t n_jobs, kt_price
       if (j > first_err) break;
       out[j] = price_solo(j);
     }
-    if (first_code) raise(first_code, error_text(first_code) + std::string(" (job ") + jid(first_err) + ")");
+    if (first_code)
+      raise(first_code, error_text(first_code, plans[first_err].prog) + std::string(" (job ") + jid(first_err) + ")");
     for (size_t i = 0; i < book.size(); ++i) out[book[i]] = result_from_stats(st[i]);
     return;
   }

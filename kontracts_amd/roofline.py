@@ -64,3 +64,54 @@ def basket_european(k: int) -> float:
 
 def gbm_asian() -> float:
     return NORMAL + 2 + 2 + 4
+
+
+# Pathwise-Greek and book counts.  A fixing point of a running sum adds the
+# spot read and the Kahan update (FIXING); with pathwise Greeks it adds the
+# spot's sensitivities into the accumulator's (GBM: d/ds0, d/dsigma, d/dmu,
+# 4 FMAs) or, under Heston, the reverse sweep's seed (exp2, mul, 2 FMAs).
+FIXING = 2 + 4
+FIXING_PATHWISE = FIXING + 4
+# Heston reverse sweep per point, no running accumulators: vp = sv^2, the
+# correlated normal (2), the floor's cut (2), the adjoint of the log step and
+# of the variance step (6), the four parameter accumulators (1 + 3 + 3 + 4),
+# d sqrt(v) / dv through rcp (3) and its cut (2), vbar (2) = 28; with running
+# accumulators the sweep also walks the log-spot back (4).
+HESTON_REVERSE = 28.0
+HESTON_REVERSE_FEEDS = 4.0
+
+
+def heston_barrier() -> float:
+    return heston_european() + 0.5
+
+
+def gbm_pathwise() -> float:
+    """Price plus d/d{s0, sigma, mu}: the Brownian sum W += sqrt(dt) w per point."""
+    return gbm_european() + 1
+
+
+def basket_pathwise(k: int) -> float:
+    return basket_european(k) + k
+
+
+def heston_pathwise(feeds: bool = False) -> float:
+    """Forward pass, recompute of every segment from its checkpoint (both
+    normals regenerated) and the reverse sweep."""
+    return 2 * heston_european() + HESTON_REVERSE + (HESTON_REVERSE_FEEDS if feeds else 0.0)
+
+
+def book_job(kind: str, model: str, n_steps: int, n_fixings: int = 0, greeks: bool = False) -> float:
+    """Lane-ops per path-step of one job of the 1,000-contract book
+    (kontracts_amd.book): kind european / asian / barrier / basket, model
+    gbm / heston / basket5; with greeks the Lipschitz kinds run the pathwise
+    kernels and the barriers the price-only kernel."""
+    fix = 0.0
+    if kind == "asian":
+        fix = n_fixings * (FIXING_PATHWISE if greeks else FIXING) / max(n_steps, 1)
+    if kind == "barrier":
+        return (gbm_barrier() if model == "gbm" else heston_barrier())
+    if kind == "basket":
+        return basket_pathwise(5) if greeks else basket_european(5)
+    if model == "gbm":
+        return (gbm_pathwise() if greeks else gbm_european()) + fix
+    return (heston_pathwise(feeds=kind == "asian") if greeks else heston_european()) + fix

@@ -18,24 +18,26 @@ DEFAULT_CANDIDATES = ((0, 1), (0, 4), (0, 16), (8, 1), (4, 1), (4, 4))
 
 
 def autotune_portfolio(jobs: Sequence[tuple], candidates=DEFAULT_CANDIDATES, repeats: int = 2,
-                       device: int | None = None) -> dict:
-    """Returns {"best": (blocks_per_cu, chunk), "device_ms": {candidate: ms}}
-    and sets the best candidate on the engine (blocks_per_cu 0 = occupancy)."""
+                       device: int | None = None, greeks: bool = False) -> dict:
+    """Returns {"best": (blocks_per_cu, chunk), "device_ms": {"bpcxchunk": ms}}
+    and sets the best candidate on the engine (blocks_per_cu 0 = occupancy).
+    greeks: time the book's call with pathwise Greeks (price_portfolio(...,
+    greeks=True)), whose persistent Greeks launches take the same knobs."""
     eng = N.engine(device)
     jobs = list(jobs)
     timings = {}
     for bpc, chunk in candidates:
         eng.set_option("portfolio_blocks_per_cu", bpc)
         eng.set_option("portfolio_chunk", chunk)
-        dsl.price_portfolio(jobs, device=device)
+        dsl.price_portfolio(jobs, device=device, greeks=greeks)
         eng.set_profiling(True)
         eng.kernel_time(reset=True)
         for _ in range(max(1, repeats)):
-            dsl.price_portfolio(jobs, device=device)
+            dsl.price_portfolio(jobs, device=device, greeks=greeks)
         ms = eng.kernel_time(reset=True)[0] / max(1, repeats)
         eng.set_profiling(False)
         timings[(bpc, chunk)] = ms
     best = min(timings, key=timings.get)
     eng.set_option("portfolio_blocks_per_cu", best[0])
     eng.set_option("portfolio_chunk", best[1])
-    return {"best": best, "device_ms": timings}
+    return {"best": best, "device_ms": {f"{k[0]}x{k[1]}": v for k, v in timings.items()}}

@@ -414,3 +414,18 @@ def test_greeks_vs_oracle():
     assert abs(g.price - o["price"]) < 0.5 * g.std_errors[0]
     assert abs(g.delta - o["delta"]) < 0.5 * g.std_errors[1] + 1e-3
     assert abs(g.vega - o["vega"]) < 0.5 * g.std_errors[3] + 0.5
+
+
+def test_time_out_of_range_names_the_date():
+    # KontractError::TimeOutOfRange carries the offending time (src/error.rs);
+    # the engine raises it when an AverageAt fixing date is not a grid point
+    # and names the date in the message
+    m = K.GBM(100, 0.2, 0.05, asset="X")
+    c = K.scale(K.average_at(K.spot("X"), [0.3, 1.0]), K.one("USD")) @ K.at(1.0)
+    with pytest.raises(K.KontractError) as e:
+        c.price(m, n_paths=1000, grid=[i / 4 for i in range(5)])
+    assert e.value.kind == "TimeOutOfRange"
+    assert str(e.value).rstrip().endswith("0.29999999999999999")
+    with pytest.raises(K.KontractError) as e:
+        K.price_portfolio([(c, m, 1000, 1, 4, [i / 4 for i in range(5)])])
+    assert e.value.kind == "TimeOutOfRange" and "0.29999999999999999" in str(e.value)
