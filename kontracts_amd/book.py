@@ -36,10 +36,12 @@ RATE = 0.05
 KINDS = ("european", "asian", "barrier", "basket")
 
 
-def _daily_grid(t: float) -> tuple[list[float], int]:
+def _daily_grid(t: float) -> tuple[np.ndarray, int]:
+    """Points t k / n, k = 0 .. n, as float64 (passed to the engine without a
+    copy)."""
     n = int(math.ceil(252.0 * t - 1e-9))
     n = max(n, 1)
-    return [t * k / n for k in range(n + 1)], n
+    return np.array([t * k / n for k in range(n + 1)], dtype=np.float64), n
 
 
 def book(n_contracts: int = 1000, seed: int = 42, paths_exp: tuple = (16, 20),
@@ -80,7 +82,7 @@ def book(n_contracts: int = 1000, seed: int = 42, paths_exp: tuple = (16, 20),
                 contract = dsl.up_and_out_call(asset, min(strike, 0.95 * h_level), h_level, t)
             else:
                 m = max(1, int(round(12.0 * t)))
-                fix = sorted({grid[min(n, max(1, int(round(i * n / m))))] for i in range(1, m + 1)})
+                fix = sorted({float(grid[min(n, max(1, int(round(i * n / m))))]) for i in range(1, m + 1)})
                 contract = dsl.scale((dsl.average_at(dsl.spot(asset), fix) - strike).clip(0.0),
                                      dsl.one("USD")) @ dsl.at(t)
         jobs.append((contract, model, n_paths, seed + j, 252, grid))

@@ -197,8 +197,77 @@ __device__ __forceinline__ void adjoint_gbm_block(const AA& A, uint32_t blk) {
     argstep[q] = 0;
   }
   double pv[1][1] = {{0.0}};
-  float v = 0.0f;
-  for (int g = 0; g < P.n_groups; ++g) {
+  // Groups of four points without ops (G_NONE: nothing evaluated, nothing
+  // fed) only advance the log-spots and the Brownian sums, with the
+  // per-point path's expressions on the same values.  The leading run of
+  // them (P.n_uniform) runs, under GBM on a grid of equal steps, in a loop
+  // of its own that takes the step coefficients and sqrt(dt) from VGPR
+  // copies of point 1's rows (FP32 instructions with SGPR operands issue at
+  // about half rate); the copies live in that loop only.  Other op-free
+  // groups read the table.
+  auto quiet_group = [&](int g, auto&& coef) {
+    float z[4 * M];
+    group_normals<M>(g, p_lo, p_hi, P.seed_lo, P.seed_hi, 0u, z);
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      const int k = 4 * g + s;
+      const float* zk = &z[s * M];
+      float sq, ta[K], tb[K], ch[K * (K + 1) / 2];
+      coef(k, sq, ta, tb, ch);
+#pragma unroll
+      for (int a = 0; a < K; ++a) {
+        float w = 0.0f;
+#pragma unroll
+        for (int c = 0; c <= a; ++c) w = fmaf(ch[a * (a + 1) / 2 + c], zk[c], w);
+        ws[a] = fmaf(sq, w, ws[a]);
+        x[0][a] = fmaf(tb[a], w, x[0][a] + ta[a]);
+      }
+    }
+  };
+  auto table_coef = [&](int k, float& sq, float (&ta)[K], float (&tb)[K], float (&ch)[K * (K + 1) / 2]) {
+    sq = ld_f32(A.adj_tab, 2 * (size_t)k);
+#pragma unroll
+    for (int a = 0; a < K; ++a) {
+      ta[a] = ld_f32(P.tab, ((size_t)k * K + a) * 2);
+      tb[a] = ld_f32(P.tab, ((size_t)k * K + a) * 2 + 1);
+#pragma unroll
+      for (int c = 0; c <= a; ++c) ch[a * (a + 1) / 2 + c] = K == 1 ? 1.0f : P.chol[a * K + c];
+    }
+  };
+  // the last group may be partial: it always takes the per-point path
+  const int n_lead = min(P.n_uniform, P.n_groups - 1);
+  int g_first = 0;
+  // (GBM only: with the basket's 26 copies the kernel spills; its price
+  // kernel gained 2% from them)
+  if (K == 1 && n_lead > 1 && P.tab_uniform) {
+    // group 0 holds point 0's zero row: it reads the table
+    quiet_group(0, table_coef);
+    float usq = vgpr_copy(ld_f32(A.adj_tab, 2)), ua[K], ub[K], uch[K * (K + 1) / 2];
+#pragma unroll
+    for (int a = 0; a < K; ++a) {
+      ua[a] = vgpr_copy(ld_f32(P.tab, ((size_t)K + a) * 2));
+      ub[a] = vgpr_copy(ld_f32(P.tab, ((size_t)K + a) * 2 + 1));
+#pragma unroll
+      for (int c = 0; c <= a; ++c) uch[a * (a + 1) / 2 + c] = vgpr_copy(K == 1 ? 1.0f : P.chol[a * K + c]);
+    }
+    auto row_coef = [&](int, float& sq, float (&ta)[K], float (&tb)[K], float (&ch)[K * (K + 1) / 2]) {
+      sq = usq;
+#pragma unroll
+      for (int a = 0; a < K; ++a) {
+        ta[a] = ua[a];
+        tb[a] = ub[a];
+      }
+#pragma unroll
+      for (int i = 0; i < K * (K + 1) / 2; ++i) ch[i] = uch[i];
+    };
+    for (int g = 1; g < n_lead; ++g) quiet_group(g, row_coef);
+    g_first = n_lead;
+  }
+  for (int g = g_first; g < P.n_groups; ++g) {
+    if (K == 1 && g + 1 < P.n_groups && ld_u32(P.groups, (size_t)g * 4) == (uint32_t)G_NONE) {
+      quiet_group(g, table_coef);
+      continue;
+    }
     float z[4 * M];
 #pragma unroll
     for (int bk = 0; bk < M; ++bk) {
@@ -220,8 +289,9 @@ __device__ __forceinline__ void adjoint_gbm_block(const AA& A, uint32_t blk) {
 #pragma unroll
           for (int c = 0; c <= a; ++c) w = fmaf(K == 1 ? 1.0f : P.chol[a * K + c], zk[c], w);
           ws[a] = fmaf(sq, w, ws[a]);
+          // advance's exact log step on the same w (computed once)
+          x[0][a] = fmaf(ld_f32(P.tab, ((size_t)k * K + a) * 2 + 1), w, x[0][a] + ld_f32(P.tab, ((size_t)k * K + a) * 2));
         }
-        advance<MODEL, K, 1>(P, k, zk, x, v);
         const uint32_t fm = ld_u32(A.feed, (size_t)k);
         int before[kAdjMaxAcc];
 #pragma unroll
@@ -521,6 +591,15 @@ __device__ __forceinline__ void adjoint_heston_block(const AA& A, uint32_t blk, 
   for (int g = n_quiet; g < P.n_groups; ++g) {
     float z[8];
     group_normals<2>(g, p_lo, p_hi, P.seed_lo, P.seed_hi, 0u, z);
+    // a later group without ops (full: only the last one can be partial)
+    if (g + 1 < P.n_groups && ld_u32(P.groups, (size_t)g * 4) == (uint32_t)G_NONE) {
+#pragma unroll
+      for (int s = 0; s < 4; ++s) {
+        checkpoint(4 * g + s);
+        advance<MK_HESTON, 1, 1>(P, 4 * g + s, &z[2 * s], x, v);
+      }
+      continue;
+    }
 #pragma unroll
     for (int s = 0; s < 4; ++s) {
       const int k = 4 * g + s;

@@ -171,6 +171,23 @@ const char* error_text(int code) {
   }
 }
 
+std::shared_ptr<const Program> lower_cached(kt_engine* E, const char* json, const Con& con,
+                                            const std::vector<double>& grid, const std::vector<std::string>& names) {
+  std::string key(json);
+  key.push_back('\0');
+  key.append((const char*)grid.data(), grid.size() * sizeof(double));
+  for (const std::string& n : names) {
+    key.push_back('\0');
+    key += n;
+  }
+  auto it = E->lower_cache.find(key);
+  if (it != E->lower_cache.end()) return it->second;
+  auto p = std::make_shared<const Program>(lower({&con}, grid, names));
+  if (E->lower_cache.size() >= 16384) E->lower_cache.clear();
+  E->lower_cache.emplace(std::move(key), p);
+  return p;
+}
+
 std::string error_text(int code, const Program& prog) {
   std::string s = error_text(code);
   if (code == KT_ERR_TIME_OUT_OF_RANGE && !std::isnan(prog.time_payload)) {

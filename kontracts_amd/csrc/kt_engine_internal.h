@@ -6,6 +6,7 @@
 #include <cstring>
 #include <memory>
 #include <string>
+#include <unordered_map>
 #include <utility>
 #include <vector>
 
@@ -60,6 +61,9 @@ struct kt_engine {
   double prof_ms = 0.0, prof_path_steps = 0.0;
   uint64_t prof_launches = 0;
   int adj_seg = kt::kAdjSeg;  // Heston adjoint checkpoint interval (kt_engine_set_option)
+  // Lowered programs of recent (contract JSON, grid, asset names): a book
+  // priced again (new market data, same contracts) skips the lowering.
+  std::unordered_map<std::string, std::shared_ptr<const kt::Program>> lower_cache;
   // Bracket a group of launches with events when profiling is enabled.
   void prof_begin();
   void prof_end(double path_steps);
@@ -116,6 +120,11 @@ const char* error_text(int code);
 // error_text with the program's payload: TimeOutOfRange names the fixing
 // date ("...: <date>"), as KontractError::TimeOutOfRange(f64) carries it.
 std::string error_text(int code, const Program& prog);
+
+// lower({con}, grid, names) of the contract whose JSON is `json`, memoised
+// on the engine (the program depends on nothing else).
+std::shared_ptr<const Program> lower_cached(kt_engine* E, const char* json, const Con& con,
+                                            const std::vector<double>& grid, const std::vector<std::string>& names);
 
 // allow_vr: the caller handles cfg's variance-reduction flags; otherwise
 // they are rejected.

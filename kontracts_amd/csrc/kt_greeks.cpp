@@ -415,9 +415,13 @@ void fill_adjoint_args(AdjArgs& A, const AdjJobHost& H, const Program& prog, con
                     ? std::min(uniform_group_prefix(prog.groups.data(), (int)prog.groups.size()), P.n_groups)
                     : 0;
   P.tab_stride = H.stride;
-  // the Heston sweep keeps the shared rows in registers when every row is equal
+  // the Heston sweep, and the GBM / basket runs of op-free groups, keep the
+  // shared rows in registers when every row is equal (GBM and basket: the
+  // model rows and sqrt(dt); Heston: the model and reverse rows)
+  bool sq_uniform = grid.size() >= 3;
+  for (size_t k = 2; k < grid.size() && sq_uniform; ++k) sq_uniform = H.adj[2 * k] == H.adj[2];
   P.tab_uniform = uniform_table_rows(H.tab, grid.size(), (size_t)H.stride) &&
-                  uniform_table_rows(H.rev, grid.size(), 8);
+                  (md.kind == KT_MODEL_HESTON ? uniform_table_rows(H.rev, grid.size(), 8) != 0 : sq_uniform);
   fill_model_args(P, md, seed);
   A.adj_tab = (const float*)dev[7];
   A.feed = (const uint32_t*)dev[8];

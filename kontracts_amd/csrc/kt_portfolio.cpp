@@ -80,7 +80,7 @@ void price_portfolio(kt_engine* E, const kt_job* jobs, uint64_t n_jobs, kt_price
       solo.push_back(j);
       continue;
     }
-    P.prog = lower({pc.con.get()}, P.grid, P.md.names);
+    P.prog = *lower_cached(E, J.contract_json, *pc.con, P.grid, P.md.names);
     P.variant = variant_of(P.md, P.prog);
     if (J.cfg.n_paths == 0) {
       out[j] = result_from_stats(KStats{0.0, 0.0, 0.0});

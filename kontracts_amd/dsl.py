@@ -7,10 +7,13 @@ GPU engine through ``kontracts_amd._native``.
 """
 from __future__ import annotations
 
+import array
 import ctypes as C
 import json
 import math
 from typing import Iterable, Sequence
+
+import numpy as np
 
 from . import _native as N
 
@@ -616,8 +619,18 @@ class _Struct:
 
 
 def _dbl(xs) -> C.Array:
-    xs = list(xs)
-    return (C.c_double * max(len(xs), 1))(*xs)
+    """A C double array of xs: a float64 numpy array is shared without a
+    copy; anything else is copied (array.array converts a list in one C
+    loop).  The ctypes array keeps the buffer alive."""
+    if isinstance(xs, np.ndarray) and xs.dtype == np.float64 and xs.ndim == 1 and xs.size \
+            and xs.flags.c_contiguous and xs.flags.writeable:
+        return (C.c_double * xs.size).from_buffer(xs)
+    if isinstance(xs, (list, tuple)) and len(xs) <= 32:
+        return (C.c_double * max(len(xs), 1))(*xs)
+    a = array.array("d", xs)
+    if not a:
+        a.append(0.0)
+    return (C.c_double * len(a)).from_buffer(a)
 
 
 def _model_struct(model) -> _Struct:
