@@ -420,6 +420,7 @@ PathArgs program_path_args(const Program& prog, const std::vector<ModelDesc>& mo
   A.n_grid = (int)grid.size();
   A.n_groups = (int)T.groups.size();
   A.n_uniform = uniform_group_prefix(T.groups.data(), A.n_groups);
+  A.serial_tail = scenario_serial_tail(T.groups.data(), A.n_groups);
   A.tab_uniform = uniform_table_rows(T.tab, grid.size(), (size_t)T.stride);
   A.tab_stride = T.stride;
   fill_model_args(A, models[0], seed);

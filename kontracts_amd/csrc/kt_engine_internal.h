@@ -53,7 +53,7 @@ struct kt_engine {
   DevBuf errp;
   bool errp_zero = false;
   int occupancy_portfolio = 0;  // cached occupancies (blocks per CU)
-  int occupancy_surface[2] = {0, 0};  // Greek surface kernel with 8 and 16 registers
+  int occupancy_surface[4] = {0, 0, 0, 0};  // Greek surface kernel: 8 / 16 registers, then their serial-tail variants
   int occupancy_adjoint[kt::APV_COUNT] = {0, 0, 0, 0, 0, 0};
   bool pathwise_tangent = false;  // Heston pathwise Greeks: forward tangents instead of the reverse sweep
   bool profiling = false;

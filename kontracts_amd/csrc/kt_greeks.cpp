@@ -173,6 +173,7 @@ void greek_surface(kt_engine* E, const char* json, const char* asset, const doub
   const uint64_t per_launch = std::max<uint64_t>(1, (1ull << 24) / nb);
   for (uint64_t c0 = 0; c0 < n_cells; c0 += per_launch) {
     const uint64_t nc = std::min(per_launch, n_cells - c0);
+    bool serial = true;  // every cell takes the scenario-serial kernel
     std::vector<LaunchTables> T(nc);
     // the discount tables depend on the rates only: the first cell's serve all
     for (uint64_t i = 0; i < nc; ++i) T[i] = launch_tables(prog, scen[order[c0 + i]], rates, grid, i == 0);
@@ -234,6 +235,8 @@ void greek_surface(kt_engine* E, const char* json, const char* asset, const doub
       A.n_grid = (int)grid.size();
       A.n_groups = (int)T[i].groups.size();
       A.n_uniform = uniform_group_prefix(T[i].groups.data(), A.n_groups);
+      A.serial_tail = scenario_serial_tail(T[i].groups.data(), A.n_groups);
+      serial = serial && A.serial_tail;
       A.tab_uniform = uniform_table_rows(T[i].tab, grid.size(), (size_t)T[i].stride);
       A.tab_stride = T[i].stride;
       fill_model_args(A, sc_models[0], cfg->seed);
@@ -254,9 +257,9 @@ void greek_surface(kt_engine* E, const char* json, const char* asset, const doub
     E->prof_begin();
     int n_cu = 0;
     check_hip(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, E->device), "device attribute");
-    int& occ = E->occupancy_surface[ns == 8 ? 0 : 1];
-    if (occ <= 0) occ = surface_occupancy_blocks_per_cu(ns);
-    check_hip(launch_surface(ns, (const PathArgs*)dev, (int)nc, (int)nb, partials, stats, n_cu, occ,
+    int& occ = E->occupancy_surface[(ns == 8 ? 0 : 1) + (serial ? 2 : 0)];
+    if (occ <= 0) occ = surface_occupancy_blocks_per_cu(ns, serial);
+    check_hip(launch_surface(ns, serial, (const PathArgs*)dev, (int)nc, (int)nb, partials, stats, n_cu, occ,
                              E->stream),
               "surface launch");
     E->prof_end((double)N * (double)(grid.size() - 1) * kScenarios * (double)nc);

@@ -5,7 +5,9 @@
 // contiguous across the grid.  A path's state (log-spot per asset, Heston
 // variance, the present values) stays in VGPRs; the op program's register
 // file does too for one scenario and 16 registers, and lives (partly) in
-// per-lane scratch for 32 registers and for the seven CRN scenarios.  Normals
+// per-lane scratch for 32 registers and for the seven CRN scenarios of
+// programs interpreted before their last two groups (the others interpret
+// their tail one scenario at a time from a file in LDS).  Normals
 // come from Philox4x32-10 keyed by the seed with counter
 // (block, path_lo, path_hi, 0): normal j = k*M + c of grid point k, component
 // c, is word j%4 of block j/4, so the stream does not depend on the launch
@@ -26,6 +28,7 @@
 // kernel reduces SC-weighted combinations of them (finite-difference Greeks).
 #include <algorithm>
 
+#include "../../include/kontracts.h"
 #include "kt_path_ops.h"
 
 namespace kt {
@@ -188,6 +191,18 @@ __device__ __forceinline__ void uniform_groups(const PA& A, const KGroup gd, int
   if constexpr (knock_out_kind<KIND>()) ko_apply<KIND, SC>(A, 0, ext, acc);
 }
 
+// The fields run_ops reads, for one scenario of a scenario launch: its spot,
+// its discount table and the launch's program (run_ops with SC = 1).
+template <int K>
+struct OneScenario {
+  const KOp* ops;
+  const double* df;
+  int df_index[1];
+  int n_grid;
+  unsigned* err;
+  float s0[K];
+};
+
 // One block of 256 paths: paths blk*256 .. blk*256+255 of the launch (global
 // path index path_base + local), statistics into partials[blk].  Shared by
 // the per-contract path kernel and the persistent portfolio kernel.
@@ -198,9 +213,11 @@ __device__ __forceinline__ void uniform_groups(const PA& A, const KGroup gd, int
 // keeps 0.5 * (pv_base + pv_anti) (src/variance_reduction.rs:196-201).
 // Complementing every bit of a Philox word maps the uniform u of its cell to
 // 1 - u, so normal_from_word(~w) == -normal_from_word(w) exactly.
-template <int MODEL, int K, int NS, int NOUT, int SC, int NSTAT, bool DEBUG, bool ANTI = false, class PA>
+template <int MODEL, int K, int NS, int NOUT, int SC, int NSTAT, bool DEBUG, bool ANTI = false, bool SERIAL = false,
+          class PA>
 __device__ __forceinline__ void path_block(const PA& A, uint32_t blk, float (&r)[SC][NS]) {
   static_assert(!ANTI || (SC == 1 && !DEBUG), "antithetic pairs: one scenario, no debug output");
+  static_assert(!SERIAL || (SC > 1 && !DEBUG && MODEL == MK_GBM && K == 1), "scenario-serial tail: GBM scenarios");
   constexpr int M = MODEL == MK_GBM ? 1 : (MODEL == MK_HESTON ? 2 : K);
   const uint64_t local = (uint64_t)blk * kBlock + threadIdx.x;
   const bool live = local < A.n_paths;
@@ -215,16 +232,31 @@ __device__ __forceinline__ void path_block(const PA& A, uint32_t blk, float (&r)
   const uint32_t flip = (ANTI && rep) ? 0xffffffffu : 0u;
   FOR_SC {
 #pragma unroll
-    for (int i = 0; i < NS; ++i) r[sc][i] = 0.0f;
-#pragma unroll
     for (int j = 0; j < NOUT; ++j) pv[sc][j] = 0.0;
 #pragma unroll
     for (int i = 0; i < K; ++i) x[sc][i] = 0.0f;
   }
-  for (int i = 0; i < A.n_init; ++i) {
-    const uint64_t w = ld_u64(A.init, (size_t)i);
-    FOR_SC r[sc][((uint32_t)w >> 8) & (NS - 1)] = __uint_as_float((uint32_t)(w >> 32));
-  }
+  // The register file starts as zeros plus the init ops.  It is written only
+  // where the dispatching loop below runs; the uniform prefix takes its
+  // monitor's initial value from the init list directly.
+  auto init_file = [&]() {
+    FOR_SC {
+#pragma unroll
+      for (int i = 0; i < NS; ++i) r[sc][i] = 0.0f;
+    }
+    for (int i = 0; i < A.n_init; ++i) {
+      const uint64_t w = ld_u64(A.init, (size_t)i);
+      FOR_SC r[sc][((uint32_t)w >> 8) & (NS - 1)] = __uint_as_float((uint32_t)(w >> 32));
+    }
+  };
+  auto init_value = [&](uint32_t slot) {
+    float val = 0.0f;
+    for (int i = 0; i < A.n_init; ++i) {
+      const uint64_t w = ld_u64(A.init, (size_t)i);
+      if ((((uint32_t)w >> 8) & (NS - 1)) == slot) val = __uint_as_float((uint32_t)(w >> 32));
+    }
+    return val;
+  };
   float v = A.v0;
   // monitor accumulators cached in registers (slot mslot, kind mkind; -1 = none)
   float macc[SC], mcomp[SC];
@@ -276,9 +308,10 @@ __device__ __forceinline__ void path_block(const PA& A, uint32_t blk, float (&r)
       auto enter = [&]() {
         if (gu.kind != G_NONE) {
           const uint32_t s0i = (uint32_t)gu.slot & (NS - 1), s1i = (s0i + 1) & (NS - 1);
+          const float a0 = init_value(s0i), c0 = gu.kind == G_AVG ? init_value(s1i) : 0.0f;
           FOR_SC {
-            macc[sc] = get(sc, s0i);
-            mcomp[sc] = gu.kind == G_AVG ? get(sc, s1i) : 0.0f;
+            macc[sc] = a0;
+            mcomp[sc] = c0;
           }
           mslot = gu.slot;
           mkind = gu.kind;
@@ -305,6 +338,93 @@ __device__ __forceinline__ void path_block(const PA& A, uint32_t blk, float (&r)
     }
   }
 
+  // Scenarios (SC > 1) after a uniform prefix that leaves at most two
+  // groups: every scenario's register file then holds the init constants and
+  // its monitor, so the remaining points are interpreted one scenario at a
+  // time with one NS-register file (kept in VGPRs) instead of SC files
+  // indexed at run time (which live in scratch).  Each scenario re-advances
+  // its log-spot path (or its source scenario's) from the end of the prefix
+  // over the same normals: the same operations on the same values, in the
+  // same order per scenario, as the dispatching loop.
+  // (SERIAL kernels only: the host launches them for programs whose uniform
+  // prefix leaves at most two groups; any other program raises Unsupported)
+  bool serial_done = false;
+  if constexpr (SERIAL) {
+    constexpr int TG = kSerialTailGroups;
+    const int nt = A.n_groups - g_begin;
+    if (!(g_begin > 0 && nt <= TG)) {
+      if (live) atomicOr(A.err, 1u << KT_ERR_UNSUPPORTED);
+    } else {
+      float zt[TG][4];
+#pragma unroll
+      for (int t = 0; t < TG; ++t)
+        if (t < nt) group_normals<1>(g_begin + t, p_lo, p_hi, A.seed_lo, A.seed_hi, flip, zt[t]);
+      const uint32_t s0i = (uint32_t)mslot & (NS - 1), s1i = (s0i + 1) & (NS - 1);
+      // One scenario per iteration (not unrolled: SC copies of the
+      // interpreter would not fit the instruction cache); the scenario's
+      // values are picked and written back by compare-selects, which keep
+      // the per-scenario arrays in VGPRs.  The scenario's register file sits
+      // in LDS, one column per lane (NS x 256 floats per workgroup): an op's
+      // run-time register index becomes an LDS address instead of a chain of
+      // selects over a VGPR array.
+      __shared__ float lds_rf[NS * kBlock];
+      const RegLds rf{&lds_rf[threadIdx.x], kBlock};
+#pragma unroll 1
+      for (int sc = 0; sc < SC; ++sc) {
+        const int src = xsrc<SC>(sc);
+        float m_acc = macc[0], m_comp = mcomp[0], x_start = x[0][0];
+#pragma unroll
+        for (int q = 1; q < SC; ++q) {
+          if (sc == q) {
+            m_acc = macc[q];
+            m_comp = mcomp[q];
+          }
+          if (src == q) x_start = x[q][0];
+        }
+#pragma unroll
+        for (int i = 0; i < NS; ++i) rf(0, (uint32_t)i) = 0.0f;
+        for (int i = 0; i < A.n_init; ++i) {
+          const uint64_t w = ld_u64(A.init, (size_t)i);
+          rf(0, ((uint32_t)w >> 8) & (NS - 1)) = __uint_as_float((uint32_t)(w >> 32));
+        }
+        if (mslot >= 0) {
+          rf(0, s0i) = m_acc;
+          if (mkind == G_AVG) rf(0, s1i) = m_comp;
+        }
+        float xs[1][1] = {{x_start}};
+        double ps[1][NOUT];
+#pragma unroll
+        for (int j = 0; j < NOUT; ++j) ps[0][j] = 0.0;
+        const OneScenario<1> V{A.ops, A.df, {A.df_index[sc]}, A.n_grid, A.err, {A.s0_scen[sc]}};
+#pragma unroll
+        for (int t = 0; t < TG; ++t) {
+          if (t < nt) {
+            const int k0 = 4 * (g_begin + t);
+#pragma unroll
+            for (int s = 0; s < 4; ++s) {
+              const int k = k0 + s;
+              if (k < A.n_grid) {
+                const float a = ld_f32(A.tab, ((size_t)k * SC + src) * 2), b = ld_f32(A.tab, ((size_t)k * SC + src) * 2 + 1);
+                xs[0][0] = fmaf(b, zt[t][s], xs[0][0] + a);
+                const uint64_t st = ld_u64(A.steps, (size_t)k);
+                const int begin = (int)(uint32_t)st, count = (int)(uint32_t)(st >> 32);
+                if (count > 0) run_ops_rf<NS, NOUT, 1, 1>(V, k, begin, count, rf, ps, xs, live);
+              }
+            }
+          }
+        }
+#pragma unroll
+        for (int q = 0; q < SC; ++q)
+          if (sc == q) {
+#pragma unroll
+            for (int j = 0; j < NOUT; ++j) pv[q][j] = ps[0][j];
+          }
+      }
+      serial_done = true;
+    }
+  }
+  if constexpr (!SERIAL) {
+  init_file();
   for (int g = g_begin; g < A.n_groups; ++g) {
     const KGroup gd = load_group(g);
     float z[4 * M];
@@ -362,6 +482,8 @@ __device__ __forceinline__ void path_block(const PA& A, uint32_t blk, float (&r)
     }
   }
   flush();
+  }  // !SERIAL
+  (void)serial_done;
   if constexpr (ANTI) {
 #pragma unroll
     for (int j = 0; j < NOUT; ++j) pair[j] = rep ? 0.5 * (pair[j] + pv[0][j]) : pv[0][j];
@@ -420,10 +542,10 @@ __device__ __forceinline__ void path_block(const PA& A, uint32_t blk, float (&r)
   __syncthreads();
 }
 
-template <int MODEL, int K, int NS, int NOUT, int SC, int NSTAT, bool DEBUG, bool ANTI>
+template <int MODEL, int K, int NS, int NOUT, int SC, int NSTAT, bool DEBUG, bool ANTI, bool SERIAL = false>
 __global__ __launch_bounds__(kBlock) void path_kernel(const PathArgs A) {
   float r[SC][NS];
-  path_block<MODEL, K, NS, NOUT, SC, NSTAT, DEBUG, ANTI>(A, blockIdx.x, r);
+  path_block<MODEL, K, NS, NOUT, SC, NSTAT, DEBUG, ANTI, SERIAL>(A, blockIdx.x, r);
 }
 
 // The seven-scenario CRN kernel allocates 123 VGPRs (4 waves per SIMD) for
@@ -446,6 +568,27 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))) voi
 path_kernel<MK_GBM, 1, 8, 1, kScenarios, kScenarioStats, false, false>(const PathArgs A) {
   float r[kScenarios][8];
   path_block<MK_GBM, 1, 8, 1, kScenarios, kScenarioStats, false, false>(A, blockIdx.x, r);
+}
+
+// The scenario-serial variants (programs whose uniform prefix leaves at most
+// kSerialTailGroups groups, e.g. a knock-out monitored at every point and a
+// payoff at the last): the tail's register file is one scenario's, in LDS,
+// so the kernel has no register file in scratch: 0.85 against 0.89-0.93 ms
+// on the 4,194,304 x 252 up-and-out, 2.4 MB of HBM writes per launch (the
+// block partials) against 1.4 GB.  It needs 65 VGPRs (7 waves per SIMD);
+// capped at 64 for 8 waves it runs at the same rate.
+template <>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))) void
+path_kernel<MK_GBM, 1, 16, 1, kScenarios, kScenarioStats, false, false, true>(const PathArgs A) {
+  float r[kScenarios][16];
+  path_block<MK_GBM, 1, 16, 1, kScenarios, kScenarioStats, false, false, true>(A, blockIdx.x, r);
+}
+
+template <>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))) void
+path_kernel<MK_GBM, 1, 8, 1, kScenarios, kScenarioStats, false, false, true>(const PathArgs A) {
+  float r[kScenarios][8];
+  path_block<MK_GBM, 1, 8, 1, kScenarios, kScenarioStats, false, false, true>(A, blockIdx.x, r);
 }
 
 // Block j reduces output j.  err (nullable): block 0 copies the launch's
@@ -524,9 +667,9 @@ __global__ __launch_bounds__(256) void portfolio_finalize_kernel(const Portfolio
 // tables instead of per_cu of them.  The map is a bijection on every full
 // chunk (identity on the last partial one), so it only changes which
 // workgroup computes a block, never a result.
-template <int NS>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))) void surface_kernel(const PathArgs* __restrict__ cells, uint32_t n_blocks,
-                                                         uint32_t total, uint32_t n_cu, uint32_t per_cu) {
+template <int NS, bool SERIAL>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))) void surface_kernel(
+    const PathArgs* __restrict__ cells, uint32_t n_blocks, uint32_t total, uint32_t n_cu, uint32_t per_cu) {
   const uint32_t L = blockIdx.x;
   const uint32_t chunk = n_cu * per_cu;
   uint32_t v = L;
@@ -538,7 +681,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))) voi
   const uint32_t cell = v / n_blocks, blk = v - cell * n_blocks;
   const KT_CONST PathArgs& A = ((const KT_CONST PathArgs*)cells)[cell];
   float r[kScenarios][NS];
-  path_block<MK_GBM, 1, NS, 1, kScenarios, kScenarioStats, false>(A, blk, r);
+  path_block<MK_GBM, 1, NS, 1, kScenarios, kScenarioStats, false, false, SERIAL>(A, blk, r);
 }
 
 // Block c * kScenarioStats + j reduces output j of cell c (cell c's partials
@@ -926,10 +1069,10 @@ __global__ __launch_bounds__(kBlock) void lsm_solve_kernel(const double* __restr
 // ---------------------------------------------------------------------------
 namespace {
 
-template <int MODEL, int K, int NS, int NOUT, int SC, int NSTAT, bool DEBUG, bool ANTI = false>
+template <int MODEL, int K, int NS, int NOUT, int SC, int NSTAT, bool DEBUG, bool ANTI = false, bool SERIAL = false>
 hipError_t go(const PathArgs& a, int n_blocks, hipStream_t s) {
-  hipLaunchKernelGGL((path_kernel<MODEL, K, NS, NOUT, SC, NSTAT, DEBUG, ANTI>), dim3(n_blocks), dim3(kBlock), 0, s,
-                     a);
+  hipLaunchKernelGGL((path_kernel<MODEL, K, NS, NOUT, SC, NSTAT, DEBUG, ANTI, SERIAL>), dim3(n_blocks), dim3(kBlock), 0,
+                     s, a);
   return hipGetLastError();
 }
 
@@ -984,10 +1127,13 @@ hipError_t launch_path(int model, int k, int ns, int nout, bool debug, int n_sce
                        hipStream_t s, bool anti) {
   if (n_scen != 1) {
     if (anti) return hipErrorInvalidValue;
+    const bool ser = a.serial_tail != 0;
     if (model == MK_GBM && n_scen == kScenarios && ns == 16 && nout == 1 && !debug)
-      return go<MK_GBM, 1, 16, 1, kScenarios, kScenarioStats, false>(a, n_blocks, s);
+      return ser ? go<MK_GBM, 1, 16, 1, kScenarios, kScenarioStats, false, false, true>(a, n_blocks, s)
+                 : go<MK_GBM, 1, 16, 1, kScenarios, kScenarioStats, false>(a, n_blocks, s);
     if (model == MK_GBM && n_scen == kScenarios && ns == 8 && nout == 1 && !debug)
-      return go<MK_GBM, 1, 8, 1, kScenarios, kScenarioStats, false>(a, n_blocks, s);
+      return ser ? go<MK_GBM, 1, 8, 1, kScenarios, kScenarioStats, false, false, true>(a, n_blocks, s)
+                 : go<MK_GBM, 1, 8, 1, kScenarios, kScenarioStats, false>(a, n_blocks, s);
     return hipErrorInvalidValue;
   }
   switch (model) {
@@ -1082,25 +1228,35 @@ hipError_t launch_portfolio(const PortfolioArgs& q, const int* n_blocks, int n_j
   return hipGetLastError();
 }
 
-int surface_occupancy_blocks_per_cu(int ns) {
+int surface_occupancy_blocks_per_cu(int ns, bool serial) {
   int n = 0;
-  const hipError_t e = ns == 8 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, surface_kernel<8>, kBlock, 0)
-                               : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, surface_kernel<16>, kBlock, 0);
+  hipError_t e;
+  if (serial)
+    e = ns == 8 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, surface_kernel<8, true>, kBlock, 0)
+                : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, surface_kernel<16, true>, kBlock, 0);
+  else
+    e = ns == 8 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, surface_kernel<8, false>, kBlock, 0)
+                : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, surface_kernel<16, false>, kBlock, 0);
   if (e != hipSuccess) return 4;
   return n > 0 ? n : 1;
 }
 
-hipError_t launch_surface(int ns, const PathArgs* cells, int n_cells, int n_blocks, const KStats* partials,
-                          KStats* out, int n_cu, int per_cu, hipStream_t s) {
+hipError_t launch_surface(int ns, bool serial, const PathArgs* cells, int n_cells, int n_blocks,
+                          const KStats* partials, KStats* out, int n_cu, int per_cu, hipStream_t s) {
   if (n_cells <= 0 || n_blocks <= 0 || n_cu <= 0 || per_cu <= 0 || (ns != 8 && ns != 16)) return hipErrorInvalidValue;
   const uint64_t total = (uint64_t)n_cells * (uint64_t)n_blocks;
   if (total > 0x7fffffffull || (uint64_t)n_cu * (uint64_t)per_cu > 0xffffffffull) return hipErrorInvalidValue;
-  if (ns == 8)
-    hipLaunchKernelGGL(surface_kernel<8>, dim3((unsigned)total), dim3(kBlock), 0, s, cells, (uint32_t)n_blocks,
-                       (uint32_t)total, (uint32_t)n_cu, (uint32_t)per_cu);
-  else
-    hipLaunchKernelGGL(surface_kernel<16>, dim3((unsigned)total), dim3(kBlock), 0, s, cells, (uint32_t)n_blocks,
-                       (uint32_t)total, (uint32_t)n_cu, (uint32_t)per_cu);
+#define KT_SURF(NSV, SER)                                                                                  \
+  hipLaunchKernelGGL((surface_kernel<NSV, SER>), dim3((unsigned)total), dim3(kBlock), 0, s, cells, (uint32_t)n_blocks, \
+                     (uint32_t)total, (uint32_t)n_cu, (uint32_t)per_cu)
+  if (ns == 8) {
+    if (serial) KT_SURF(8, true);
+    else KT_SURF(8, false);
+  } else {
+    if (serial) KT_SURF(16, true);
+    else KT_SURF(16, false);
+  }
+#undef KT_SURF
   hipLaunchKernelGGL(surface_finalize_kernel, dim3(n_cells * kScenarioStats), dim3(256), 0, s, partials, n_blocks,
                      out);
   return hipGetLastError();

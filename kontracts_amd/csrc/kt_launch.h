@@ -50,6 +50,7 @@ struct PathArgs {
   int df_index[kScenarios];       // discount table of each scenario (0 when SC == 1)
   double stat_w[kScenarioStats][kScenarios];  // scenario kernel: output j = sum_sc w[j][sc] pv_sc
   int combine;  // one scenario: > 0 makes output 0 sum_{o < combine} stat_w[0][o] pv_o
+  int serial_tail;  // scenarios: scenario_serial_tail of the groups (picks the kernel variant)
   KStats* partials;  // [n_blocks][NOUT]
   unsigned* err;     // bit e set <=> error code e raised on a live path
   double* pv_out;    // debug: per-path pv of output 0 (nullable)
@@ -182,9 +183,10 @@ hipError_t launch_finalize(const KStats* partials, int n_blocks, int nout, int n
 // partials + c * n_blocks * kScenarioStats; out[c * kScenarioStats + j].
 // n_cu, per_cu: compute units and resident blocks per CU, for the
 // co-residency-aware workgroup map (never changes results).
-hipError_t launch_surface(int ns, const PathArgs* cells, int n_cells, int n_blocks, const KStats* partials,
-                          KStats* out, int n_cu, int per_cu, hipStream_t s);
-int surface_occupancy_blocks_per_cu(int ns);
+// serial: every cell's serial_tail is set (the scenario-serial kernel).
+hipError_t launch_surface(int ns, bool serial, const PathArgs* cells, int n_cells, int n_blocks,
+                          const KStats* partials, KStats* out, int n_cu, int per_cu, hipStream_t s);
+int surface_occupancy_blocks_per_cu(int ns, bool serial);
 hipError_t launch_debug_philox(const uint32_t* ctr, uint32_t k0, uint32_t k1, uint64_t n, uint32_t* out,
                                hipStream_t s);
 hipError_t launch_debug_copy(const void* src, void* dst, uint64_t bytes, hipStream_t s);

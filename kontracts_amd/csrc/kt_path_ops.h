@@ -49,9 +49,11 @@ __device__ constexpr int xsrc(int sc) {
   return (SC == kScenarios && (sc == 1 || sc == 2)) ? 0 : sc;
 }
 
-template <int NS, int NOUT, int K, int SC, class PA>
-__device__ __forceinline__ void run_ops(const PA& A, int k, int begin, int count, float (&r)[SC][NS],
-                                        double (&pv)[SC][NOUT], const float (&x)[SC][K], bool live) {
+// The op interpreter over a register file accessed as R(sc, slot): the
+// caller's VGPR array, or a file in LDS.
+template <int NS, int NOUT, int K, int SC, class PA, class RF>
+__device__ __forceinline__ void run_ops_rf(const PA& A, int k, int begin, int count, const RF& R,
+                                           double (&pv)[SC][NOUT], const float (&x)[SC][K], bool live) {
   for (int i = 0; i < count; ++i) {
     const uint64_t word = ld_u64(A.ops, (size_t)begin + i);
     const uint2 w = make_uint2((uint32_t)word, (uint32_t)(word >> 32));
@@ -61,78 +63,99 @@ __device__ __forceinline__ void run_ops(const PA& A, int k, int begin, int count
     const uint32_t b = (w.x >> 24) & (NS - 1);
     const float imm = __uint_as_float(w.y);
     switch (code) {
-      case OP_CONST: FOR_SC r[sc][d] = imm; break;
+      case OP_CONST: FOR_SC R(sc, d) = imm; break;
       case OP_SPOT: {
         const uint32_t asset = (w.x >> 16) & 0xffu;
         const uint32_t ai = asset < (uint32_t)K ? asset : 0u;
-        FOR_SC r[sc][d] = scen_s0<K, SC>(A, sc, ai) * exp2_fast(x[xsrc<SC>(sc)][ai]);
+        FOR_SC R(sc, d) = scen_s0<K, SC>(A, sc, ai) * exp2_fast(x[xsrc<SC>(sc)][ai]);
         break;
       }
-      case OP_NEG: FOR_SC r[sc][d] = -r[sc][a]; break;
-      case OP_ADD: FOR_SC r[sc][d] = r[sc][a] + r[sc][b]; break;
-      case OP_SUB: FOR_SC r[sc][d] = r[sc][a] - r[sc][b]; break;
-      case OP_MUL: FOR_SC r[sc][d] = r[sc][a] * r[sc][b]; break;
-      case OP_DIV: FOR_SC r[sc][d] = r[sc][a] / r[sc][b]; break;
-      case OP_MAX: FOR_SC r[sc][d] = fmaxf(r[sc][a], r[sc][b]); break;
-      case OP_MIN: FOR_SC r[sc][d] = fminf(r[sc][a], r[sc][b]); break;
-      case OP_ADDI: FOR_SC r[sc][d] = r[sc][a] + imm; break;
-      case OP_SUBI: FOR_SC r[sc][d] = r[sc][a] - imm; break;
-      case OP_RSUBI: FOR_SC r[sc][d] = imm - r[sc][a]; break;
-      case OP_MULI: FOR_SC r[sc][d] = r[sc][a] * imm; break;
-      case OP_DIVI: FOR_SC r[sc][d] = r[sc][a] / imm; break;
-      case OP_RDIVI: FOR_SC r[sc][d] = imm / r[sc][a]; break;
-      case OP_MAXI: FOR_SC r[sc][d] = fmaxf(r[sc][a], imm); break;
-      case OP_MINI: FOR_SC r[sc][d] = fminf(r[sc][a], imm); break;
-      case OP_GE: FOR_SC r[sc][d] = r[sc][a] >= r[sc][b] ? 1.0f : 0.0f; break;
-      case OP_GT: FOR_SC r[sc][d] = r[sc][a] > r[sc][b] ? 1.0f : 0.0f; break;
-      case OP_LE: FOR_SC r[sc][d] = r[sc][a] <= r[sc][b] ? 1.0f : 0.0f; break;
-      case OP_LT: FOR_SC r[sc][d] = r[sc][a] < r[sc][b] ? 1.0f : 0.0f; break;
-      case OP_GEI: FOR_SC r[sc][d] = r[sc][a] >= imm ? 1.0f : 0.0f; break;
-      case OP_GTI: FOR_SC r[sc][d] = r[sc][a] > imm ? 1.0f : 0.0f; break;
-      case OP_LEI: FOR_SC r[sc][d] = r[sc][a] <= imm ? 1.0f : 0.0f; break;
-      case OP_LTI: FOR_SC r[sc][d] = r[sc][a] < imm ? 1.0f : 0.0f; break;
-      case OP_AND: FOR_SC r[sc][d] = r[sc][a] * r[sc][b]; break;
-      case OP_OR: FOR_SC r[sc][d] = fmaxf(r[sc][a], r[sc][b]); break;
-      case OP_NOT: FOR_SC r[sc][d] = 1.0f - r[sc][a]; break;
+      case OP_NEG: FOR_SC R(sc, d) = -R(sc, a); break;
+      case OP_ADD: FOR_SC R(sc, d) = R(sc, a) + R(sc, b); break;
+      case OP_SUB: FOR_SC R(sc, d) = R(sc, a) - R(sc, b); break;
+      case OP_MUL: FOR_SC R(sc, d) = R(sc, a) * R(sc, b); break;
+      case OP_DIV: FOR_SC R(sc, d) = R(sc, a) / R(sc, b); break;
+      case OP_MAX: FOR_SC R(sc, d) = fmaxf(R(sc, a), R(sc, b)); break;
+      case OP_MIN: FOR_SC R(sc, d) = fminf(R(sc, a), R(sc, b)); break;
+      case OP_ADDI: FOR_SC R(sc, d) = R(sc, a) + imm; break;
+      case OP_SUBI: FOR_SC R(sc, d) = R(sc, a) - imm; break;
+      case OP_RSUBI: FOR_SC R(sc, d) = imm - R(sc, a); break;
+      case OP_MULI: FOR_SC R(sc, d) = R(sc, a) * imm; break;
+      case OP_DIVI: FOR_SC R(sc, d) = R(sc, a) / imm; break;
+      case OP_RDIVI: FOR_SC R(sc, d) = imm / R(sc, a); break;
+      case OP_MAXI: FOR_SC R(sc, d) = fmaxf(R(sc, a), imm); break;
+      case OP_MINI: FOR_SC R(sc, d) = fminf(R(sc, a), imm); break;
+      case OP_GE: FOR_SC R(sc, d) = R(sc, a) >= R(sc, b) ? 1.0f : 0.0f; break;
+      case OP_GT: FOR_SC R(sc, d) = R(sc, a) > R(sc, b) ? 1.0f : 0.0f; break;
+      case OP_LE: FOR_SC R(sc, d) = R(sc, a) <= R(sc, b) ? 1.0f : 0.0f; break;
+      case OP_LT: FOR_SC R(sc, d) = R(sc, a) < R(sc, b) ? 1.0f : 0.0f; break;
+      case OP_GEI: FOR_SC R(sc, d) = R(sc, a) >= imm ? 1.0f : 0.0f; break;
+      case OP_GTI: FOR_SC R(sc, d) = R(sc, a) > imm ? 1.0f : 0.0f; break;
+      case OP_LEI: FOR_SC R(sc, d) = R(sc, a) <= imm ? 1.0f : 0.0f; break;
+      case OP_LTI: FOR_SC R(sc, d) = R(sc, a) < imm ? 1.0f : 0.0f; break;
+      case OP_AND: FOR_SC R(sc, d) = R(sc, a) * R(sc, b); break;
+      case OP_OR: FOR_SC R(sc, d) = fmaxf(R(sc, a), R(sc, b)); break;
+      case OP_NOT: FOR_SC R(sc, d) = 1.0f - R(sc, a); break;
       case OP_WHEN: {
         const uint32_t dn = (d + 1) & (NS - 1);
         FOR_SC {
-          const float n = r[sc][a] * (1.0f - r[sc][d]) * r[sc][b];
-          r[sc][d] = r[sc][d] + n;
-          r[sc][dn] = n;
+          const float n = R(sc, a) * (1.0f - R(sc, d)) * R(sc, b);
+          R(sc, d) = R(sc, d) + n;
+          R(sc, dn) = n;
         }
         break;
       }
-      case OP_UNTIL: FOR_SC r[sc][d] = fmaxf(r[sc][d], r[sc][a] * r[sc][b]); break;
+      case OP_UNTIL: FOR_SC R(sc, d) = fmaxf(R(sc, d), R(sc, a) * R(sc, b)); break;
       case OP_EMIT: {
         const uint32_t o = ((w.x >> 8) & 0xffu) & (NOUT - 1);
         FOR_SC {
           const double df = ld_f64(A.df, (size_t)A.df_index[sc] * A.n_grid + k);
-          pv[sc][o] += (double)(imm * (r[sc][a] * r[sc][b])) * df;
+          pv[sc][o] += (double)(imm * (R(sc, a) * R(sc, b))) * df;
         }
         break;
       }
-      case OP_RMAX: FOR_SC r[sc][d] = fmaxf(r[sc][d], r[sc][a]); break;
-      case OP_RMIN: FOR_SC r[sc][d] = fminf(r[sc][d], r[sc][a]); break;
+      case OP_RMAX: FOR_SC R(sc, d) = fmaxf(R(sc, d), R(sc, a)); break;
+      case OP_RMIN: FOR_SC R(sc, d) = fminf(R(sc, d), R(sc, a)); break;
       case OP_AVG: {
         const uint32_t dc = (d + 1) & (NS - 1);
         FOR_SC {
-          const float y = r[sc][a] - r[sc][dc];
-          const float t = r[sc][d] + y;
-          r[sc][dc] = (t - r[sc][d]) - y;
-          r[sc][d] = t;
+          const float y = R(sc, a) - R(sc, dc);
+          const float t = R(sc, d) + y;
+          R(sc, dc) = (t - R(sc, d)) - y;
+          R(sc, d) = t;
         }
         break;
       }
       case OP_FLAGERR: {
         bool hit = false;
-        FOR_SC hit = hit || r[sc][a] != 0.0f;
+        FOR_SC hit = hit || R(sc, a) != 0.0f;
         if (live && hit) atomicOr(A.err, 1u << ((uint32_t)imm & 31u));
         break;
       }
       default: break;
     }
   }
+}
+
+
+// A register array: slot i of scenario sc is r[sc][i].
+template <int SC, int NS>
+struct RegArray {
+  float (&r)[SC][NS];
+  __device__ __forceinline__ float& operator()(int sc, uint32_t i) const { return r[sc][i]; }
+};
+
+// One scenario's register file in LDS: slot i of this lane is base[i * stride].
+struct RegLds {
+  float* base;
+  int stride;
+  __device__ __forceinline__ float& operator()(int, uint32_t i) const { return base[i * (uint32_t)stride]; }
+};
+
+template <int NS, int NOUT, int K, int SC, class PA>
+__device__ __forceinline__ void run_ops(const PA& A, int k, int begin, int count, float (&r)[SC][NS],
+                                        double (&pv)[SC][NOUT], const float (&x)[SC][K], bool live) {
+  run_ops_rf<NS, NOUT, K, SC>(A, k, begin, count, RegArray<SC, NS>{r}, pv, x, live);
 }
 
 // Advance every scenario of one path to grid point k with that point's

@@ -101,6 +101,22 @@ inline int uniform_group_prefix(const KGroup* g, int n) {
   return i;
 }
 
+// Groups the GBM scenario kernels can interpret one scenario at a time after
+// the uniform prefix: at most this many.
+constexpr int kSerialTailGroups = 2;
+
+// 1 when the GBM scenario kernels run the program with its uniform prefix
+// followed by a scenario-serial tail: the prefix is a kind with a loop of
+// its own (no ops, knock-out >= or <=, running sum, running max) and at most
+// kSerialTailGroups groups follow it.
+inline int scenario_serial_tail(const KGroup* g, int n) {
+  const int nu = uniform_group_prefix(g, n);
+  if (nu <= 0) return 0;
+  const int k = g[0].kind;
+  const bool looped = k == G_NONE || k == G_KO_GE || k == G_KO_LE || k == G_AVG || k == G_RMAX;
+  return looped && n - nu <= kSerialTailGroups ? 1 : 0;
+}
+
 // Result statistics of one output over a set of paths: count, mean and the
 // sum of squared deviations (combined with Chan et al.'s pairwise formula).
 struct KStats {
