@@ -207,6 +207,8 @@ kt_engine::~kt_engine() {
   (void)hipSetDevice(device);
   if (pf_exec) (void)hipGraphExecDestroy(pf_exec);
   if (pf_graph) (void)hipGraphDestroy(pf_graph);
+  if (lsm_exec) (void)hipGraphExecDestroy(lsm_exec);
+  if (lsm_graph) (void)hipGraphDestroy(lsm_graph);
   if (stage_busy && stage_ev) (void)hipEventSynchronize(stage_ev);
   for (DevBuf* b : {&arena, &partials, &stats, &errw, &pv, &z, &scratch, &paths, &aux, &book, &errp})
     if (b->p) (void)hipFree(b->p);
@@ -799,6 +801,9 @@ kt_status kt_engine_set_option(kt_engine* E, const char* name, int64_t value) {
   } else if (key == "portfolio_graph") {
     if (value != 0 && value != 1) raise(KT_ERR_INVALID_ARGUMENT, "portfolio_graph must be 0 or 1");
     E->pf_use_graph = value == 1;
+  } else if (key == "lsm_graph") {
+    if (value != 0 && value != 1) raise(KT_ERR_INVALID_ARGUMENT, "lsm_graph must be 0 or 1");
+    E->lsm_use_graph = value == 1;
   } else {
     raise(KT_ERR_INVALID_ARGUMENT, "unknown engine option: " + key);
   }

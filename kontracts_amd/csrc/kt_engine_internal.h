@@ -40,6 +40,13 @@ struct kt_engine {
   hipGraph_t pf_graph = nullptr;
   hipGraphExec_t pf_exec = nullptr;
   std::vector<uint64_t> pf_key;
+  // Longstaff-Schwartz: the ~100 dependent launches of one pricing (paths
+  // kernel, backward passes, solves, finalize) captured in a hipGraph and
+  // replayed while every launch argument stays the same (lsm_graph option).
+  bool lsm_use_graph = true;
+  hipGraph_t lsm_graph = nullptr;
+  hipGraphExec_t lsm_exec = nullptr;
+  std::string lsm_key;
   // Host -> device staging (tables, arguments) and device -> host readback
   // (statistics, error words), both page-locked.  stage_ev marks the last
   // copy out of the staging buffer; it is waited on before the buffer is

@@ -784,9 +784,15 @@ __global__ void debug_normals_kernel(const uint32_t* __restrict__ w, uint64_t n,
 // the exercised payoff value_at(d) and the regression spot (asset 0) as one
 // FP32 pair, laid out [date][path] so that the backward passes stream them
 // with coalesced 8-byte loads.  The payoff program has unit discount factors.
-template <int MODEL, int K, int NS>
-__global__ __launch_bounds__(kBlock) void lsm_paths_kernel(const PathArgs A, float2* __restrict__ cols,
-                                                           uint64_t stride) {
+// VANILLA: the payoff at every date is scale * max(+-(S - strike), floor)
+// (LsmPayoff), evaluated as straight-line code with the interpreter's
+// operations in its order (spot, subtraction, max, the product with the
+// init constant 1 and the scale), so the stored values are the same; the
+// interpreter's per-op dispatch costs about 140 scalar instructions per
+// date, which bound the kernel.
+template <int MODEL, int K, int NS, bool VANILLA>
+__global__ __launch_bounds__(kBlock) void lsm_paths_kernel(const PathArgs A, const LsmPayoff pay,
+                                                           float2* __restrict__ cols, uint64_t stride) {
   constexpr int M = MODEL == MK_GBM ? 1 : (MODEL == MK_HESTON ? 2 : K);
   const uint64_t local = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
   const bool live = local < A.n_paths;
@@ -819,11 +825,22 @@ __global__ __launch_bounds__(kBlock) void lsm_paths_kernel(const PathArgs A, flo
       if (k < A.n_grid) {
         advance<MODEL, K, 1>(A, k, &z[s * M], x, v);
         if (k >= 1) {
-          double pv[1][1] = {{0.0}};
-          const uint64_t st = ld_u64(A.steps, (size_t)k);
-          const int begin = (int)(uint32_t)st, count = (int)(uint32_t)(st >> 32);
-          if (count > 0) run_ops<NS, 1, K, 1>(A, k, begin, count, r, pv, x, live);
-          if (live) cols[(size_t)(k - 1) * stride + local] = make_float2((float)pv[0][0], A.s0[0] * exp2_fast(x[0][0]));
+          const float S = A.s0[0] * exp2_fast(x[0][0]);
+          float payoff;
+          if constexpr (VANILLA) {
+            // no contraction of the spot's product into the subtraction: the
+            // interpreter rounds the spot before it subtracts
+#pragma clang fp contract(off)
+            const float d = pay.kind == 1 ? S - pay.strike : pay.strike - S;
+            payoff = (float)((double)(pay.scale * (fmaxf(d, pay.floor) * 1.0f)) * 1.0);
+          } else {
+            double pv[1][1] = {{0.0}};
+            const uint64_t st = ld_u64(A.steps, (size_t)k);
+            const int begin = (int)(uint32_t)st, count = (int)(uint32_t)(st >> 32);
+            if (count > 0) run_ops<NS, 1, K, 1>(A, k, begin, count, r, pv, x, live);
+            payoff = (float)pv[0][0];
+          }
+          if (live) cols[(size_t)(k - 1) * stride + local] = make_float2(payoff, S);
         }
       }
     }
@@ -897,49 +914,80 @@ __global__ __launch_bounds__(kBlock) void lsm_pass_kernel(const LsmPass P) {
 #pragma unroll
     for (int j = 0; j < MB; ++j) c[j] = ld_f64(P.coef, j);
   }
+  // Paths i = lane + t * step for trips t = 0, 1, ... (the same count on
+  // every lane); four trips per iteration, their loads issued before any of
+  // their arithmetic so that each lane keeps 12 loads in flight.  The sums
+  // still take the paths in trip order.
   const uint64_t step = (uint64_t)gridDim.x * kBlock;
-  const uint64_t n_round = (P.n + step - 1) / step * step;  // every lane runs the same trip count
-  for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < n_round; i += step) {
-    const bool live = i < P.n;
-    const uint64_t ii = live ? i : 0;
-    double val;
-    if (P.d_b < 0) {
-      val = (double)P.cols[(size_t)(P.n_dates - 1) * P.n + ii].x;
-    } else {
-      const double y = P.value[ii] * P.disc_b;
-      const float2 ps = P.cols[(size_t)P.d_b * P.n + ii];
-      val = y;
-      if (ps.x > 0.0f) {
-        double cont = y;
-        if (have) {
-          const double S = (double)ps.y;
-          cont = c[MB - 1];
+  const uint64_t trips = (P.n + step - 1) / step;
+  const uint64_t lane0 = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+  constexpr int U = 4;
+  for (uint64_t t0 = 0; t0 < trips; t0 += U) {
+    double vin[U];
+    float2 pb[U], pa[U];
 #pragma unroll
-          for (int j = MB - 2; j >= 0; --j) cont = cont * S + c[j];
+    for (int u = 0; u < U; ++u) {
+      const uint64_t i = lane0 + (t0 + u) * step;
+      const bool on = t0 + u < trips;  // uniform across the grid
+      const uint64_t ii = on && i < P.n ? i : 0;
+      vin[u] = 0.0;
+      pb[u] = make_float2(0.0f, 0.0f);
+      pa[u] = make_float2(0.0f, 0.0f);
+      if (on) {
+        if (P.d_b < 0) {
+          pb[u] = P.cols[(size_t)(P.n_dates - 1) * P.n + ii];
+        } else {
+          vin[u] = P.value[ii];
+          pb[u] = P.cols[(size_t)P.d_b * P.n + ii];
         }
-        val = (double)ps.x > cont ? (double)ps.x : y;
+        if (regress) pa[u] = P.cols[(size_t)P.d_a * P.n + ii];
       }
     }
-    if (P.final_pass) {
-      const double pv = val * P.disc0;
-      if (live) P.value[ii] = pv;
-      st = stats_combine(st, wave_stats(pv, live));
-    } else if (live) {
-      P.value[ii] = val;
-    }
-    if (regress) {
-      const float2 ps = P.cols[(size_t)P.d_a * P.n + ii];
-      if (live && ps.x > 0.0f) {
-        const double S = (double)ps.y;
-        const double y = val * P.disc_a;
-        double pw = 1.0;
 #pragma unroll
-        for (int p = 0; p < NP; ++p) {
-          acc[p] += pw;
-          if (p < MB) acc[NP + p] = fma(pw, y, acc[NP + p]);
-          pw *= S;
+    for (int u = 0; u < U; ++u) {
+      if (t0 + u >= trips) break;
+      const uint64_t i = lane0 + (t0 + u) * step;
+      const bool live = i < P.n;
+      const uint64_t ii = live ? i : 0;
+      double val;
+      if (P.d_b < 0) {
+        val = (double)pb[u].x;
+      } else {
+        const double y = vin[u] * P.disc_b;
+        const float2 ps = pb[u];
+        val = y;
+        if (ps.x > 0.0f) {
+          double cont = y;
+          if (have) {
+            const double S = (double)ps.y;
+            cont = c[MB - 1];
+#pragma unroll
+            for (int j = MB - 2; j >= 0; --j) cont = cont * S + c[j];
+          }
+          val = (double)ps.x > cont ? (double)ps.x : y;
         }
-        acc[NV - 1] += 1.0;
+      }
+      if (P.final_pass) {
+        const double pv = val * P.disc0;
+        if (live) P.value[ii] = pv;
+        st = stats_combine(st, wave_stats(pv, live));
+      } else if (live) {
+        P.value[ii] = val;
+      }
+      if (regress) {
+        const float2 ps = pa[u];
+        if (live && ps.x > 0.0f) {
+          const double S = (double)ps.y;
+          const double y = val * P.disc_a;
+          double pw = 1.0;
+#pragma unroll
+          for (int p = 0; p < NP; ++p) {
+            acc[p] += pw;
+            if (p < MB) acc[NP + p] = fma(pw, y, acc[NP + p]);
+            pw *= S;
+          }
+          acc[NV - 1] += 1.0;
+        }
       }
     }
   }
@@ -1153,15 +1201,20 @@ hipError_t launch_path(int model, int k, int ns, int nout, bool debug, int n_sce
   return hipErrorInvalidValue;
 }
 
-hipError_t launch_lsm_paths(int model, int k, int ns, const PathArgs& a, int n_blocks, float2* cols,
-                            uint64_t stride, hipStream_t s) {
+hipError_t launch_lsm_paths(int model, int k, int ns, const PathArgs& a, const LsmPayoff& pay, int n_blocks,
+                            float2* cols, uint64_t stride, hipStream_t s) {
 #define KT_LSM(MODEL, KK, NSV) \
-  hipLaunchKernelGGL((lsm_paths_kernel<MODEL, KK, NSV>), dim3(n_blocks), dim3(kBlock), 0, s, a, cols, stride)
+  hipLaunchKernelGGL((lsm_paths_kernel<MODEL, KK, NSV, false>), dim3(n_blocks), dim3(kBlock), 0, s, a, pay, cols, stride)
+#define KT_LSM_V(MODEL) \
+  hipLaunchKernelGGL((lsm_paths_kernel<MODEL, 1, 16, true>), dim3(n_blocks), dim3(kBlock), 0, s, a, pay, cols, stride)
+  const bool vanilla = pay.kind == 1 || pay.kind == 2;
   if (model == MK_GBM) {
-    if (ns == 16) KT_LSM(MK_GBM, 1, 16);
+    if (vanilla) KT_LSM_V(MK_GBM);
+    else if (ns == 16) KT_LSM(MK_GBM, 1, 16);
     else KT_LSM(MK_GBM, 1, 32);
   } else if (model == MK_HESTON) {
-    if (ns == 16) KT_LSM(MK_HESTON, 1, 16);
+    if (vanilla) KT_LSM_V(MK_HESTON);
+    else if (ns == 16) KT_LSM(MK_HESTON, 1, 16);
     else KT_LSM(MK_HESTON, 1, 32);
   } else {
     switch (k) {
@@ -1176,6 +1229,7 @@ hipError_t launch_lsm_paths(int model, int k, int ns, const PathArgs& a, int n_b
     }
   }
 #undef KT_LSM
+#undef KT_LSM_V
   return hipGetLastError();
 }
 

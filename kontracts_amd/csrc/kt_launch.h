@@ -140,9 +140,16 @@ struct LsmPassArgs {
   int n_dates, m, d_b, d_a, final_pass;
   double disc_b, disc_a, disc0;
 };
+// An exercised payoff that is, at every exercise date, the program
+// scale * max(S - strike, floor) (kind 1) or scale * max(strike - S, floor)
+// (kind 2) of asset 0 (lsm_vanilla_payoff); kind 0: interpret the program.
+struct LsmPayoff {
+  int kind;
+  float strike, floor, scale;
+};
 // Writes cols[(k-1)*stride + local path] = {payoff, spot} for k = 1..n_grid-1.
-hipError_t launch_lsm_paths(int model, int k_assets, int ns, const PathArgs& a, int n_blocks, float2* cols,
-                            uint64_t stride, hipStream_t s);
+hipError_t launch_lsm_paths(int model, int k_assets, int ns, const PathArgs& a, const LsmPayoff& pay, int n_blocks,
+                            float2* cols, uint64_t stride, hipStream_t s);
 // reg holds kLsmRegCount x n_blocks doubles; m in [2, kLsmBasisMax].
 hipError_t launch_lsm_pass(const LsmPassArgs& p, int n_blocks, hipStream_t s);
 // One block: reduce reg and write coef[0..kLsmBasisMax] (coefficients, fitted flag).

@@ -17,6 +17,42 @@
 
 namespace kt {
 
+namespace {
+
+// The exercised payoff as a vanilla put / call, when every exercise date's
+// op list is spot(asset 0) -> subi / rsubi strike -> maxi floor -> emit of
+// that times the constant-1 register with a scale: the paths kernel then
+// evaluates it as straight-line code (the same operations).
+LsmPayoff lsm_vanilla_payoff(const Program& prog, int model_kind) {
+  LsmPayoff none{0, 0.0f, 0.0f, 0.0f};
+  if (model_kind != MK_GBM && model_kind != MK_HESTON) return none;
+  bool one = false;
+  for (const KOp& op : prog.init) {
+    if (op.d != kSlotOne) continue;
+    one = op.code == OP_CONST && op.imm == 1.0f;
+  }
+  if (!one || prog.steps.size() < 2 || prog.steps[0].count != 0) return none;
+  LsmPayoff pay = none;
+  for (size_t k = 1; k < prog.steps.size(); ++k) {
+    const KStep& st = prog.steps[k];
+    if (st.count != 4 || (size_t)st.begin + 4 > prog.ops.size()) return none;
+    const KOp* o = &prog.ops[st.begin];
+    if (o[0].code != OP_SPOT || o[0].a != 0) return none;
+    if ((o[1].code != OP_SUBI && o[1].code != OP_RSUBI) || o[1].a != o[0].d) return none;
+    if (o[2].code != OP_MAXI || o[2].a != o[1].d) return none;
+    if (o[3].code != OP_EMIT || o[3].a != o[2].d || o[3].b != kSlotOne) return none;
+    if (o[0].d == kSlotOne || o[1].d == kSlotOne || o[2].d == kSlotOne) return none;
+    const LsmPayoff here{o[1].code == OP_SUBI ? 1 : 2, o[1].imm, o[2].imm, o[3].imm};
+    if (k > 1 && (here.kind != pay.kind || here.strike != pay.strike || here.floor != pay.floor ||
+                  here.scale != pay.scale))
+      return none;
+    pay = here;
+  }
+  return pay;
+}
+
+}  // namespace
+
 void price_american(kt_engine* E, const char* json, const double* dates, uint64_t n_dates, const kt_model* model,
                     const kt_mc_config* cfg, int n_basis, kt_price_result* out, double* pv_host,
                     double* coef_host) {
@@ -102,43 +138,86 @@ void price_american(kt_engine* E, const char* json, const double* dates, uint64_
   fill_model_args(A, md, cfg->seed);
   A.err = err;
 
-  E->prof_begin();
+  const LsmPayoff pay = lsm_vanilla_payoff(prog, kind);
   const uint64_t kMaxBlocksPerLaunch = 1ull << 20;
-  for (uint64_t b0 = 0; b0 < total_blocks; b0 += kMaxBlocksPerLaunch) {
-    const uint64_t nb = std::min(kMaxBlocksPerLaunch, total_blocks - b0);
-    A.path_base = b0 * kBlock;
-    A.n_paths = std::min<uint64_t>(nb * kBlock, N - b0 * kBlock);
-    check_hip(launch_lsm_paths(kind, K, ns, A, (int)nb, cols + b0 * kBlock, N, E->stream), "LSM paths launch");
-  }
-
   auto disc = [&](int d_from, int d_to) {  // exercise-date indices, d_from < d_to
     return std::exp(-rate * (grid[d_to + 1] - grid[d_from + 1]));
   };
   auto coef_row = [&](int d) { return coef + (size_t)d * (kLsmBasisMax + 1); };
-  LsmPassArgs P{};
-  P.cols = cols;
-  P.value = value;
-  P.reg = reg;
-  P.stats = partials;
-  P.n = N;
-  P.n_dates = D;
-  P.m = m;
-  P.disc0 = std::exp(-rate * grid[1]);
   // Pass for date d: decide at d (d = D-1: start from the last payoff), then
   // accumulate date d-1's regression, solved before the next pass; the last
   // pass (d = 0) discounts to t = 0.
+  std::vector<LsmPassArgs> passes;
   for (int d = D - 1; d >= 0; --d) {
+    LsmPassArgs P{};
+    P.cols = cols;
+    P.value = value;
+    P.reg = reg;
+    P.stats = partials;
+    P.n = N;
+    P.n_dates = D;
+    P.m = m;
+    P.disc0 = std::exp(-rate * grid[1]);
     P.d_b = d == D - 1 ? -1 : d;
     P.coef = coef_row(d == D - 1 ? 0 : d);  // unused on the first pass
     P.disc_b = d == D - 1 ? 0.0 : disc(d, d + 1);
     P.d_a = d - 1;
     P.disc_a = d >= 1 ? disc(d - 1, d) : 0.0;
     P.final_pass = d == 0;
-    check_hip(launch_lsm_pass(P, nbp, E->stream), "LSM pass launch");
-    if (d >= 1) check_hip(launch_lsm_solve(reg, nbp, m, coef_row(d - 1), E->stream), "LSM solve launch");
+    passes.push_back(P);
+  }
+  // The whole dependent chain, from the paths kernel to the finalize.
+  auto enqueue = [&]() -> hipError_t {
+    for (uint64_t b0 = 0; b0 < total_blocks; b0 += kMaxBlocksPerLaunch) {
+      const uint64_t nb = std::min(kMaxBlocksPerLaunch, total_blocks - b0);
+      PathArgs Ab = A;
+      Ab.path_base = b0 * kBlock;
+      Ab.n_paths = std::min<uint64_t>(nb * kBlock, N - b0 * kBlock);
+      hipError_t e = launch_lsm_paths(kind, K, ns, Ab, pay, (int)nb, cols + b0 * kBlock, N, E->stream);
+      if (e != hipSuccess) return e;
+    }
+    for (int i = 0; i < D; ++i) {
+      hipError_t e = launch_lsm_pass(passes[i], nbp, E->stream);
+      if (e != hipSuccess) return e;
+      const int d = D - 1 - i;
+      if (d >= 1) {
+        e = launch_lsm_solve(reg, nbp, m, coef_row(d - 1), E->stream);
+        if (e != hipSuccess) return e;
+      }
+    }
+    return launch_finalize(partials, nbp, 1, 1, stats, E->stream);
+  };
+  E->prof_begin();
+  if (!E->lsm_use_graph) {
+    check_hip(enqueue(), "LSM launch");
+  } else {
+    // the graph replays while every launch argument is the same
+    std::string key((const char*)&A, sizeof(A));
+    key.append((const char*)&pay, sizeof(pay));
+    for (const LsmPassArgs& P : passes) key.append((const char*)&P, sizeof(P));
+    const uint64_t shape[6] = {N, (uint64_t)D, (uint64_t)m, (uint64_t)nbp, (uint64_t)(uintptr_t)stats,
+                               (uint64_t)kind * 64 + (uint64_t)K * 4 + (uint64_t)(ns == 16)};
+    key.append((const char*)shape, sizeof(shape));
+    if (!E->lsm_exec || E->lsm_key != key) {
+      if (E->lsm_exec) (void)hipGraphExecDestroy(E->lsm_exec);
+      if (E->lsm_graph) (void)hipGraphDestroy(E->lsm_graph);
+      E->lsm_exec = nullptr;
+      E->lsm_graph = nullptr;
+      E->lsm_key.clear();
+      check_hip(hipStreamBeginCapture(E->stream, hipStreamCaptureModeThreadLocal), "begin capture");
+      const hipError_t le = enqueue();
+      hipGraph_t g = nullptr;
+      const hipError_t ce = hipStreamEndCapture(E->stream, &g);
+      if (g && (le != hipSuccess || ce != hipSuccess)) (void)hipGraphDestroy(g);
+      check_hip(le, "LSM launch");
+      check_hip(ce, "end capture");
+      E->lsm_graph = g;
+      check_hip(hipGraphInstantiate(&E->lsm_exec, g, nullptr, nullptr, 0), "graph instantiate");
+      E->lsm_key = key;
+    }
+    check_hip(hipGraphLaunch(E->lsm_exec, E->stream), "graph launch");
   }
   E->prof_end((double)N * (double)D);
-  check_hip(launch_finalize(partials, nbp, 1, 1, stats, E->stream), "finalize launch");
 
   KStats st{};
   unsigned err_bits = 0;

@@ -197,3 +197,22 @@ def test_lsm_zero_paths_and_tiny_counts(eng):
     for n in (1, 2, 3, 255, 257):
         compare_with_oracle(eng, put, model, O.gbm("S", 36.0, 0.06, 0.2), dates_uniform(1.0, 5), n, 9,
                             flip_budget=0.0)
+
+
+@pytest.mark.parametrize("kind", ["put", "call"])
+def test_lsm_straight_line_payoff_equals_interpreter(eng, kind):
+    """A vanilla put / call takes the paths kernel's straight-line payoff; the
+    same payoff plus 0.0 (one more op) is interpreted.  The two must store
+    the same payoffs, fit the same regressions and give every path the same
+    value, to the last bit."""
+    k = 40.0
+    base = (K.const_(k) - K.spot("S")) if kind == "put" else (K.spot("S") - k)
+    vanilla = K.scale(base.max(0.0), K.one("USD"))
+    interpreted = K.scale(base.max(0.0) + 0.0, K.one("USD"))
+    model = K.GBM(36.0, 0.2, 0.06, asset="S")
+    dates = dates_uniform(1.0, 50)
+    gv, pv_v, cv = gpu_american(eng, vanilla, model, dates, 100_000, 42)
+    gi, pv_i, ci = gpu_american(eng, interpreted, model, dates, 100_000, 42)
+    assert np.array_equal(pv_v, pv_i)
+    assert np.array_equal(cv, ci)
+    assert gv.price == gi.price and gv.std_error == gi.std_error
