@@ -273,6 +273,10 @@ kt_status kt_engine_kernel_time(kt_engine* engine, double* total_ms, uint64_t* n
  *                      captured hipGraph while the book keeps its shape; 0
  *                      (default) launches the three kernels directly, which
  *                      measured faster on ROCm for books of 10-100 jobs.
+ *   "lsm_graph":       1 (default) replays kt_price_american's launch chain
+ *                      (paths kernel, backward passes, solves) from a
+ *                      captured hipGraph while its launch arguments are
+ *                      unchanged; 0 launches each kernel directly.
  * Results never depend on these: each job's reduction order is fixed.
  * Unknown names or values return KT_ERR_INVALID_ARGUMENT. */
 kt_status kt_engine_set_option(kt_engine* engine, const char* name, int64_t value);
