@@ -273,6 +273,11 @@ kt_status kt_engine_kernel_time(kt_engine* engine, double* total_ms, uint64_t* n
  *                      captured hipGraph while the book keeps its shape; 0
  *                      (default) launches the three kernels directly, which
  *                      measured faster on ROCm for books of 10-100 jobs.
+ *   "portfolio_waves": register budget of the persistent price kernel: 0
+ *                      (default: the compiler's), 6 or 8 waves per SIMD.
+ *   "portfolio_adjoint_waves": register budget of the persistent Greeks
+ *                      kernels: 0 (default: 128 VGPRs for Heston and baskets,
+ *                      the compiler's otherwise) or 5 waves per SIMD.
  *   "lsm_graph":       1 (default) replays kt_price_american's launch chain
  *                      (paths kernel, backward passes, solves) from a
  *                      captured hipGraph while its launch arguments are

@@ -816,36 +816,23 @@ __device__ __forceinline__ void portfolio_adjoint_body(const AdjPortfolioArgs& Q
   }
 }
 
+// The register budget per variant: the Heston and basket variants need a
+// few registers more than 128, which would cost a quarter of their occupancy
+// (3 waves per SIMD instead of 4), so they are capped at 128 (4 waves); the
+// others take the compiler's choice.  WAVES = 5 caps every variant at 102
+// VGPRs (a tile the autotuner may pick).
 template <int VARIANT>
-__global__ __launch_bounds__(kBlock) void portfolio_adjoint_kernel(const AdjPortfolioArgs Q) {
+constexpr int adjoint_default_waves() {
+  return (VARIANT == APV_BASKET5 || VARIANT == APV_HESTON_FEEDS || VARIANT == APV_HESTON ||
+          VARIANT == APV_HESTON_FEEDS_T)
+             ? 4
+             : 1;
+}
+
+template <int VARIANT, int WAVES>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(
+    WAVES > 0 ? WAVES : adjoint_default_waves<VARIANT>()))) void portfolio_adjoint_kernel(const AdjPortfolioArgs Q) {
   portfolio_adjoint_body<VARIANT>(Q);
-}
-
-// The basket and Heston variants need a few registers more than 128, which
-// would cost a quarter of their occupancy (3 waves per SIMD instead of 4):
-// cap them at 128.
-template <>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void
-portfolio_adjoint_kernel<APV_BASKET5>(const AdjPortfolioArgs Q) {
-  portfolio_adjoint_body<APV_BASKET5>(Q);
-}
-
-template <>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void
-portfolio_adjoint_kernel<APV_HESTON_FEEDS>(const AdjPortfolioArgs Q) {
-  portfolio_adjoint_body<APV_HESTON_FEEDS>(Q);
-}
-
-template <>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void
-portfolio_adjoint_kernel<APV_HESTON>(const AdjPortfolioArgs Q) {
-  portfolio_adjoint_body<APV_HESTON>(Q);
-}
-
-template <>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void
-portfolio_adjoint_kernel<APV_HESTON_FEEDS_T>(const AdjPortfolioArgs Q) {
-  portfolio_adjoint_body<APV_HESTON_FEEDS_T>(Q);
 }
 
 // Block b: output b % kAdjMaxOut of job b / kAdjMaxOut (outputs beyond the
@@ -877,50 +864,47 @@ int portfolio_adjoint_variant(int model, int k_assets, int n_slots, int n_acc, b
   return -1;
 }
 
-int portfolio_adjoint_occupancy(int variant) {
+namespace {
+
+template <int VARIANT>
+int adjoint_occupancy_of(int waves) {
   int n = 0;
-  hipError_t e = hipErrorInvalidValue;
-  switch (variant) {
-    case APV_GBM: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, portfolio_adjoint_kernel<APV_GBM>, kBlock, 0); break;
-    case APV_HESTON:
-      e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, portfolio_adjoint_kernel<APV_HESTON>, kBlock, 0);
-      break;
-    case APV_HESTON_FEEDS:
-      e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, portfolio_adjoint_kernel<APV_HESTON_FEEDS>, kBlock, 0);
-      break;
-    case APV_BASKET5:
-      e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, portfolio_adjoint_kernel<APV_BASKET5>, kBlock, 0);
-      break;
-    case APV_HESTON_T:
-      e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, portfolio_adjoint_kernel<APV_HESTON_T>, kBlock, 0);
-      break;
-    case APV_HESTON_FEEDS_T:
-      e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, portfolio_adjoint_kernel<APV_HESTON_FEEDS_T>, kBlock, 0);
-      break;
-  }
+  const hipError_t e =
+      waves == 5 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, portfolio_adjoint_kernel<VARIANT, 5>, kBlock, 0)
+                 : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, portfolio_adjoint_kernel<VARIANT, 0>, kBlock, 0);
   if (e != hipSuccess) return 1;
   return n > 0 ? n : 1;
 }
 
-hipError_t launch_portfolio_adjoint(int variant, const AdjPortfolioArgs& q, int grid, hipStream_t s) {
+template <int VARIANT>
+void launch_adjoint_variant(const AdjPortfolioArgs& q, int grid, hipStream_t s, int waves) {
+  if (waves == 5) hipLaunchKernelGGL((portfolio_adjoint_kernel<VARIANT, 5>), dim3(grid), dim3(kBlock), 0, s, q);
+  else hipLaunchKernelGGL((portfolio_adjoint_kernel<VARIANT, 0>), dim3(grid), dim3(kBlock), 0, s, q);
+}
+
+}  // namespace
+
+int portfolio_adjoint_occupancy(int variant, int waves) {
+  switch (variant) {
+    case APV_GBM: return adjoint_occupancy_of<APV_GBM>(waves);
+    case APV_HESTON: return adjoint_occupancy_of<APV_HESTON>(waves);
+    case APV_HESTON_FEEDS: return adjoint_occupancy_of<APV_HESTON_FEEDS>(waves);
+    case APV_BASKET5: return adjoint_occupancy_of<APV_BASKET5>(waves);
+    case APV_HESTON_T: return adjoint_occupancy_of<APV_HESTON_T>(waves);
+    case APV_HESTON_FEEDS_T: return adjoint_occupancy_of<APV_HESTON_FEEDS_T>(waves);
+  }
+  return 1;
+}
+
+hipError_t launch_portfolio_adjoint(int variant, const AdjPortfolioArgs& q, int grid, hipStream_t s, int waves) {
   if (grid <= 0) return hipErrorInvalidValue;
   switch (variant) {
-    case APV_GBM: hipLaunchKernelGGL(portfolio_adjoint_kernel<APV_GBM>, dim3(grid), dim3(kBlock), 0, s, q); break;
-    case APV_HESTON:
-      hipLaunchKernelGGL(portfolio_adjoint_kernel<APV_HESTON>, dim3(grid), dim3(kBlock), 0, s, q);
-      break;
-    case APV_HESTON_FEEDS:
-      hipLaunchKernelGGL(portfolio_adjoint_kernel<APV_HESTON_FEEDS>, dim3(grid), dim3(kBlock), 0, s, q);
-      break;
-    case APV_BASKET5:
-      hipLaunchKernelGGL(portfolio_adjoint_kernel<APV_BASKET5>, dim3(grid), dim3(kBlock), 0, s, q);
-      break;
-    case APV_HESTON_T:
-      hipLaunchKernelGGL(portfolio_adjoint_kernel<APV_HESTON_T>, dim3(grid), dim3(kBlock), 0, s, q);
-      break;
-    case APV_HESTON_FEEDS_T:
-      hipLaunchKernelGGL(portfolio_adjoint_kernel<APV_HESTON_FEEDS_T>, dim3(grid), dim3(kBlock), 0, s, q);
-      break;
+    case APV_GBM: launch_adjoint_variant<APV_GBM>(q, grid, s, waves); break;
+    case APV_HESTON: launch_adjoint_variant<APV_HESTON>(q, grid, s, waves); break;
+    case APV_HESTON_FEEDS: launch_adjoint_variant<APV_HESTON_FEEDS>(q, grid, s, waves); break;
+    case APV_BASKET5: launch_adjoint_variant<APV_BASKET5>(q, grid, s, waves); break;
+    case APV_HESTON_T: launch_adjoint_variant<APV_HESTON_T>(q, grid, s, waves); break;
+    case APV_HESTON_FEEDS_T: launch_adjoint_variant<APV_HESTON_FEEDS_T>(q, grid, s, waves); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();

@@ -801,6 +801,12 @@ kt_status kt_engine_set_option(kt_engine* E, const char* name, int64_t value) {
   } else if (key == "portfolio_graph") {
     if (value != 0 && value != 1) raise(KT_ERR_INVALID_ARGUMENT, "portfolio_graph must be 0 or 1");
     E->pf_use_graph = value == 1;
+  } else if (key == "portfolio_waves") {
+    if (value != 0 && value != 6 && value != 8) raise(KT_ERR_INVALID_ARGUMENT, "portfolio_waves must be 0, 6 or 8");
+    E->pf_waves = (int)value;
+  } else if (key == "portfolio_adjoint_waves") {
+    if (value != 0 && value != 5) raise(KT_ERR_INVALID_ARGUMENT, "portfolio_adjoint_waves must be 0 or 5");
+    E->pf_adj_waves = (int)value;
   } else if (key == "lsm_graph") {
     if (value != 0 && value != 1) raise(KT_ERR_INVALID_ARGUMENT, "lsm_graph must be 0 or 1");
     E->lsm_use_graph = value == 1;

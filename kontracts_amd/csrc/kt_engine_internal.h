@@ -34,6 +34,8 @@ struct kt_engine {
   // persistent portfolio launch: tile choice and the captured launch graph
   int pf_blocks_per_cu = 0;  // 0: the kernel's occupancy limit
   int pf_chunk = 1;
+  int pf_waves = 0;          // register-budget tile of the price kernel: 0, 6 or 8 waves per SIMD
+  int pf_adj_waves = 0;      // ... of the Greeks kernels: 0 (per-variant default) or 5
   // Direct launches by default: on this stack a hipGraphLaunch of the
   // three-kernel sequence costs more host time than the three launches.
   bool pf_use_graph = false;
@@ -59,9 +61,9 @@ struct kt_engine {
   // calls unless a call was interrupted.
   DevBuf errp;
   bool errp_zero = false;
-  int occupancy_portfolio = 0;  // cached occupancies (blocks per CU)
+  int occupancy_portfolio[3] = {0, 0, 0};  // cached occupancies (blocks per CU): price kernel at 0 / 6 / 8 waves
   int occupancy_surface[4] = {0, 0, 0, 0};  // Greek surface kernel: 8 / 16 registers, then their serial-tail variants
-  int occupancy_adjoint[kt::APV_COUNT] = {0, 0, 0, 0, 0, 0};
+  int occupancy_adjoint[2][kt::APV_COUNT] = {};  // Greeks kernels at the default budget / 5 waves
   bool pathwise_tangent = false;  // Heston pathwise Greeks: forward tangents instead of the reverse sweep
   bool profiling = false;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;

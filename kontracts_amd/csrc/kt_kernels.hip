@@ -615,8 +615,11 @@ __global__ void portfolio_reset_kernel(unsigned* counter, unsigned* err, int n_j
   for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n_jobs; j += gridDim.x * blockDim.x) err[j] = 0u;
 }
 
-template <int NS>
-__global__ __launch_bounds__(kBlock) void portfolio_kernel(const PortfolioArgs Q) {
+// WAVES > 0: the register budget for WAVES waves per SIMD (a tile the
+// autotuner may pick); 0 leaves it to the compiler.
+template <int NS, int WAVES>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WAVES > 0 ? WAVES : 1))) void portfolio_kernel(
+    const PortfolioArgs Q) {
   __shared__ uint32_t s_first;
   float r[1][NS];
   const KT_CONST PortfolioJob* jobs = (const KT_CONST PortfolioJob*)Q.jobs;
@@ -1267,17 +1270,27 @@ hipError_t launch_diff_stats(const double* a, const double* b, double scale, uin
   return hipGetLastError();
 }
 
-int portfolio_occupancy_blocks_per_cu() {
+int portfolio_occupancy_blocks_per_cu(int waves) {
   int n = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, portfolio_kernel<16>, kBlock, 0) != hipSuccess) return 4;
+  hipError_t e;
+  switch (waves) {
+    case 6: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, portfolio_kernel<16, 6>, kBlock, 0); break;
+    case 8: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, portfolio_kernel<16, 8>, kBlock, 0); break;
+    default: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, portfolio_kernel<16, 0>, kBlock, 0); break;
+  }
+  if (e != hipSuccess) return 4;
   return n > 0 ? n : 1;
 }
 
 hipError_t launch_portfolio(const PortfolioArgs& q, const int* n_blocks, int n_jobs, int grid, unsigned* err,
-                            KStats* out, unsigned* err_copy, hipStream_t s) {
+                            KStats* out, unsigned* err_copy, hipStream_t s, int waves) {
   hipLaunchKernelGGL(portfolio_reset_kernel, dim3(std::max(1, std::min(64, (n_jobs + 255) / 256))), dim3(256), 0, s,
                      q.counter, err, n_jobs);
-  hipLaunchKernelGGL(portfolio_kernel<16>, dim3(grid), dim3(kBlock), 0, s, q);
+  switch (waves) {
+    case 6: hipLaunchKernelGGL((portfolio_kernel<16, 6>), dim3(grid), dim3(kBlock), 0, s, q); break;
+    case 8: hipLaunchKernelGGL((portfolio_kernel<16, 8>), dim3(grid), dim3(kBlock), 0, s, q); break;
+    default: hipLaunchKernelGGL((portfolio_kernel<16, 0>), dim3(grid), dim3(kBlock), 0, s, q); break;
+  }
   hipLaunchKernelGGL(portfolio_finalize_kernel, dim3(n_jobs), dim3(256), 0, s, q.jobs, n_blocks, out, err_copy);
   return hipGetLastError();
 }

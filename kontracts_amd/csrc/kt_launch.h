@@ -120,8 +120,10 @@ struct AdjPortfolioArgs {
 };
 // -1 when no portfolio variant covers the job (it runs per contract).
 int portfolio_adjoint_variant(int model, int k_assets, int n_slots, int n_acc, bool tangent);
-int portfolio_adjoint_occupancy(int variant);
-hipError_t launch_portfolio_adjoint(int variant, const AdjPortfolioArgs& q, int grid, hipStream_t s);
+// waves: the register-budget tile of the persistent Greeks kernels, 0 for
+// each variant's default budget or 5 (at most 102 VGPRs: 5 waves per SIMD).
+int portfolio_adjoint_occupancy(int variant, int waves = 0);
+hipError_t launch_portfolio_adjoint(int variant, const AdjPortfolioArgs& q, int grid, hipStream_t s, int waves = 0);
 // out[job][kAdjMaxOut]: the first n_out[job] outputs of each job.
 hipError_t launch_portfolio_adjoint_finalize(const AdjArgs* jobs, const int* n_blocks, const int* n_out, int n_jobs,
                                              KStats* out, hipStream_t s);
@@ -174,10 +176,12 @@ struct PortfolioArgs {
 // Reset the queue and the jobs' error words err[n_jobs], run the persistent
 // grid, finalize each job into out[job] and copy its error word to
 // err_copy[job].
+// waves: the register-budget tile of the persistent price kernel: 0 for the
+// compiler's choice, 6 or 8 waves per SIMD (at most 80 / 64 VGPRs).
 hipError_t launch_portfolio(const PortfolioArgs& q, const int* n_blocks, int n_jobs, int grid, unsigned* err,
-                            KStats* out, unsigned* err_copy, hipStream_t s);
+                            KStats* out, unsigned* err_copy, hipStream_t s, int waves = 0);
 // Resident 256-lane blocks of the persistent kernel per compute unit.
-int portfolio_occupancy_blocks_per_cu();
+int portfolio_occupancy_blocks_per_cu(int waves = 0);
 
 // partials[block] = statistics of (a[i] - b[i]) * scale over the block's paths.
 hipError_t launch_diff_stats(const double* a, const double* b, double scale, uint64_t n, KStats* partials,

@@ -212,8 +212,9 @@ void price_portfolio(kt_engine* E, const kt_job* jobs, uint64_t n_jobs, kt_price
     const int* nblk_dev = (const int*)(dev + jobs_bytes + items_bytes);
     int n_cu = 0;
     check_hip(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, E->device), "device attribute");
-    if (E->occupancy_portfolio <= 0) E->occupancy_portfolio = portfolio_occupancy_blocks_per_cu();
-    const int bpc = E->pf_blocks_per_cu > 0 ? E->pf_blocks_per_cu : E->occupancy_portfolio;
+    int& occ = E->occupancy_portfolio[E->pf_waves == 6 ? 1 : E->pf_waves == 8 ? 2 : 0];
+    if (occ <= 0) occ = portfolio_occupancy_blocks_per_cu(E->pf_waves);
+    const int bpc = E->pf_blocks_per_cu > 0 ? E->pf_blocks_per_cu : occ;
     const uint64_t slots = (uint64_t)n_cu * bpc;
     // A queue chunk of c items saves atomics only where items outnumber the
     // resident workgroups c times over; on a smaller book it would serialise
@@ -224,10 +225,11 @@ void price_portfolio(kt_engine* E, const kt_job* jobs, uint64_t n_jobs, kt_price
 
     // Replay the captured launch sequence while the book keeps its shape.
     const std::vector<uint64_t> key = {(uint64_t)dev, (uint64_t)partials, (uint64_t)stats, (uint64_t)errw,
-                                       n_items, book.size(), (uint64_t)grid, (uint64_t)Q.chunk};
+                                       n_items, book.size(), (uint64_t)grid, (uint64_t)Q.chunk,
+                                       (uint64_t)E->pf_waves};
     E->prof_begin();
     if (!E->pf_use_graph) {
-      check_hip(launch_portfolio(Q, nblk_dev, (int)book.size(), grid, errw, stats, err_copy, E->stream),
+      check_hip(launch_portfolio(Q, nblk_dev, (int)book.size(), grid, errw, stats, err_copy, E->stream, E->pf_waves),
                 "portfolio launch");
     } else if (!E->pf_exec || E->pf_key != key) {
       if (E->pf_exec) (void)hipGraphExecDestroy(E->pf_exec);
@@ -236,7 +238,8 @@ void price_portfolio(kt_engine* E, const kt_job* jobs, uint64_t n_jobs, kt_price
       E->pf_graph = nullptr;
       E->pf_key.clear();
       check_hip(hipStreamBeginCapture(E->stream, hipStreamCaptureModeThreadLocal), "begin capture");
-      hipError_t le = launch_portfolio(Q, nblk_dev, (int)book.size(), grid, errw, stats, err_copy, E->stream);
+      hipError_t le =
+          launch_portfolio(Q, nblk_dev, (int)book.size(), grid, errw, stats, err_copy, E->stream, E->pf_waves);
       hipGraph_t g = nullptr;
       hipError_t ce = hipStreamEndCapture(E->stream, &g);
       check_hip(le, "portfolio launch");

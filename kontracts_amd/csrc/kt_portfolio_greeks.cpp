@@ -167,9 +167,9 @@ void price_portfolio_greeks(kt_engine* E, const kt_job* jobs, uint64_t n_jobs, k
     int grid_v[APV_COUNT] = {0, 0, 0, 0, 0, 0}, chunk_v[APV_COUNT] = {1, 1, 1, 1, 1, 1};
     for (int v = 0; v < APV_COUNT; ++v) {
       if (vitems[v].empty()) continue;
-      if (E->occupancy_adjoint[v] <= 0) E->occupancy_adjoint[v] = portfolio_adjoint_occupancy(v);
-      const int bpc = E->pf_blocks_per_cu > 0 ? std::min(E->pf_blocks_per_cu, E->occupancy_adjoint[v])
-                                              : E->occupancy_adjoint[v];
+      int& occ = E->occupancy_adjoint[E->pf_adj_waves == 5 ? 1 : 0][v];
+      if (occ <= 0) occ = portfolio_adjoint_occupancy(v, E->pf_adj_waves);
+      const int bpc = E->pf_blocks_per_cu > 0 ? std::min(E->pf_blocks_per_cu, occ) : occ;
       // as for prices: no chunk larger than the items per resident workgroup
       const uint64_t slots = (uint64_t)n_cu * bpc, n_it = vitems[v].size() / 2;
       chunk_v[v] = (int)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)std::max(1, E->pf_chunk), n_it / slots));
@@ -225,7 +225,7 @@ void price_portfolio_greeks(kt_engine* E, const kt_job* jobs, uint64_t n_jobs, k
       Q.counter = counters + v;
       Q.ck = (v == APV_HESTON || v == APV_HESTON_FEEDS) ? ck : nullptr;
       Q.ck_floats = ck_floats;
-      check_hip(launch_portfolio_adjoint(v, Q, grid_v[v], E->stream), "portfolio adjoint launch");
+      check_hip(launch_portfolio_adjoint(v, Q, grid_v[v], E->stream, E->pf_adj_waves), "portfolio adjoint launch");
     }
     double path_steps = 0.0;
     for (const AdjJob& a : adj) path_steps += (double)jobs[a.index].cfg.n_paths * (double)(a.grid.size() - 1);

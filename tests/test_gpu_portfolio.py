@@ -87,9 +87,48 @@ def test_portfolio_tile_configs_and_graph_replay():
         eng.set_option("portfolio_blocks_per_cu", 0)
         eng.set_option("portfolio_chunk", 1)
         eng.set_option("portfolio_graph", 0)
-    for bad in (("portfolio_chunk", 0), ("portfolio_blocks_per_cu", -1), ("portfolio_graph", 2)):
+    for bad in (("portfolio_chunk", 0), ("portfolio_blocks_per_cu", -1), ("portfolio_graph", 2),
+                ("portfolio_waves", 5), ("portfolio_adjoint_waves", 4), ("lsm_graph", 2)):
         with pytest.raises(N.KontractError):
             eng.set_option(*bad)
+
+
+def test_portfolio_register_budget_tiles():
+    """The register-budget variants of the persistent kernels run the same
+    code: every job stays identical to per-contract pricing (and Greeks)."""
+    eng = N.engine(0)
+    jobs = small_book(24) + edge_jobs()
+    want = price_each(jobs)
+    gjobs = small_book(12)
+    gwant = per_contract_greeks(gjobs)
+    try:
+        for pw in (6, 8):
+            eng.set_option("portfolio_waves", pw)
+            assert_identical(K.price_portfolio(jobs), want)
+        eng.set_option("portfolio_waves", 0)
+        eng.set_option("portfolio_adjoint_waves", 5)
+        assert_greeks_identical(K.price_portfolio(gjobs, greeks=True), gwant)
+    finally:
+        eng.set_option("portfolio_waves", 0)
+        eng.set_option("portfolio_adjoint_waves", 0)
+
+
+def test_autotuner_sets_a_measured_tile():
+    from kontracts_amd import tune as T
+    eng = N.engine(0)
+    jobs = small_book(16)
+    want = price_each(jobs)
+    try:
+        r = T.autotune_portfolio(jobs, candidates=((0, 1), (2, 4)), repeats=1)
+        assert r["best"] in {tuple(int(x.strip("waw")) for x in k.replace("x", "/").split("/"))
+                             for k in r["device_ms"]}
+        assert len(r["device_ms"]) == 2 + 2  # two (grid, chunk) pairs, then price budgets 6 and 8
+        assert all(v > 0 for v in r["device_ms"].values())
+        assert_identical(K.price_portfolio(jobs), want)  # under the tuned tile
+    finally:
+        for name in ("portfolio_blocks_per_cu", "portfolio_waves", "portfolio_adjoint_waves"):
+            eng.set_option(name, 0)
+        eng.set_option("portfolio_chunk", 1)
 
 
 def test_portfolio_errors_like_a_loop():
