@@ -209,33 +209,63 @@ kt_engine::~kt_engine() {
   if (pf_graph) (void)hipGraphDestroy(pf_graph);
   if (lsm_exec) (void)hipGraphExecDestroy(lsm_exec);
   if (lsm_graph) (void)hipGraphDestroy(lsm_graph);
-  if (stage_busy && stage_ev) (void)hipEventSynchronize(stage_ev);
-  for (DevBuf* b : {&arena, &partials, &stats, &errw, &pv, &z, &scratch, &paths, &aux, &book, &errp})
+  if (stream) (void)hipStreamSynchronize(stream);
+  for (DevBuf* b : {&arena, &partials, &stats, &errw, &pv, &z, &scratch, &paths, &aux, &book, &errp, &book2, &partials2,
+                    &stats2, &errw2})
     if (b->p) (void)hipFree(b->p);
-  for (PinBuf* b : {&stage_buf, &readback_buf})
-    if (b->p) (void)hipHostFree(b->p);
-  if (stage_ev) (void)hipEventDestroy(stage_ev);
-  if (ev0) (void)hipEventDestroy(ev0);
-  if (ev1) (void)hipEventDestroy(ev1);
+  for (Staging& st : staging) {
+    if (st.buf.p) (void)hipHostFree(st.buf.p);
+    if (st.ev) (void)hipEventDestroy(st.ev);
+  }
+  for (PinBuf& b : readback_buf)
+    if (b.p) (void)hipHostFree(b.p);
+  for (const ProfBracket& pb : prof_open) {
+    (void)hipEventDestroy(pb.a);
+    (void)hipEventDestroy(pb.b);
+  }
+  if (prof_cur) (void)hipEventDestroy(prof_cur);
+  for (hipEvent_t e : prof_pool) (void)hipEventDestroy(e);
   if (stream) (void)hipStreamDestroy(stream);
+}
+
+static hipEvent_t pooled_event(std::vector<hipEvent_t>& pool) {
+  if (!pool.empty()) {
+    hipEvent_t e = pool.back();
+    pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  check_hip(hipEventCreate(&e), "hipEventCreate");
+  return e;
 }
 
 void kt_engine::prof_begin() {
   if (!profiling) return;
-  if (!ev0) check_hip(hipEventCreate(&ev0), "hipEventCreate");
-  if (!ev1) check_hip(hipEventCreate(&ev1), "hipEventCreate");
-  check_hip(hipEventRecord(ev0, stream), "hipEventRecord");
+  if (!prof_cur) prof_cur = pooled_event(prof_pool);
+  check_hip(hipEventRecord(prof_cur, stream), "hipEventRecord");
 }
 
 void kt_engine::prof_end(double path_steps) {
-  if (!profiling) return;
-  check_hip(hipEventRecord(ev1, stream), "hipEventRecord");
-  check_hip(hipEventSynchronize(ev1), "hipEventSynchronize");
-  float ms = 0.0f;
-  check_hip(hipEventElapsedTime(&ms, ev0, ev1), "hipEventElapsedTime");
-  prof_ms += ms;
-  prof_path_steps += path_steps;
-  prof_launches += 1;
+  if (!profiling || !prof_cur) return;
+  hipEvent_t b = pooled_event(prof_pool);
+  check_hip(hipEventRecord(b, stream), "hipEventRecord");
+  prof_open.push_back(ProfBracket{prof_cur, b, path_steps});
+  prof_cur = nullptr;
+}
+
+// Waits for every recorded bracket and adds it to the totals.
+void kt_engine::prof_resolve() {
+  for (const ProfBracket& pb : prof_open) {
+    check_hip(hipEventSynchronize(pb.b), "hipEventSynchronize");
+    float ms = 0.0f;
+    check_hip(hipEventElapsedTime(&ms, pb.a, pb.b), "hipEventElapsedTime");
+    prof_ms += ms;
+    prof_path_steps += pb.path_steps;
+    prof_launches += 1;
+    prof_pool.push_back(pb.a);
+    prof_pool.push_back(pb.b);
+  }
+  prof_open.clear();
 }
 
 void* kt_engine::ensure(DevBuf& b, size_t bytes) {
@@ -271,22 +301,26 @@ static char* grow_pinned(PinBuf& b, size_t bytes) {
   return b.p;
 }
 
-char* kt_engine::stage(size_t bytes) {
-  if (stage_busy) {
-    check_hip(hipEventSynchronize(stage_ev), "hipEventSynchronize");
-    stage_busy = false;
+char* kt_engine::stage(size_t bytes, int slot) {
+  Staging& st = staging[slot];
+  if (st.busy) {
+    check_hip(hipEventSynchronize(st.ev), "hipEventSynchronize");
+    st.busy = false;
   }
-  return grow_pinned(stage_buf, std::max<size_t>(bytes, 256));
+  return grow_pinned(st.buf, std::max<size_t>(bytes, 256));
 }
 
-void kt_engine::stage_copy(void* dev, size_t bytes) {
-  if (!stage_ev) check_hip(hipEventCreateWithFlags(&stage_ev, hipEventDisableTiming), "hipEventCreate");
-  check_hip(hipMemcpyAsync(dev, stage_buf.p, bytes, hipMemcpyHostToDevice, stream), "upload");
-  check_hip(hipEventRecord(stage_ev, stream), "hipEventRecord");
-  stage_busy = true;
+void kt_engine::stage_copy(void* dev, size_t bytes, int slot, size_t offset) {
+  Staging& st = staging[slot];
+  if (!st.ev) check_hip(hipEventCreateWithFlags(&st.ev, hipEventDisableTiming), "hipEventCreate");
+  check_hip(hipMemcpyAsync((char*)dev + offset, st.buf.p + offset, bytes, hipMemcpyHostToDevice, stream), "upload");
+  check_hip(hipEventRecord(st.ev, stream), "hipEventRecord");
+  st.busy = true;
 }
 
-char* kt_engine::readback(size_t bytes) { return grow_pinned(readback_buf, std::max<size_t>(bytes, 256)); }
+char* kt_engine::readback(size_t bytes, int slot) {
+  return grow_pinned(readback_buf[slot], std::max<size_t>(bytes, 256));
+}
 
 // Upload a set of host blobs into the arena; returns device pointers.
 std::vector<void*> kt_engine::upload(const std::vector<std::pair<const void*, size_t>>& blobs) {
@@ -917,6 +951,10 @@ kt_status kt_engine_set_profiling(kt_engine* E, int32_t enabled) {
 kt_status kt_engine_kernel_time(kt_engine* E, double* total_ms, uint64_t* n_launches, double* path_steps,
                                 int32_t reset) {
   if (!E) return (kt_status)set_error(nullptr, KT_ERR_INVALID_ARGUMENT, "null engine");
+  KT_GUARD_BEGIN
+  check_hip(hipSetDevice(E->device), "hipSetDevice");
+  E->prof_resolve();
+  KT_GUARD_END(E)
   if (total_ms) *total_ms = E->prof_ms;
   if (n_launches) *n_launches = E->prof_launches;
   if (path_steps) *path_steps = E->prof_path_steps;

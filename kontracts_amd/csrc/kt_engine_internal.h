@@ -26,11 +26,22 @@ struct PinBuf {
   size_t cap = 0;
 };
 
+// A page-locked staging buffer and the event that marks the last copy out
+// of it (waited on before the buffer is rewritten).
+struct Staging {
+  PinBuf buf;
+  hipEvent_t ev = nullptr;
+  bool busy = false;
+};
+
 struct kt_engine {
   int device = 0;
   hipStream_t stream = nullptr;
   std::string err;
   DevBuf arena, partials, stats, errw, pv, z, scratch, paths, aux, book;
+  // A second set for a price-only book launched while the Greeks launches of
+  // the same call still run (slot 1 of launch_price_book).
+  DevBuf book2, partials2, stats2, errw2;
   // persistent portfolio launch: tile choice and the captured launch graph
   int pf_blocks_per_cu = 0;  // 0: the kernel's occupancy limit
   int pf_chunk = 1;
@@ -50,12 +61,10 @@ struct kt_engine {
   hipGraphExec_t lsm_exec = nullptr;
   std::string lsm_key;
   // Host -> device staging (tables, arguments) and device -> host readback
-  // (statistics, error words), both page-locked.  stage_ev marks the last
-  // copy out of the staging buffer; it is waited on before the buffer is
-  // rewritten.
-  PinBuf stage_buf, readback_buf;
-  hipEvent_t stage_ev = nullptr;
-  bool stage_busy = false;
+  // (statistics, error words), both page-locked, in two slots: slot 1 serves
+  // a price-only book prepared while slot 0's launches run.
+  Staging staging[2];
+  PinBuf readback_buf[2];
   // Error word of the per-contract path launches: the finalize kernel copies
   // it next to the statistics and clears it, so it is known zero between
   // calls unless a call was interrupted.
@@ -66,9 +75,18 @@ struct kt_engine {
   int occupancy_adjoint[2][kt::APV_COUNT] = {};  // Greeks kernels at the default budget / 5 waves
   bool pathwise_tangent = false;  // Heston pathwise Greeks: forward tangents instead of the reverse sweep
   bool profiling = false;
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  // Profiling brackets: event pairs recorded on the stream and resolved when
+  // the totals are read (no host wait inside a call).
+  struct ProfBracket {
+    hipEvent_t a, b;
+    double path_steps;
+  };
+  std::vector<hipEvent_t> prof_pool;
+  std::vector<ProfBracket> prof_open;
+  hipEvent_t prof_cur = nullptr;
   double prof_ms = 0.0, prof_path_steps = 0.0;
   uint64_t prof_launches = 0;
+  void prof_resolve();
   int adj_seg = kt::kAdjSeg;  // Heston adjoint checkpoint interval (kt_engine_set_option)
   // Lowered programs of recent (contract JSON, grid, asset names): a book
   // priced again (new market data, same contracts) skips the lowering.
@@ -80,11 +98,12 @@ struct kt_engine {
   void* ensure(DevBuf& b, size_t bytes);
   std::vector<void*> upload(const std::vector<std::pair<const void*, size_t>>& blobs);
   // Staging buffer of at least `bytes`, free to write (an earlier copy out of
-  // it has completed); stage_copy sends its first `bytes` to dev.
-  char* stage(size_t bytes);
-  void stage_copy(void* dev, size_t bytes);
+  // it has completed); stage_copy sends bytes [offset, offset + bytes) of it
+  // to the same range of dev.
+  char* stage(size_t bytes, int slot = 0);
+  void stage_copy(void* dev, size_t bytes, int slot = 0, size_t offset = 0);
   // Readback buffer of at least `bytes` (valid after the stream syncs).
-  char* readback(size_t bytes);
+  char* readback(size_t bytes, int slot = 0);
 };
 
 namespace kt {
@@ -202,6 +221,21 @@ void price_american(kt_engine* E, const char* json, const double* dates, uint64_
 // job_index (nullable): the caller's number of each job, for messages.
 void price_portfolio(kt_engine* E, const kt_job* jobs, uint64_t n_jobs, kt_price_result* out,
                      const uint64_t* job_index = nullptr);
+// A price-only book in two halves: launch_price_book prepares the jobs,
+// uploads through staging slot `slot` into that slot's device buffers and
+// enqueues the persistent launch and the statistics' readback without
+// waiting; finish_price_book waits for the stream and writes out[] (or
+// raises the first failing job's error, as a loop of price_gbm would).  The
+// Greeks book launches its price-only jobs in slot 1 before it prepares its
+// own launches in slot 0.
+struct PriceBookRun;
+struct PriceBookRunDeleter {
+  void operator()(PriceBookRun* r) const;
+};
+using PriceBookHandle = std::unique_ptr<PriceBookRun, PriceBookRunDeleter>;
+PriceBookHandle launch_price_book(kt_engine* E, const kt_job* jobs, uint64_t n_jobs, kt_price_result* out,
+                                  const uint64_t* job_index, int slot);
+void finish_price_book(kt_engine* E, PriceBookRun& run);
 // Host tables of one pathwise-Greeks job (the per-contract launch and the
 // portfolio share them): blobs in the order fill_adjoint_args expects.
 struct AdjJobHost {

@@ -55,17 +55,41 @@ int variant_of(const ModelDesc& md, const Program& prog) {
 
 }  // namespace
 
-void price_portfolio(kt_engine* E, const kt_job* jobs, uint64_t n_jobs, kt_price_result* out,
-                     const uint64_t* job_index) {
+// A launched price-only book awaiting its stream: the plans, which jobs run
+// in the persistent launch and which alone, and where the statistics land.
+struct PriceBookRun {
+  const kt_job* jobs = nullptr;
+  uint64_t n_jobs = 0;
+  kt_price_result* out = nullptr;
+  const uint64_t* job_index = nullptr;
+  int slot = 0;
+  std::vector<JobPlan> plans;
+  std::vector<uint64_t> solo;  // jobs priced by the per-contract path (after the stream syncs)
+  std::vector<uint64_t> book;  // jobs in the persistent launch
+  char* rb = nullptr;          // page-locked readback: statistics, then error words
+};
+
+void PriceBookRunDeleter::operator()(PriceBookRun* r) const { delete r; }
+
+PriceBookHandle launch_price_book(kt_engine* E, const kt_job* jobs, uint64_t n_jobs, kt_price_result* out,
+                                  const uint64_t* job_index, int slot) {
+  PriceBookHandle run(new PriceBookRun());
+  PriceBookRun& R = *run;
+  R.jobs = jobs;
+  R.n_jobs = n_jobs;
+  R.out = out;
+  R.job_index = job_index;
+  R.slot = slot;
   // job numbers in messages: the caller's numbering when it passes a subset
   auto jid = [&](uint64_t j) { return std::to_string(job_index ? job_index[j] : j); };
-  if (n_jobs == 0) return;
+  if (n_jobs == 0) return run;
   if (!jobs || !out) raise(KT_ERR_INVALID_ARGUMENT, "null jobs or output");
   if (n_jobs > (1ull << 24)) raise(KT_ERR_INVALID_ARGUMENT, "too many jobs");
   // Front half of every job, in order: the first failing job raises.
-  std::vector<JobPlan> plans(n_jobs);
-  std::vector<uint64_t> solo;  // jobs priced by the per-contract path
-  std::vector<uint64_t> book;  // jobs in the persistent launch
+  std::vector<JobPlan>& plans = R.plans;
+  plans.resize(n_jobs);
+  std::vector<uint64_t>& solo = R.solo;
+  std::vector<uint64_t>& book = R.book;
   for (uint64_t j = 0; j < n_jobs; ++j) {
     const kt_job& J = jobs[j];
     if (!J.contract_json) raise(KT_ERR_INVALID_ARGUMENT, "null contract in job " + jid(j));
@@ -100,185 +124,200 @@ void price_portfolio(kt_engine* E, const kt_job* jobs, uint64_t n_jobs, kt_price
     P.cost = per_point * (double)P.grid.size() + 4.0 * (double)P.prog.ops.size();
     book.push_back(j);
   }
+  if (book.empty()) return run;
 
+  // Work items, longest job first (then by job index), blocks in order.
+  std::vector<uint64_t> order = book;
+  std::stable_sort(order.begin(), order.end(),
+                   [&](uint64_t a, uint64_t b) { return plans[a].cost > plans[b].cost; });
+  std::vector<uint32_t> pos(n_jobs, 0);  // job -> index in the launch
+  for (size_t i = 0; i < book.size(); ++i) pos[book[i]] = (uint32_t)i;
+  uint64_t n_items = 0, n_partials = 0;
+  for (uint64_t j : book) n_items += plans[j].n_blocks;
+  if (n_items > 0xffffffffull) raise(KT_ERR_INVALID_ARGUMENT, "book too large");
+  std::vector<int> nblk(book.size());
+  std::vector<uint64_t> part_off(book.size());
+  for (size_t i = 0; i < book.size(); ++i) {
+    nblk[i] = (int)plans[book[i]].n_blocks;
+    part_off[i] = n_partials;
+    n_partials += plans[book[i]].n_blocks;
+  }
+  // Arena layout: [jobs][items][nblk][tables of each job].
+  const size_t jobs_bytes = align256(book.size() * sizeof(PortfolioJob));
+  const size_t items_bytes = align256(2 * n_items * sizeof(uint32_t));
+  const size_t nblk_bytes = align256(nblk.size() * sizeof(int));
+  size_t cursor = jobs_bytes + items_bytes + nblk_bytes;
+  for (size_t i = 0; i < book.size(); ++i) {
+    JobPlan& P = plans[book[i]];
+    const size_t sizes[7] = {std::max<size_t>(P.prog.ops.size(), 1) * sizeof(KOp), P.prog.steps.size() * sizeof(KStep),
+                             P.groups.size() * sizeof(KGroup),   P.prog.init.size() * sizeof(KOp),
+                             P.df.size() * sizeof(double),       P.tab.size() * sizeof(float),
+                             P.thr.size() * sizeof(float)};
+    for (int t = 0; t < 7; ++t) {
+      P.off[t] = cursor;
+      cursor += align256(std::max<size_t>(sizes[t], 1));
+    }
+  }
+  DevBuf& book_buf = slot ? E->book2 : E->book;
+  char* dev = (char*)E->ensure(book_buf, cursor);
+  KStats* partials = (KStats*)E->ensure(slot ? E->partials2 : E->partials, n_partials * sizeof(KStats));
+  // [statistics of each job][error word of each job]: one readback
+  const size_t rb_bytes = book.size() * sizeof(KStats) + align256(book.size() * sizeof(unsigned));
+  KStats* stats = (KStats*)E->ensure(slot ? E->stats2 : E->stats, std::max<size_t>(rb_bytes, kAdjMaxOut * sizeof(KStats)));
+  unsigned* err_copy = (unsigned*)(stats + book.size());
+  unsigned* errw = (unsigned*)E->ensure(slot ? E->errw2 : E->errw, align256((book.size() + 64) * sizeof(unsigned)));
+  unsigned* counter = errw + book.size();
+
+  // Everything the kernels read is written below; the alignment padding
+  // between blobs is never read.
+  char* H = E->stage(cursor, slot);
+  {
+    uint32_t* it = (uint32_t*)(H + jobs_bytes);
+    for (uint64_t j : order)
+      for (uint64_t b = 0; b < plans[j].n_blocks; ++b) {
+        *it++ = pos[j];
+        *it++ = (uint32_t)b;
+      }
+  }
+  static const KOp kNop{OP_NOP, 0, 0, 0, 0.0f};
+  for (size_t i = 0; i < book.size(); ++i) {
+    JobPlan& P = plans[book[i]];
+    const kt_job& J = jobs[book[i]];
+    PortfolioJob pj{};
+    PathArgs& A = pj.a;
+    A.ops = (const KOp*)(dev + P.off[0]);
+    A.steps = (const KStep*)(dev + P.off[1]);
+    A.groups = (const KGroup*)(dev + P.off[2]);
+    A.init = (const KOp*)(dev + P.off[3]);
+    A.df = (const double*)(dev + P.off[4]);
+    A.tab = (const float*)(dev + P.off[5]);
+    A.thr = (const float*)(dev + P.off[6]);
+    A.n_init = (int)P.prog.init.size();
+    A.n_grid = (int)P.grid.size();
+    A.n_groups = (int)P.groups.size();
+    A.n_uniform = uniform_group_prefix(P.groups.data(), A.n_groups);
+    A.tab_uniform = uniform_table_rows(P.tab, P.grid.size(), (size_t)P.stride);
+    A.tab_stride = P.stride;
+    A.n_paths = J.cfg.n_paths;
+    A.path_base = 0;
+    fill_model_args(A, P.md, J.cfg.seed);
+    A.df_index[0] = 0;
+    A.partials = partials + part_off[i];
+    A.err = errw + i;
+    pj.variant = P.variant;
+    std::memcpy(H + i * sizeof(PortfolioJob), &pj, sizeof(PortfolioJob));
+    const Blob blobs[7] = {{P.prog.ops.empty() ? &kNop : P.prog.ops.data(),
+                            std::max<size_t>(P.prog.ops.size(), 1) * sizeof(KOp)},
+                           {P.prog.steps.data(), P.prog.steps.size() * sizeof(KStep)},
+                           {P.groups.data(), P.groups.size() * sizeof(KGroup)},
+                           {P.prog.init.data(), P.prog.init.size() * sizeof(KOp)},
+                           {P.df.data(), P.df.size() * sizeof(double)},
+                           {P.tab.data(), P.tab.size() * sizeof(float)},
+                           {P.thr.data(), P.thr.size() * sizeof(float)}};
+    for (int t = 0; t < 7; ++t)
+      if (blobs[t].bytes) std::memcpy(H + P.off[t], blobs[t].p, blobs[t].bytes);
+  }
+  std::memcpy(H + jobs_bytes + items_bytes, nblk.data(), nblk.size() * sizeof(int));
+  E->stage_copy(dev, cursor, slot);
+
+  PortfolioArgs Q{};
+  Q.jobs = (const PortfolioJob*)dev;
+  Q.items = (const uint32_t*)(dev + jobs_bytes);
+  Q.n_items = (uint32_t)n_items;
+  Q.counter = counter;
+  const int* nblk_dev = (const int*)(dev + jobs_bytes + items_bytes);
+  int n_cu = 0;
+  check_hip(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, E->device), "device attribute");
+  int& occ = E->occupancy_portfolio[E->pf_waves == 6 ? 1 : E->pf_waves == 8 ? 2 : 0];
+  if (occ <= 0) occ = portfolio_occupancy_blocks_per_cu(E->pf_waves);
+  const int bpc = E->pf_blocks_per_cu > 0 ? E->pf_blocks_per_cu : occ;
+  const uint64_t slots = (uint64_t)n_cu * bpc;
+  // A queue chunk of c items saves atomics only where items outnumber the
+  // resident workgroups c times over; on a smaller book it would serialise
+  // items on a few workgroups.  Which workgroup takes an item never changes
+  // a result.
+  Q.chunk = (int)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)std::max(1, E->pf_chunk), n_items / slots));
+  const int grid = (int)std::max<uint64_t>(1, std::min<uint64_t>(slots, (n_items + Q.chunk - 1) / Q.chunk));
+
+  // Replay the captured launch sequence while the book keeps its shape
+  // (slot 0 only; slot 1 launches directly).
+  const std::vector<uint64_t> key = {(uint64_t)dev, (uint64_t)partials, (uint64_t)stats, (uint64_t)errw,
+                                     n_items, book.size(), (uint64_t)grid, (uint64_t)Q.chunk,
+                                     (uint64_t)E->pf_waves};
+  const bool graph = E->pf_use_graph && slot == 0;
+  E->prof_begin();
+  if (!graph) {
+    check_hip(launch_portfolio(Q, nblk_dev, (int)book.size(), grid, errw, stats, err_copy, E->stream, E->pf_waves),
+              "portfolio launch");
+  } else if (!E->pf_exec || E->pf_key != key) {
+    if (E->pf_exec) (void)hipGraphExecDestroy(E->pf_exec);
+    if (E->pf_graph) (void)hipGraphDestroy(E->pf_graph);
+    E->pf_exec = nullptr;
+    E->pf_graph = nullptr;
+    E->pf_key.clear();
+    check_hip(hipStreamBeginCapture(E->stream, hipStreamCaptureModeThreadLocal), "begin capture");
+    hipError_t le =
+        launch_portfolio(Q, nblk_dev, (int)book.size(), grid, errw, stats, err_copy, E->stream, E->pf_waves);
+    hipGraph_t g = nullptr;
+    hipError_t ce = hipStreamEndCapture(E->stream, &g);
+    check_hip(le, "portfolio launch");
+    check_hip(ce, "end capture");
+    E->pf_graph = g;
+    check_hip(hipGraphInstantiate(&E->pf_exec, g, nullptr, nullptr, 0), "graph instantiate");
+    E->pf_key = key;
+  }
+  if (graph) check_hip(hipGraphLaunch(E->pf_exec, E->stream), "graph launch");
+  double path_steps = 0.0;
+  for (uint64_t j : book) path_steps += (double)jobs[j].cfg.n_paths * (double)(plans[j].grid.size() - 1);
+  E->prof_end(path_steps);
+
+  R.rb = E->readback(rb_bytes, slot);
+  check_hip(hipMemcpyAsync(R.rb, stats, rb_bytes, hipMemcpyDeviceToHost, E->stream), "copy stats");
+  return run;
+}
+
+void finish_price_book(kt_engine* E, PriceBookRun& R) {
+  if (R.n_jobs == 0) return;
+  auto jid = [&](uint64_t j) { return std::to_string(R.job_index ? R.job_index[j] : j); };
   auto price_solo = [&](uint64_t j) {
-    const JobPlan& P = plans[j];
-    const kt_mc_config& c = jobs[j].cfg;
+    const JobPlan& P = R.plans[j];
+    const kt_mc_config& c = R.jobs[j].cfg;
     if (P.con)
       return price_vr(E, *P.con, P.md, P.grid, c.rate, c.n_paths, c.seed, c.antithetic != 0, c.control_variate != 0);
     return result_from_stats(run_program(E, P.prog, P.md, P.grid, c.rate, c.n_paths, c.seed, nullptr, nullptr)[0]);
   };
-
-  if (!book.empty()) {
-    // Work items, longest job first (then by job index), blocks in order.
-    std::vector<uint64_t> order = book;
-    std::stable_sort(order.begin(), order.end(),
-                     [&](uint64_t a, uint64_t b) { return plans[a].cost > plans[b].cost; });
-    std::vector<uint32_t> pos(n_jobs, 0);  // job -> index in the launch
-    for (size_t i = 0; i < book.size(); ++i) pos[book[i]] = (uint32_t)i;
-    uint64_t n_items = 0, n_partials = 0;
-    for (uint64_t j : book) n_items += plans[j].n_blocks;
-    if (n_items > 0xffffffffull) raise(KT_ERR_INVALID_ARGUMENT, "book too large");
-    std::vector<uint32_t> items;
-    items.reserve(2 * n_items);
-    for (uint64_t j : order)
-      for (uint64_t b = 0; b < plans[j].n_blocks; ++b) {
-        items.push_back(pos[j]);
-        items.push_back((uint32_t)b);
-      }
-    std::vector<int> nblk(book.size());
-    std::vector<uint64_t> part_off(book.size());
-    for (size_t i = 0; i < book.size(); ++i) {
-      nblk[i] = (int)plans[book[i]].n_blocks;
-      part_off[i] = n_partials;
-      n_partials += plans[book[i]].n_blocks;
-    }
-    // Arena layout: [jobs][items][nblk][tables of each job].
-    const size_t jobs_bytes = align256(book.size() * sizeof(PortfolioJob));
-    const size_t items_bytes = align256(items.size() * sizeof(uint32_t));
-    const size_t nblk_bytes = align256(nblk.size() * sizeof(int));
-    size_t cursor = jobs_bytes + items_bytes + nblk_bytes;
-    std::vector<std::vector<KOp>> ops_store(book.size());
-    for (size_t i = 0; i < book.size(); ++i) {
-      JobPlan& P = plans[book[i]];
-      ops_store[i] = P.prog.ops;
-      if (ops_store[i].empty()) ops_store[i].push_back(KOp{OP_NOP, 0, 0, 0, 0.0f});
-      const size_t sizes[7] = {ops_store[i].size() * sizeof(KOp), P.prog.steps.size() * sizeof(KStep),
-                               P.groups.size() * sizeof(KGroup), P.prog.init.size() * sizeof(KOp),
-                               P.df.size() * sizeof(double),     P.tab.size() * sizeof(float),
-                               P.thr.size() * sizeof(float)};
-      for (int t = 0; t < 7; ++t) {
-        P.off[t] = cursor;
-        cursor += align256(std::max<size_t>(sizes[t], 1));
-      }
-    }
-    char* dev = (char*)E->ensure(E->book, cursor);
-    KStats* partials = (KStats*)E->ensure(E->partials, n_partials * sizeof(KStats));
-    // [statistics of each job][error word of each job]: one readback
-    const size_t rb_bytes = book.size() * sizeof(KStats) + align256(book.size() * sizeof(unsigned));
-    KStats* stats = (KStats*)E->ensure(E->stats, std::max<size_t>(rb_bytes, kAdjMaxOut * sizeof(KStats)));
-    unsigned* err_copy = (unsigned*)(stats + book.size());
-    unsigned* errw = (unsigned*)E->ensure(E->errw, align256((book.size() + 64) * sizeof(unsigned)));
-    unsigned* counter = errw + book.size();
-
-    char* H = E->stage(cursor);
-    std::memset(H, 0, cursor);
-    std::vector<PortfolioJob> pj(book.size());
-    for (size_t i = 0; i < book.size(); ++i) {
-      JobPlan& P = plans[book[i]];
-      const kt_job& J = jobs[book[i]];
-      PathArgs& A = pj[i].a;
-      A = PathArgs{};
-      A.ops = (const KOp*)(dev + P.off[0]);
-      A.steps = (const KStep*)(dev + P.off[1]);
-      A.groups = (const KGroup*)(dev + P.off[2]);
-      A.init = (const KOp*)(dev + P.off[3]);
-      A.df = (const double*)(dev + P.off[4]);
-      A.tab = (const float*)(dev + P.off[5]);
-      A.thr = (const float*)(dev + P.off[6]);
-      A.n_init = (int)P.prog.init.size();
-      A.n_grid = (int)P.grid.size();
-      A.n_groups = (int)P.groups.size();
-      A.n_uniform = uniform_group_prefix(P.groups.data(), A.n_groups);
-      A.tab_uniform = uniform_table_rows(P.tab, P.grid.size(), (size_t)P.stride);
-      A.tab_stride = P.stride;
-      A.n_paths = J.cfg.n_paths;
-      A.path_base = 0;
-      fill_model_args(A, P.md, J.cfg.seed);
-      A.df_index[0] = 0;
-      A.partials = partials + part_off[i];
-      A.err = errw + i;
-      pj[i].variant = P.variant;
-      const Blob blobs[7] = {{ops_store[i].data(), ops_store[i].size() * sizeof(KOp)},
-                             {P.prog.steps.data(), P.prog.steps.size() * sizeof(KStep)},
-                             {P.groups.data(), P.groups.size() * sizeof(KGroup)},
-                             {P.prog.init.data(), P.prog.init.size() * sizeof(KOp)},
-                             {P.df.data(), P.df.size() * sizeof(double)},
-                             {P.tab.data(), P.tab.size() * sizeof(float)},
-                             {P.thr.data(), P.thr.size() * sizeof(float)}};
-      for (int t = 0; t < 7; ++t)
-        if (blobs[t].bytes) std::memcpy(H + P.off[t], blobs[t].p, blobs[t].bytes);
-    }
-    std::memcpy(H, pj.data(), pj.size() * sizeof(PortfolioJob));
-    std::memcpy(H + jobs_bytes, items.data(), items.size() * sizeof(uint32_t));
-    std::memcpy(H + jobs_bytes + items_bytes, nblk.data(), nblk.size() * sizeof(int));
-    E->stage_copy(dev, cursor);
-
-    PortfolioArgs Q{};
-    Q.jobs = (const PortfolioJob*)dev;
-    Q.items = (const uint32_t*)(dev + jobs_bytes);
-    Q.n_items = (uint32_t)n_items;
-    Q.counter = counter;
-    const int* nblk_dev = (const int*)(dev + jobs_bytes + items_bytes);
-    int n_cu = 0;
-    check_hip(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, E->device), "device attribute");
-    int& occ = E->occupancy_portfolio[E->pf_waves == 6 ? 1 : E->pf_waves == 8 ? 2 : 0];
-    if (occ <= 0) occ = portfolio_occupancy_blocks_per_cu(E->pf_waves);
-    const int bpc = E->pf_blocks_per_cu > 0 ? E->pf_blocks_per_cu : occ;
-    const uint64_t slots = (uint64_t)n_cu * bpc;
-    // A queue chunk of c items saves atomics only where items outnumber the
-    // resident workgroups c times over; on a smaller book it would serialise
-    // items on a few workgroups.  Which workgroup takes an item never changes
-    // a result.
-    Q.chunk = (int)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)std::max(1, E->pf_chunk), n_items / slots));
-    const int grid = (int)std::max<uint64_t>(1, std::min<uint64_t>(slots, (n_items + Q.chunk - 1) / Q.chunk));
-
-    // Replay the captured launch sequence while the book keeps its shape.
-    const std::vector<uint64_t> key = {(uint64_t)dev, (uint64_t)partials, (uint64_t)stats, (uint64_t)errw,
-                                       n_items, book.size(), (uint64_t)grid, (uint64_t)Q.chunk,
-                                       (uint64_t)E->pf_waves};
-    E->prof_begin();
-    if (!E->pf_use_graph) {
-      check_hip(launch_portfolio(Q, nblk_dev, (int)book.size(), grid, errw, stats, err_copy, E->stream, E->pf_waves),
-                "portfolio launch");
-    } else if (!E->pf_exec || E->pf_key != key) {
-      if (E->pf_exec) (void)hipGraphExecDestroy(E->pf_exec);
-      if (E->pf_graph) (void)hipGraphDestroy(E->pf_graph);
-      E->pf_exec = nullptr;
-      E->pf_graph = nullptr;
-      E->pf_key.clear();
-      check_hip(hipStreamBeginCapture(E->stream, hipStreamCaptureModeThreadLocal), "begin capture");
-      hipError_t le =
-          launch_portfolio(Q, nblk_dev, (int)book.size(), grid, errw, stats, err_copy, E->stream, E->pf_waves);
-      hipGraph_t g = nullptr;
-      hipError_t ce = hipStreamEndCapture(E->stream, &g);
-      check_hip(le, "portfolio launch");
-      check_hip(ce, "end capture");
-      E->pf_graph = g;
-      check_hip(hipGraphInstantiate(&E->pf_exec, g, nullptr, nullptr, 0), "graph instantiate");
-      E->pf_key = key;
-    }
-    if (E->pf_use_graph) check_hip(hipGraphLaunch(E->pf_exec, E->stream), "graph launch");
-    double path_steps = 0.0;
-    for (uint64_t j : book) path_steps += (double)jobs[j].cfg.n_paths * (double)(plans[j].grid.size() - 1);
-    E->prof_end(path_steps);
-
-    char* rb = E->readback(rb_bytes);
-    check_hip(hipMemcpyAsync(rb, stats, rb_bytes, hipMemcpyDeviceToHost, E->stream), "copy stats");
-    check_hip(hipStreamSynchronize(E->stream), "stream sync");
-    std::vector<KStats> st(book.size());
-    std::vector<unsigned> err_bits(book.size());
-    std::memcpy(st.data(), rb, book.size() * sizeof(KStats));
-    std::memcpy(err_bits.data(), rb + book.size() * sizeof(KStats), book.size() * sizeof(unsigned));
-    // A path error fails the call like the reference loop would at that job,
-    // unless an earlier job (solo or in the book) fails first.
-    uint64_t first_err = n_jobs;
-    int first_code = 0;
-    for (size_t i = 0; i < book.size(); ++i)
-      if (err_bits[i] && book[i] < first_err) {
-        first_err = book[i];
-        first_code = lowest_error_code(err_bits[i]);
-      }
-    for (uint64_t j : solo) {
-      if (j > first_err) break;
-      out[j] = price_solo(j);
-    }
-    if (first_code)
-      raise(first_code, error_text(first_code, plans[first_err].prog) + std::string(" (job ") + jid(first_err) + ")");
-    for (size_t i = 0; i < book.size(); ++i) out[book[i]] = result_from_stats(st[i]);
+  if (R.book.empty()) {
+    for (uint64_t j : R.solo) R.out[j] = price_solo(j);
     return;
   }
-  for (uint64_t j : solo) out[j] = price_solo(j);
+  check_hip(hipStreamSynchronize(E->stream), "stream sync");
+  const size_t nb = R.book.size();
+  std::vector<KStats> st(nb);
+  std::vector<unsigned> err_bits(nb);
+  std::memcpy(st.data(), R.rb, nb * sizeof(KStats));
+  std::memcpy(err_bits.data(), R.rb + nb * sizeof(KStats), nb * sizeof(unsigned));
+  // A path error fails the call like the reference loop would at that job,
+  // unless an earlier job (solo or in the book) fails first.
+  uint64_t first_err = R.n_jobs;
+  int first_code = 0;
+  for (size_t i = 0; i < nb; ++i)
+    if (err_bits[i] && R.book[i] < first_err) {
+      first_err = R.book[i];
+      first_code = lowest_error_code(err_bits[i]);
+    }
+  for (uint64_t j : R.solo) {
+    if (j > first_err) break;
+    R.out[j] = price_solo(j);
+  }
+  if (first_code)
+    raise(first_code, error_text(first_code, R.plans[first_err].prog) + std::string(" (job ") + jid(first_err) + ")");
+  for (size_t i = 0; i < nb; ++i) R.out[R.book[i]] = result_from_stats(st[i]);
+}
+
+void price_portfolio(kt_engine* E, const kt_job* jobs, uint64_t n_jobs, kt_price_result* out,
+                     const uint64_t* job_index) {
+  PriceBookHandle run = launch_price_book(E, jobs, n_jobs, out, job_index, 0);
+  finish_price_book(E, *run);
 }
 
 }  // namespace kt

@@ -27,6 +27,8 @@ struct AdjJob {
   ModelDesc md;
   std::vector<double> grid;
   Program prog;
+  AdjPlan plan;
+  double rate = 0.0;
   AdjJobHost H;
   int variant = -1;
   int need = 0;
@@ -108,7 +110,8 @@ void price_portfolio_greeks(kt_engine* E, const kt_job* jobs, uint64_t n_jobs, k
     a.md = std::move(md);
     a.grid = pc.grid;
     a.prog = std::move(prog);
-    a.H = adjoint_job_host(a.prog, a.md, a.grid, J.cfg.rate, std::move(plan));
+    a.plan = std::move(plan);
+    a.rate = J.cfg.rate;
     a.variant = variant;
     a.need = need;
     a.n_blocks = (J.cfg.n_paths + kBlock - 1) / kBlock;
@@ -117,6 +120,19 @@ void price_portfolio_greeks(kt_engine* E, const kt_job* jobs, uint64_t n_jobs, k
     a.cost = per_point * (double)a.grid.size() + 4.0 * (double)a.prog.ops.size();
     adj.push_back(std::move(a));
   }
+
+  // The price-only jobs go first, in the second staging slot and buffer
+  // set: their persistent launch runs while the Greeks launches below are
+  // prepared on the host.
+  std::vector<kt_job> po_jobs;
+  std::vector<kt_price_result> po_out(price_only.size());
+  PriceBookHandle po_run;
+  if (!price_only.empty()) {
+    po_jobs.reserve(price_only.size());
+    for (uint64_t j : price_only) po_jobs.push_back(jobs[j]);
+    po_run = launch_price_book(E, po_jobs.data(), po_jobs.size(), po_out.data(), price_only.data(), 1);
+  }
+  for (AdjJob& a : adj) a.H = adjoint_job_host(a.prog, a.md, a.grid, a.rate, std::move(a.plan));
 
   if (!adj.empty()) {
     const size_t n_adj = adj.size();
@@ -184,8 +200,8 @@ void price_portfolio_greeks(kt_engine* E, const kt_job* jobs, uint64_t n_jobs, k
       ck = (float*)E->ensure(E->aux,
                              (size_t)std::max(grid_v[APV_HESTON], grid_v[APV_HESTON_FEEDS]) * ck_floats * sizeof(float));
 
+    // every byte the kernels read is written below (padding is never read)
     char* H = E->stage(cursor);
-    std::memset(H, 0, cursor);
     std::vector<int> nblk(n_adj), nout(n_adj);
     for (size_t i = 0; i < n_adj; ++i) {
       AdjJob& a = adj[i];
@@ -244,13 +260,9 @@ void price_portfolio_greeks(kt_engine* E, const kt_job* jobs, uint64_t n_jobs, k
 
   // Jobs without Greeks: the persistent price-only launch (errors numbered in
   // the caller's book).
-  if (!price_only.empty()) {
-    std::vector<kt_job> sub;
-    sub.reserve(price_only.size());
-    for (uint64_t j : price_only) sub.push_back(jobs[j]);
-    std::vector<kt_price_result> r(sub.size());
-    price_portfolio(E, sub.data(), sub.size(), r.data(), price_only.data());
-    for (size_t i = 0; i < price_only.size(); ++i) out[price_only[i]] = r[i];
+  if (po_run) {
+    finish_price_book(E, *po_run);
+    for (size_t i = 0; i < price_only.size(); ++i) out[price_only[i]] = po_out[i];
   }
   // Greeks jobs no portfolio variant covers: one launch each.
   for (uint64_t j : solo) {
