@@ -121,7 +121,10 @@ def bench_book(args, eng, K, B, R, T, threads):
     # algorithmic lane-ops of one step: each job's paths x steps x its kernel's count
     lane_ops = sum(job[2] * (len(job[5]) - 1) * R.book_job(kind, model, n, nf, greeks=True)
                    for job, (kind, model, n, nf) in zip(jobs, profiles))
-    tuned = T.autotune_portfolio(jobs, repeats=1, greeks=True)
+    if args.no_tune:
+        tuned = {"best": (0, 1, 0, 0), "device_ms": {}}
+    else:
+        tuned = T.autotune_portfolio(jobs, repeats=1, greeks=True)
     log(f"config4: autotuned tile {tuned['best']} ({', '.join(f'{k}: {v:.1f} ms' for k, v in tuned['device_ms'].items())})")
 
     def step():
@@ -134,20 +137,6 @@ def bench_book(args, eng, K, B, R, T, threads):
     dev_per_step = dev_ms / args.steps
     log(f"config4: {1e3 * wall / args.steps:.1f} ms/step ({dev_per_step:.1f} ms device), {value:.4e} path-steps/s")
     n_greeks = sum(1 for _, g in res if g is not None)
-
-    # the reference's way: one call per contract (price_gbm loop), pathwise
-    # Greeks per contract where the payoff admits them
-    def per_contract():
-        for i, job in enumerate(jobs):
-            kw = dict(n_paths=job[2], seed=job[3], steps_per_year=job[4], grid=job[5])
-            if profiles[i][0] == "barrier":
-                job[0].price(job[1], **kw)
-            else:
-                job[0].pathwise_greeks(job[1], **kw)
-
-    _, wall_pc, dev_pc, _, _ = timed_steps(eng, per_contract, 1)
-    # price only, the same book
-    _, wall_po, dev_po, _, _ = timed_steps(eng, lambda: K.price_portfolio(jobs), 2)
     lane_ops_po = sum(job[2] * (len(job[5]) - 1) * R.book_job(kind, model, n, nf, greeks=False)
                       for job, (kind, model, n, nf) in zip(jobs, profiles))
     roofline = roof(lane_ops, dev_per_step,
@@ -169,6 +158,22 @@ def bench_book(args, eng, K, B, R, T, threads):
         "tile": {"best": list(tuned["best"]), "device_ms": tuned["device_ms"]},
         "parallelism": "single GPU",
     }
+    if args.no_compare:
+        return value, wall / args.steps, info, roofline, {}, None
+
+    # the reference's way: one call per contract (price_gbm loop), pathwise
+    # Greeks per contract where the payoff admits them
+    def per_contract():
+        for i, job in enumerate(jobs):
+            kw = dict(n_paths=job[2], seed=job[3], steps_per_year=job[4], grid=job[5])
+            if profiles[i][0] == "barrier":
+                job[0].price(job[1], **kw)
+            else:
+                job[0].pathwise_greeks(job[1], **kw)
+
+    _, wall_pc, dev_pc, _, _ = timed_steps(eng, per_contract, 1)
+    # price only, the same book
+    _, wall_po, dev_po, _, _ = timed_steps(eng, lambda: K.price_portfolio(jobs), 2)
     extra = {
         "per_contract_s": wall_pc, "per_contract_device_ms": dev_pc,
         "speedup_vs_per_contract": wall_pc / (wall / args.steps),
@@ -371,10 +376,21 @@ def bench_config1(eng, K, R):
     m1 = K.GBM(100.0, 0.2, 0.05, asset="X")
     r1, t1, k1 = median_call(eng, lambda: c1.price(m1, n_paths=1_000_000, grid=grid))
     ps = 1_000_000 * 252.0
+    # the book's 5-asset basket call priced alone (pairwise rho 0.3)
+    fs = [K.GbmFactor(f"A{i}", 100.0, 0.05, 0.15 + 0.05 * i) for i in range(5)]
+    b5 = K.correlated_gbm(fs, [[1.0 if i == j else 0.3 for j in range(5)] for i in range(5)], 0.05)
+    avg5 = (K.spot("A0") + K.spot("A1") + K.spot("A2") + K.spot("A3") + K.spot("A4")) / 5.0
+    cb = K.scale((avg5 - 100.0).clip(0.0), K.one("USD")) @ K.at(1.0)
+    rb, tb, kb = median_call(eng, lambda: cb.price(b5, n_paths=1 << 20, grid=grid))
+    psb = (1 << 20) * 252.0
     return {"workload": "European call GBM s0=K=100 sigma=0.2 r=0.05 T=1, 1,000,000 paths x 252 steps (explicit "
                         "grid), seed 42", "price": r1.price, "std_error": r1.std_error,
             "black_scholes": 10.450583572185565, "ms": t1 * 1e3, "kernel_ms": k1, "path_steps_per_s": ps / t1,
-            "roofline": roof(ps * R.gbm_european(), k1, "path_kernel<GBM>")}
+            "roofline": roof(ps * R.gbm_european(), k1, "path_kernel<GBM>"),
+            "basket5": {"workload": "5-asset basket call (vols 0.15-0.35, pairwise rho 0.3), 1,048,576 paths x 252 "
+                                    "steps", "price": rb.price, "std_error": rb.std_error, "ms": tb * 1e3,
+                        "kernel_ms": kb, "path_steps_per_s": psb / tb,
+                        "roofline": roof(psb * R.basket_european(5), kb, "path_kernel<basket K=5>")}}
 
 
 def bench_config5(eng, K):
@@ -406,6 +422,9 @@ def main():
     ap.add_argument("--cpu-paths", type=int, default=262_144, help="config-2 CPU sample paths")
     ap.add_argument("--cpu-paths-book", type=int, default=4096, help="paths per contract of the book's CPU sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-compare", action="store_true",
+                    help="book: skip the per-contract and launch-bound comparisons (profiling runs)")
+    ap.add_argument("--no-tune", action="store_true", help="book: default tile instead of the autotuner (profiling runs)")
     ap.add_argument("--only", choices=["book", "config2", "config3", "config1", "config5"], default=None,
                     help="run one workload only (profiling runs)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
