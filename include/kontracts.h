@@ -276,8 +276,9 @@ kt_status kt_engine_kernel_time(kt_engine* engine, double* total_ms, uint64_t* n
  *   "portfolio_waves": register budget of the persistent price kernel: 0
  *                      (default: the compiler's), 6 or 8 waves per SIMD.
  *   "portfolio_adjoint_waves": register budget of the persistent Greeks
- *                      kernels: 0 (default: 128 VGPRs for Heston and baskets,
- *                      the compiler's otherwise) or 5 waves per SIMD.
+ *                      kernels: 0 (default: 102 VGPRs for baskets and Heston
+ *                      with running accumulators, 128 for Heston, the
+ *                      compiler's for GBM) or 5 waves per SIMD for all.
  *   "lsm_graph":       1 (default) replays kt_price_american's launch chain
  *                      (paths kernel, backward passes, solves) from a
  *                      captured hipGraph while its launch arguments are

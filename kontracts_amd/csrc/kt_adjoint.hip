@@ -199,12 +199,12 @@ __device__ __forceinline__ void adjoint_gbm_block(const AA& A, uint32_t blk) {
   double pv[1][1] = {{0.0}};
   // Groups of four points without ops (G_NONE: nothing evaluated, nothing
   // fed) only advance the log-spots and the Brownian sums, with the
-  // per-point path's expressions on the same values.  The leading run of
-  // them (P.n_uniform) runs, under GBM on a grid of equal steps, in a loop
-  // of its own that takes the step coefficients and sqrt(dt) from VGPR
-  // copies of point 1's rows (FP32 instructions with SGPR operands issue at
-  // about half rate); the copies live in that loop only.  Other op-free
-  // groups read the table.
+  // per-point path's expressions on the same values.  Under GBM every run of
+  // them is a loop of its own (the register file and the accumulators'
+  // state stay where they are, no copies per group); on a grid of equal
+  // steps it takes the step coefficients and sqrt(dt) from VGPR copies of
+  // point 1's rows (FP32 instructions with SGPR operands issue at about half
+  // rate), otherwise it reads the table.
   auto quiet_group = [&](int g, auto&& coef) {
     float z[4 * M];
     group_normals<M>(g, p_lo, p_hi, P.seed_lo, P.seed_hi, 0u, z);
@@ -235,39 +235,53 @@ __device__ __forceinline__ void adjoint_gbm_block(const AA& A, uint32_t blk) {
     }
   };
   // the last group may be partial: it always takes the per-point path
-  const int n_lead = min(P.n_uniform, P.n_groups - 1);
-  int g_first = 0;
+  auto op_free = [&](int g) {
+    return g + 1 < P.n_groups && ld_u32(P.groups, (size_t)g * 4) == (uint32_t)G_NONE;
+  };
+  int g = 0;
   // (GBM only: with the basket's 26 copies the kernel spills; its price
   // kernel gained 2% from them)
-  if (K == 1 && n_lead > 1 && P.tab_uniform) {
-    // group 0 holds point 0's zero row: it reads the table
-    quiet_group(0, table_coef);
-    float usq = vgpr_copy(ld_f32(A.adj_tab, 2)), ua[K], ub[K], uch[K * (K + 1) / 2];
+  const bool urow = K == 1 && P.tab_uniform;
+  float usq = 0.0f, ua[K], ub[K], uch[K * (K + 1) / 2];
+  if (urow) {
+    usq = vgpr_copy(ld_f32(A.adj_tab, 2));
 #pragma unroll
     for (int a = 0; a < K; ++a) {
       ua[a] = vgpr_copy(ld_f32(P.tab, ((size_t)K + a) * 2));
       ub[a] = vgpr_copy(ld_f32(P.tab, ((size_t)K + a) * 2 + 1));
+      // (one asset: the factor is the literal 1, which the compiler folds away)
 #pragma unroll
-      for (int c = 0; c <= a; ++c) uch[a * (a + 1) / 2 + c] = vgpr_copy(K == 1 ? 1.0f : P.chol[a * K + c]);
+      for (int c = 0; c <= a; ++c) uch[a * (a + 1) / 2 + c] = K == 1 ? 1.0f : vgpr_copy(P.chol[a * K + c]);
     }
-    auto row_coef = [&](int, float& sq, float (&ta)[K], float (&tb)[K], float (&ch)[K * (K + 1) / 2]) {
-      sq = usq;
-#pragma unroll
-      for (int a = 0; a < K; ++a) {
-        ta[a] = ua[a];
-        tb[a] = ub[a];
-      }
-#pragma unroll
-      for (int i = 0; i < K * (K + 1) / 2; ++i) ch[i] = uch[i];
-    };
-    for (int g = 1; g < n_lead; ++g) quiet_group(g, row_coef);
-    g_first = n_lead;
   }
-  for (int g = g_first; g < P.n_groups; ++g) {
-    if (K == 1 && g + 1 < P.n_groups && ld_u32(P.groups, (size_t)g * 4) == (uint32_t)G_NONE) {
-      quiet_group(g, table_coef);
-      continue;
+  auto row_coef = [&](int, float& sq, float (&ta)[K], float (&tb)[K], float (&ch)[K * (K + 1) / 2]) {
+    sq = usq;
+#pragma unroll
+    for (int a = 0; a < K; ++a) {
+      ta[a] = ua[a];
+      tb[a] = ub[a];
     }
+#pragma unroll
+    for (int i = 0; i < K * (K + 1) / 2; ++i) ch[i] = uch[i];
+  };
+  // (a loop of their own needs a second instance of the basket's Cholesky
+  // step, which costs it occupancy: its op-free groups take the per-point
+  // path and skip the ops)
+  constexpr bool kRuns = K == 1;
+  // group 0 holds point 0's zero row: it reads the table
+  if (kRuns && op_free(0)) {
+    quiet_group(0, table_coef);
+    g = 1;
+  }
+  for (;; ++g) {
+    if (kRuns) {
+      if (urow)
+        for (; op_free(g); ++g) quiet_group(g, row_coef);
+      else
+        for (; op_free(g); ++g) quiet_group(g, table_coef);
+    }
+    if (g >= P.n_groups) break;
+    const bool quiet = !kRuns && op_free(g);
     float z[4 * M];
 #pragma unroll
     for (int bk = 0; bk < M; ++bk) {
@@ -292,6 +306,7 @@ __device__ __forceinline__ void adjoint_gbm_block(const AA& A, uint32_t blk) {
           // advance's exact log step on the same w (computed once)
           x[0][a] = fmaf(ld_f32(P.tab, ((size_t)k * K + a) * 2 + 1), w, x[0][a] + ld_f32(P.tab, ((size_t)k * K + a) * 2));
         }
+        if (quiet) continue;
         const uint32_t fm = ld_u32(A.feed, (size_t)k);
         int before[kAdjMaxAcc];
 #pragma unroll
@@ -708,16 +723,18 @@ __device__ __forceinline__ void adjoint_heston_block(const AA& A, uint32_t blk, 
         const int p = SEG * c + i;
         if (p <= n) {
           if constexpr (FEEDS) {
-            float seed = p == n ? ds_n : 0.0f;
+            // spot seeds only where the payoff reads the spot: the flow date
+            // and the fed points (a uniform branch)
             const uint32_t fm = ld_u32(A.feed, (size_t)p);
-            if (fm) {
+            if (fm != 0u || p == n) {
+              float seed = p == n ? ds_n : 0.0f;
 #pragma unroll
               for (int q = 0; q < kAdjMaxAcc; ++q)
                 if ((fm >> q) & 1u) seed += (A.acc_kind[q] == OP_AVG || argstep[q] == p) ? abar[q] : 0.0f;
+              const float g = exp2_fast(xs[0][0]), S = s0 * g;  // S / s0 = 2^x
+              xb = fmaf(seed * S, kLn2f, xb);
+              g_s0 = fmaf(seed, g, g_s0);
             }
-            const float g = exp2_fast(xs[0][0]), S = s0 * g;  // S / s0 = 2^x
-            xb = fmaf(seed * S, kLn2f, xb);
-            g_s0 = fmaf(seed, g, g_s0);
           }
           if (p >= 1) {
             // one 32-byte row per point: a single scalar load
@@ -816,17 +833,18 @@ __device__ __forceinline__ void portfolio_adjoint_body(const AdjPortfolioArgs& Q
   }
 }
 
-// The register budget per variant: the Heston and basket variants need a
-// few registers more than 128, which would cost a quarter of their occupancy
-// (3 waves per SIMD instead of 4), so they are capped at 128 (4 waves); the
-// others take the compiler's choice.  WAVES = 5 caps every variant at 102
-// VGPRs (a tile the autotuner may pick).
+// The register budget per variant: the Heston variants need a few registers
+// more than 128, which would cost a quarter of their occupancy (3 waves per
+// SIMD instead of 4), so they are capped at 128 (4 waves); the basket and the
+// Heston variant with running accumulators run faster still at 102 (5 waves,
+// spilling a few values around their loops: measured 1.6% and 6% on the
+// 1,000-contract book); GBM takes the compiler's choice.  WAVES = 5 caps
+// every variant at 102 VGPRs (a tile the autotuner may pick).
 template <int VARIANT>
 constexpr int adjoint_default_waves() {
-  return (VARIANT == APV_BASKET5 || VARIANT == APV_HESTON_FEEDS || VARIANT == APV_HESTON ||
-          VARIANT == APV_HESTON_FEEDS_T)
-             ? 4
-             : 1;
+  return (VARIANT == APV_BASKET5 || VARIANT == APV_HESTON_FEEDS)   ? 5
+         : (VARIANT == APV_HESTON || VARIANT == APV_HESTON_FEEDS_T) ? 4
+                                                                     : 1;
 }
 
 template <int VARIANT, int WAVES>
