@@ -90,6 +90,20 @@ def median_call(eng, fn, reps=5):
     return out, statistics.median(walls), statistics.median(devs)
 
 
+def interleaved_calls(eng, fns, reps=5):
+    """Median wall and device time of each function over reps rounds that
+    call every function once, after one warm call of each."""
+    outs = [fn() for fn in fns]
+    walls = [[] for _ in fns]
+    devs = [[] for _ in fns]
+    for _ in range(reps):
+        for i, fn in enumerate(fns):
+            outs[i], w, d, _, _ = timed_steps(eng, fn, 1)
+            walls[i].append(w)
+            devs[i].append(d)
+    return [(outs[i], statistics.median(walls[i]), statistics.median(devs[i])) for i in range(len(fns))]
+
+
 def roof(lane_ops_per_launch, avg_launch_ms, kernel, extra=None):
     from kontracts_amd import roofline as R
     achieved = lane_ops_per_launch / (avg_launch_ms / 1e3) / 1e12 if avg_launch_ms > 0 else 0.0
@@ -333,17 +347,26 @@ def bench_config3(eng, K, R):
     hm = K.heston(100, 0.04, 2.0, 0.04, 0.3, -0.5, 0.05, asset="X")
     n = 2_097_152
     hb = [1.0, 1e-3, 1e-2, 1e-3, 1e-2, 1e-2]
-    rf, tf, kf = median_call(eng, lambda: c.price(hm, n_paths=n, steps_per_year=365))
-    gA, tA, kA = median_call(eng, lambda: c.pathwise_greeks(hm, n_paths=n, steps_per_year=365))
-    _, tB, kB = median_call(eng, lambda: c.bump_greeks(hm, bumps=hb, n_paths=n, steps_per_year=365,
-                                                       std_errors=False))
+
+    def tangent():
+        eng.set_option("pathwise_mode", 1)
+        try:
+            return c.pathwise_greeks(hm, n_paths=n, steps_per_year=365)
+        finally:
+            eng.set_option("pathwise_mode", 0)
+
+    # the engine clock settles after some tens of launches: warm it up, then
+    # time the four calls in interleaved rounds so that each sees the same
+    # clock conditions
+    for _ in range(40):
+        c.price(hm, n_paths=n, steps_per_year=365)
+    (rf, tf, kf), (gA, tA, kA), (_, tB, kB), (gT, tT, kT) = interleaved_calls(eng, [
+        lambda: c.price(hm, n_paths=n, steps_per_year=365),
+        lambda: c.pathwise_greeks(hm, n_paths=n, steps_per_year=365),
+        lambda: c.bump_greeks(hm, bumps=hb, n_paths=n, steps_per_year=365, std_errors=False),
+        tangent])
     gB = c.bump_greeks(hm, bumps=hb, n_paths=n, steps_per_year=365)
     eq = {nm: (tB / tA) * (sb / sa) ** 2 for nm, sa, sb in zip(gA.names, gA.std_errors, gB.std_errors)}
-    eng.set_option("pathwise_mode", 1)
-    try:
-        gT, tT, kT = median_call(eng, lambda: c.pathwise_greeks(hm, n_paths=n, steps_per_year=365))
-    finally:
-        eng.set_option("pathwise_mode", 0)
     eqT = {nm: (tB / tT) * (sb / sa) ** 2 for nm, sa, sb in zip(gT.names, gT.std_errors, gB.std_errors)}
     ps = float(n) * 365
     log(f"config3: pathwise {tA * 1e3:.2f} ms vs 13 bumps {tB * 1e3:.2f} ms: {tB / tA:.2f}x raw, "
@@ -352,7 +375,8 @@ def bench_config3(eng, K, R):
         "workload": "Heston call s0=K=100 v0=theta=0.04 kappa=2 sigma_v=0.3 rho=-0.5 r=0.05 T=1, 2,097,152 paths x "
                     "365 steps; d price / d {s0, v0, kappa, theta, sigma_v, rho}: pathwise reverse sweep with a "
                     "checkpoint every 16 steps vs 13 CRN bump pricings",
-        "timing": "median of 5 calls after a warm one",
+        "timing": "after 40 warm-up pricings, median of 5 interleaved rounds of (price, pathwise, 13 bumps, "
+                  "tangent)",
         "forward_ms": tf * 1e3, "forward_kernel_ms": kf, "forward_path_steps_per_s": ps / tf,
         "forward_roofline": roof(ps * R.heston_european(), kf, "path_kernel<Heston>"),
         "pathwise_ms": tA * 1e3, "pathwise_kernel_ms": kA,

@@ -447,20 +447,30 @@ __device__ __forceinline__ void tangent_heston_block(const AA& A, uint32_t blk) 
   for (int q = 0; q < (FEEDS ? kAdjMaxAcc : 1); ++q)
 #pragma unroll
     for (int j = 0; j <= NT; ++j) W[q][j] = 0.0f;
-  // one Euler step at point k >= 1 with its tangents
-  auto step = [&](int k, const float* zk) {
+  // Step coefficients of a grid of equal steps (rows 1 .. n shared): VGPR
+  // copies for the leading op-free groups (FP32 instructions with SGPR
+  // operands issue at about half rate).  Filled only when P.tab_uniform.
+  StepRow<MK_HESTON, 1, 1> urow;
+  float u_rev[7], u_theta = 0.0f, u_rr = 0.0f;
+#pragma unroll
+  for (int q = 0; q < 7; ++q) u_rev[q] = 0.0f;
+  // one Euler step at point k >= 1 with its tangents; U: coefficients from
+  // the VGPR copies (the same values, so the same results)
+  auto step = [&](int k, const float* zk, auto use_row) {
+    constexpr bool U = decltype(use_row)::value;
     const float z1 = zk[0], z2 = zk[1];
     if (k >= 1) {
-      const float t1 = ld_f32(A.rev, 8 * (size_t)k + 0);
-      const float t2 = ld_f32(A.rev, 8 * (size_t)k + 1);
-      const float t3 = ld_f32(A.rev, 8 * (size_t)k + 2);
-      const float t5 = ld_f32(A.rev, 8 * (size_t)k + 3);
-      const float dt = ld_f32(A.rev, 8 * (size_t)k + 4);
-      const float sq = ld_f32(A.rev, 8 * (size_t)k + 5);
-      const float kdt = ld_f32(A.rev, 8 * (size_t)k + 6);
+      const float t1 = U ? u_rev[0] : ld_f32(A.rev, 8 * (size_t)k + 0);
+      const float t2 = U ? u_rev[1] : ld_f32(A.rev, 8 * (size_t)k + 1);
+      const float t3 = U ? u_rev[2] : ld_f32(A.rev, 8 * (size_t)k + 2);
+      const float t5 = U ? u_rev[3] : ld_f32(A.rev, 8 * (size_t)k + 3);
+      const float dt = U ? u_rev[4] : ld_f32(A.rev, 8 * (size_t)k + 4);
+      const float sq = U ? u_rev[5] : ld_f32(A.rev, 8 * (size_t)k + 5);
+      const float kdt = U ? u_rev[6] : ld_f32(A.rev, 8 * (size_t)k + 6);
+      const float th = U ? u_theta : theta, rr = U ? u_rr : rho_ratio;
       const float vp = fmaxf(v, 0.0f);  // = v under the floor scheme
       const float sv = __builtin_amdgcn_sqrtf(vp);
-      const float w = fmaf(rho, z1, rho_perp * z2);
+      const float w = U ? fmaf(urow.rho, z1, urow.rho_perp * z2) : fmaf(rho, z1, rho_perp * z2);
       const bool pass = !ft || v > 0.0f;  // d vp / d v
       const float hs = sv > 0.0f ? 0.5f * __builtin_amdgcn_rsqf(vp) : 0.0f;  // d sqrt(vp) / d vp
       const float cx = t2 * z1, cv = t5 * w;
@@ -472,23 +482,49 @@ __device__ __forceinline__ void tangent_heston_block(const AA& A, uint32_t blk) 
         dx[j] = fmaf(cx, dsv, fmaf(t1, dvp, dx[j]));
         dvn[j] = fmaf(cv, dsv, fmaf(t3, dvp, dv[j]));
       }
-      dvn[1] = fmaf(dt, theta - vp, dvn[1]);                  // kappa
-      dvn[2] += kdt;                                          // theta
-      dvn[3] = fmaf(sq * sv, w, dvn[3]);                      // sigma_v
-      dvn[4] = fmaf(t5 * sv, fmaf(-rho_ratio, z2, z1), dvn[4]);  // rho
-      advance<MK_HESTON, 1, 1>(P, k, zk, x, v);
+      dvn[1] = fmaf(dt, th - vp, dvn[1]);                // kappa
+      dvn[2] += kdt;                                     // theta
+      dvn[3] = fmaf(sq * sv, w, dvn[3]);                 // sigma_v
+      dvn[4] = fmaf(t5 * sv, fmaf(-rr, z2, z1), dvn[4]);  // rho
+      if constexpr (U) advance_row<MK_HESTON, 1, 1>(urow, zk, x, v);
+      else advance<MK_HESTON, 1, 1>(P, k, zk, x, v);
       const bool keep = ft || v > 0.0f;  // the floor cuts the derivative
 #pragma unroll
       for (int j = 0; j < NT; ++j) dv[j] = keep ? dvn[j] : 0.0f;
     }
   };
-  // leading groups whose points run no ops and feed nothing (P.n_uniform)
+  // leading groups whose points run no ops and feed nothing (P.n_uniform);
+  // on a grid of equal steps groups 1 .. n_quiet - 1 take the VGPR rows
+  // (group 0 holds point 0's zero row)
   const int n_quiet = min(P.n_uniform, P.n_groups);
-  for (int g = 0; g < n_quiet; ++g) {
+  const std::integral_constant<bool, true> kRow;
+  const std::integral_constant<bool, false> kTable;
+  int g_row = 0;
+  if (P.tab_uniform && n_quiet > 1) {
+    {
+      float z[8];
+      group_normals<2>(0, p_lo, p_hi, P.seed_lo, P.seed_hi, 0u, z);
+#pragma unroll
+      for (int s = 0; s < 4; ++s) step(s, &z[2 * s], kTable);
+    }
+    urow = vgpr_row<MK_HESTON, 1, 1>(P, 1);
+#pragma unroll
+    for (int q = 0; q < 7; ++q) u_rev[q] = vgpr_copy(ld_f32(A.rev, 8 + q));
+    u_theta = vgpr_copy(theta);
+    u_rr = vgpr_copy(rho_ratio);
+    for (int g = 1; g < n_quiet; ++g) {
+      float z[8];
+      group_normals<2>(g, p_lo, p_hi, P.seed_lo, P.seed_hi, 0u, z);
+#pragma unroll
+      for (int s = 0; s < 4; ++s) step(4 * g + s, &z[2 * s], kRow);
+    }
+    g_row = n_quiet;
+  }
+  for (int g = g_row; g < n_quiet; ++g) {
     float z[8];
     group_normals<2>(g, p_lo, p_hi, P.seed_lo, P.seed_hi, 0u, z);
 #pragma unroll
-    for (int s = 0; s < 4; ++s) step(4 * g + s, &z[2 * s]);
+    for (int s = 0; s < 4; ++s) step(4 * g + s, &z[2 * s], kTable);
   }
   for (int g = n_quiet; g < P.n_groups; ++g) {
     float z[8];
@@ -497,7 +533,7 @@ __device__ __forceinline__ void tangent_heston_block(const AA& A, uint32_t blk) 
     for (int s = 0; s < 4; ++s) {
       const int k = 4 * g + s;
       if (k <= n) {
-        step(k, &z[2 * s]);
+        step(k, &z[2 * s], kTable);
         if constexpr (FEEDS) {
           const uint32_t fm = ld_u32(A.feed, (size_t)k);
           int before[kAdjMaxAcc];
@@ -592,9 +628,35 @@ __device__ __forceinline__ void adjoint_heston_block(const AA& A, uint32_t blk, 
       ck[(2 * c + 1) * cs + lane] = v;
     }
   };
-  // leading groups without ops (P.n_uniform): advance and checkpoint only
+  // leading groups without ops (P.n_uniform): advance and checkpoint only.
+  // On a grid of equal steps groups 1 .. n_quiet - 1 take the step row from
+  // VGPR copies of row 1 (group 0 holds point 0's zero row and reads the
+  // table): the same expressions on the same values as advance().
   const int n_quiet = min(P.n_uniform, P.n_groups);
-  for (int g = 0; g < n_quiet; ++g) {
+  int g_row = 0;
+  if (P.tab_uniform && n_quiet > 1) {
+    {
+      float z[8];
+      group_normals<2>(0, p_lo, p_hi, P.seed_lo, P.seed_hi, 0u, z);
+#pragma unroll
+      for (int s = 0; s < 4; ++s) {
+        checkpoint(s);
+        advance<MK_HESTON, 1, 1>(P, s, &z[2 * s], x, v);
+      }
+    }
+    const StepRow<MK_HESTON, 1, 1> R = vgpr_row<MK_HESTON, 1, 1>(P, 1);
+    for (int g = 1; g < n_quiet; ++g) {
+      float z[8];
+      group_normals<2>(g, p_lo, p_hi, P.seed_lo, P.seed_hi, 0u, z);
+#pragma unroll
+      for (int s = 0; s < 4; ++s) {
+        checkpoint(4 * g + s);
+        advance_row<MK_HESTON, 1, 1>(R, &z[2 * s], x, v);
+      }
+    }
+    g_row = n_quiet;
+  }
+  for (int g = g_row; g < n_quiet; ++g) {
     float z[8];
     group_normals<2>(g, p_lo, p_hi, P.seed_lo, P.seed_hi, 0u, z);
 #pragma unroll
@@ -662,12 +724,25 @@ __device__ __forceinline__ void adjoint_heston_block(const AA& A, uint32_t blk, 
   // row, shared by points 1 .. n, from VGPR copies (FP32 instructions with
   // SGPR operands issue at about half rate: 5% on config 3), the other reads
   // them per point.  Both evaluate the same expressions on the same values.
+  //
+  // On the equal-step instance the per-point constants are folded out of the
+  // loop: xb t1 and xb t2 / 2 (xb is constant without feeds), t5 / 2 and
+  // dt theta are VGPR copies; the theta, sigma_v and rho sums carry
+  // sum vnb, sum vnb sqrt(v) w and sum vnb sqrt(v) (z1 - rho z2 / rho') and are
+  // scaled by kappa dt, sqrt(dt) sigma_v-row and t5 once at the end.
   auto sweep = [&](auto urow) {
     constexpr bool U = decltype(urow)::value && !FEEDS;
-    float u_rev[7], u_e0 = 0.0f, u_d1 = 0.0f, u_d2 = 0.0f, u_rho = 0.0f, u_rhop = 0.0f, u_lo = 0.0f;
+    float u_e0 = 0.0f, u_d1 = 0.0f, u_d2 = 0.0f, u_rho = 0.0f, u_rhop = 0.0f, u_lo = 0.0f;
+    float u_c1 = 0.0f, u_c2h = 0.0f, u_t3 = 0.0f, u_t5h = 0.0f, u_dt = 0.0f, u_dtth = 0.0f, u_rr = 0.0f;
+    float s_vnb = 0.0f, s_sv = 0.0f, s_rho = 0.0f;
     if constexpr (U) {
-#pragma unroll
-      for (int q = 0; q < 7; ++q) u_rev[q] = vgpr_copy(ld_f32(A.rev, 8 + q));
+      u_c1 = xb * ld_f32(A.rev, 8 + 0);
+      u_c2h = 0.5f * xb * ld_f32(A.rev, 8 + 1);
+      u_t3 = vgpr_copy(ld_f32(A.rev, 8 + 2));
+      u_t5h = vgpr_copy(0.5f * ld_f32(A.rev, 8 + 3));
+      u_dt = vgpr_copy(ld_f32(A.rev, 8 + 4));
+      u_dtth = vgpr_copy(ld_f32(A.rev, 8 + 4) * theta);
+      u_rr = vgpr_copy(rho_ratio);
       u_e0 = vgpr_copy(ld_f32(P.tab, kHestonStride + 3));
       u_d1 = vgpr_copy(ld_f32(P.tab, kHestonStride + 4));
       u_d2 = vgpr_copy(ld_f32(P.tab, kHestonStride + 5));
@@ -736,15 +811,32 @@ __device__ __forceinline__ void adjoint_heston_block(const AA& A, uint32_t blk, 
               g_s0 = fmaf(seed, g, g_s0);
             }
           }
-          if (p >= 1) {
+          if (p >= 1 && U) {
+            const float sv = svs[i], z1 = z1s[i], z2 = z2s[i];
+            const float vp = sv * sv;  // max(v, 0) to rounding
+            const float w = fmaf(u_rho, z1, u_rhop * z2);
+            const float vnb = (ft || sv_next > 0.0f) ? vbar : 0.0f;
+            const float a = vnb * sv;
+            s_vnb += vnb;
+            g_k = fmaf(vnb, fmaf(-u_dt, vp, u_dtth), g_k);
+            s_sv = fmaf(a, w, s_sv);
+            s_rho = fmaf(a, fmaf(-u_rr, z2, z1), s_rho);
+            // half of d/d sqrt(v), times 1 / sqrt(v) (hardware reciprocal):
+            // d sqrt(v) / dv = 0.5 / sqrt(v)
+            const float svbh = fmaf(u_c2h, z1, (vnb * u_t5h) * w);
+            const float rs = sv > 0.0f ? __builtin_amdgcn_rcpf(sv) : 0.0f;
+            const float vpb = fmaf(svbh, rs, fmaf(vnb, u_t3, u_c1));
+            vbar = vnb + ((!ft || sv > 0.0f) ? vpb : 0.0f);
+            sv_next = sv;
+          } else if (p >= 1) {
             // one 32-byte row per point: a single scalar load
-            const float t1 = U ? u_rev[0] : ld_f32(A.rev, 8 * (size_t)p + 0);
-            const float t2 = U ? u_rev[1] : ld_f32(A.rev, 8 * (size_t)p + 1);
-            const float t3 = U ? u_rev[2] : ld_f32(A.rev, 8 * (size_t)p + 2);
-            const float t5 = U ? u_rev[3] : ld_f32(A.rev, 8 * (size_t)p + 3);
-            const float dt = U ? u_rev[4] : ld_f32(A.rev, 8 * (size_t)p + 4);
-            const float sq = U ? u_rev[5] : ld_f32(A.rev, 8 * (size_t)p + 5);
-            const float kdt = U ? u_rev[6] : ld_f32(A.rev, 8 * (size_t)p + 6);
+            const float t1 = ld_f32(A.rev, 8 * (size_t)p + 0);
+            const float t2 = ld_f32(A.rev, 8 * (size_t)p + 1);
+            const float t3 = ld_f32(A.rev, 8 * (size_t)p + 2);
+            const float t5 = ld_f32(A.rev, 8 * (size_t)p + 3);
+            const float dt = ld_f32(A.rev, 8 * (size_t)p + 4);
+            const float sq = ld_f32(A.rev, 8 * (size_t)p + 5);
+            const float kdt = ld_f32(A.rev, 8 * (size_t)p + 6);
             const float sv = svs[i], z1 = z1s[i], z2 = z2s[i];
             const float vp = sv * sv;  // max(v, 0) to rounding
             const float w = fmaf(rho, z1, rho_perp * z2);
@@ -768,6 +860,12 @@ __device__ __forceinline__ void adjoint_heston_block(const AA& A, uint32_t blk, 
           }
         }
       }
+    }
+    if constexpr (U) {
+      // every row is row 1 (point 0 has no step)
+      g_th = ld_f32(A.rev, 8 + 6) * s_vnb;
+      g_sv = ld_f32(A.rev, 8 + 5) * s_sv;
+      g_rho = ld_f32(A.rev, 8 + 3) * s_rho;
     }
   };
   // (one instance only with running accumulators: U needs none)

@@ -152,13 +152,13 @@ constexpr bool uniform_loop_built() {
 // groups of one level share their thresholds, so the extreme runs over the
 // whole prefix and is compared once at its end: a point crosses the level
 // exactly when the prefix's extreme does, and the flag only changes from
-// its initial value to 1, so the result equals a compare per group.  Under
-// GBM and correlated GBM on a grid of equal steps (tab_uniform) groups
-// 1 .. n-1 take the step coefficients (and the Cholesky factor) from VGPR
-// copies of table row 1 (group 0 holds the zero row of point 0 and reads the
-// table): 10% on the seven-scenario kernel, 2% on the 5-asset basket.  The
-// Heston step measured 7% slower that way on a running average (more VGPRs)
-// and keeps reading the table.
+// its initial value to 1, so the result equals a compare per group.  On a
+// grid of equal steps (tab_uniform) groups 1 .. n-1 take the step
+// coefficients (and the Cholesky factor) from VGPR copies of table row 1
+// (group 0 holds the zero row of point 0 and reads the table): 10% on the
+// seven-scenario kernel, 2% on the 5-asset basket, 2-3% on the Heston
+// European.  Heston runs under a running sum or extreme keep reading the
+// table (7% slower with the copies: more VGPRs).
 template <int KIND, int MODEL, int K, int SC, bool DEBUG, class PA>
 __device__ __forceinline__ void uniform_groups(const PA& A, const KGroup gd, int n, uint32_t p_lo, uint32_t p_hi,
                                                uint32_t flip, float (&x)[SC][K], float& v, float (&acc)[SC],
@@ -167,7 +167,8 @@ __device__ __forceinline__ void uniform_groups(const PA& A, const KGroup gd, int
   float ext[SC];
   if constexpr (knock_out_kind<KIND>()) ko_init<KIND, SC>(ext);
   int g_table = 0;  // groups before g_table have run
-  if constexpr (MODEL == MK_GBM || MODEL == MK_BASKET) {
+  constexpr bool kRowKind = MODEL != MK_HESTON || !(KIND == G_AVG || KIND == G_RMAX || KIND == G_RMIN);
+  if constexpr (kRowKind) {
     if (A.tab_uniform) {
       {
         float z[4 * M];
