@@ -728,8 +728,8 @@ __device__ __forceinline__ void adjoint_heston_block(const AA& A, uint32_t blk, 
   // On the equal-step instance the per-point constants are folded out of the
   // loop: xb t1 and xb t2 / 2 (xb is constant without feeds), t5 / 2 and
   // dt theta are VGPR copies; the theta, sigma_v and rho sums carry
-  // sum vnb, sum vnb sqrt(v) w and sum vnb sqrt(v) (z1 - rho z2 / rho') and are
-  // scaled by kappa dt, sqrt(dt) sigma_v-row and t5 once at the end.
+  // sum vnb, sum vnb sqrt(v) w and sum vnb sqrt(v) (z1 - rho z2 / rho'), and
+  // are multiplied once at the end by the row's kappa dt, sqrt(dt) and t5.
   auto sweep = [&](auto urow) {
     constexpr bool U = decltype(urow)::value && !FEEDS;
     float u_e0 = 0.0f, u_d1 = 0.0f, u_d2 = 0.0f, u_rho = 0.0f, u_rhop = 0.0f, u_lo = 0.0f;
