@@ -560,13 +560,31 @@ std::vector<KStats> run_program(kt_engine* E, const Program& prog, const ModelDe
 }
 
 // Shared front half of price_gbm: parse, plan, grid, validation.
-PreparedContract prepare(const char* json, const ModelDesc& md, const kt_mc_config* cfg, bool allow_vr) {
+PreparedContract prepare(const char* json, const ModelDesc& md, const kt_mc_config* cfg, bool allow_vr,
+                         kt_engine* E) {
   if (!cfg) raise(KT_ERR_INVALID_ARGUMENT, "null config");
   if (!allow_vr && (cfg->antithetic || cfg->control_variate))
     raise(KT_ERR_UNSUPPORTED, "variance reduction is not available for this entry point");
   PreparedContract pc;
-  pc.con = parse_contract(json);
-  Plan plan = compile_plan(*pc.con);
+  Plan plan_local;
+  const Plan* plan_p = &plan_local;
+  if (E) {
+    // a text that fails to parse raises here and is never cached
+    auto it = E->parse_cache.find(json);
+    if (it == E->parse_cache.end()) {
+      kt_engine::Parsed entry;
+      entry.con = parse_contract(json);
+      entry.plan = compile_plan(*entry.con);
+      if (E->parse_cache.size() >= 16384) E->parse_cache.clear();
+      it = E->parse_cache.emplace(std::string(json), std::move(entry)).first;
+    }
+    pc.con = it->second.con;
+    plan_p = &it->second.plan;
+  } else {
+    pc.con = parse_contract(json);
+    plan_local = compile_plan(*pc.con);
+  }
+  const Plan& plan = *plan_p;
   if (cfg->grid && cfg->n_grid) {
     pc.grid.assign(cfg->grid, cfg->grid + cfg->n_grid);
   } else if (cfg->grid == nullptr && cfg->n_grid == 0) {

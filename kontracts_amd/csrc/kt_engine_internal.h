@@ -91,6 +91,12 @@ struct kt_engine {
   // Lowered programs of recent (contract JSON, grid, asset names): a book
   // priced again (new market data, same contracts) skips the lowering.
   std::unordered_map<std::string, std::shared_ptr<const kt::Program>> lower_cache;
+  // Parsed contracts and their plans of recent contract JSON texts.
+  struct Parsed {
+    std::shared_ptr<const kt::Con> con;
+    kt::Plan plan;
+  };
+  std::unordered_map<std::string, Parsed> parse_cache;
   // Bracket a group of launches with events when profiling is enabled.
   void prof_begin();
   void prof_end(double path_steps);
@@ -118,7 +124,7 @@ struct ModelDesc {
 };
 
 struct PreparedContract {
-  std::unique_ptr<Con> con;
+  std::shared_ptr<const Con> con;
   std::vector<double> grid;
 };
 
@@ -156,7 +162,9 @@ std::shared_ptr<const Program> lower_cached(kt_engine* E, const char* json, cons
 
 // allow_vr: the caller handles cfg's variance-reduction flags; otherwise
 // they are rejected.
-PreparedContract prepare(const char* json, const ModelDesc& md, const kt_mc_config* cfg, bool allow_vr = false);
+// E (optional): parse through the engine's cache of contract texts.
+PreparedContract prepare(const char* json, const ModelDesc& md, const kt_mc_config* cfg, bool allow_vr = false,
+                         kt_engine* E = nullptr);
 // pv_keep (device, n_paths doubles, nullable): receives each path's pv and
 // stays on the device.
 std::vector<KStats> run_program(kt_engine* E, const Program& prog, const ModelDesc& md,

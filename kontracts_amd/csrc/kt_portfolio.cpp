@@ -28,10 +28,10 @@ struct Blob {
 };
 
 struct JobPlan {
-  std::unique_ptr<Con> con;  // kept for variance-reduced jobs
+  std::shared_ptr<const Con> con;  // kept for variance-reduced jobs
   ModelDesc md;
   std::vector<double> grid;
-  Program prog;
+  std::shared_ptr<const Program> prog;  // the engine's memoised lowering (shared, not copied)
   std::vector<KGroup> groups;
   std::vector<float> tab, thr;
   std::vector<double> df;
@@ -95,7 +95,7 @@ PriceBookHandle launch_price_book(kt_engine* E, const kt_job* jobs, uint64_t n_j
     if (!J.contract_json) raise(KT_ERR_INVALID_ARGUMENT, "null contract in job " + jid(j));
     JobPlan& P = plans[j];
     P.md = model_from_abi(&J.model);
-    PreparedContract pc = prepare(J.contract_json, P.md, &J.cfg, true);
+    PreparedContract pc = prepare(J.contract_json, P.md, &J.cfg, true, E);
     P.grid = pc.grid;
     out[j] = kt_price_result{};
     if (J.cfg.antithetic || J.cfg.control_variate) {
@@ -104,8 +104,8 @@ PriceBookHandle launch_price_book(kt_engine* E, const kt_job* jobs, uint64_t n_j
       solo.push_back(j);
       continue;
     }
-    P.prog = *lower_cached(E, J.contract_json, *pc.con, P.grid, P.md.names);
-    P.variant = variant_of(P.md, P.prog);
+    P.prog = lower_cached(E, J.contract_json, *pc.con, P.grid, P.md.names);
+    P.variant = variant_of(P.md, *P.prog);
     if (J.cfg.n_paths == 0) {
       out[j] = result_from_stats(KStats{0.0, 0.0, 0.0});
       continue;
@@ -117,11 +117,11 @@ PriceBookHandle launch_price_book(kt_engine* E, const kt_job* jobs, uint64_t n_j
     P.tab = model_table(P.md, P.grid, &P.stride);
     P.df.resize(P.grid.size());
     for (size_t k = 0; k < P.grid.size(); ++k) P.df[k] = std::exp(-J.cfg.rate * P.grid[k]);
-    P.groups = knockout_tables(P.prog, {P.md}, &P.thr);
+    P.groups = knockout_tables(*P.prog, {P.md}, &P.thr);
     P.n_blocks = (J.cfg.n_paths + kBlock - 1) / kBlock;
     if (P.n_blocks > 0xffffffffull) raise(KT_ERR_INVALID_ARGUMENT, "too many paths in job " + jid(j));
     const double per_point = P.variant == PV_GBM ? 40.0 : P.variant == PV_HESTON ? 85.0 : 41.0 * P.md.names.size();
-    P.cost = per_point * (double)P.grid.size() + 4.0 * (double)P.prog.ops.size();
+    P.cost = per_point * (double)P.grid.size() + 4.0 * (double)P.prog->ops.size();
     book.push_back(j);
   }
   if (book.empty()) return run;
@@ -149,8 +149,8 @@ PriceBookHandle launch_price_book(kt_engine* E, const kt_job* jobs, uint64_t n_j
   size_t cursor = jobs_bytes + items_bytes + nblk_bytes;
   for (size_t i = 0; i < book.size(); ++i) {
     JobPlan& P = plans[book[i]];
-    const size_t sizes[7] = {std::max<size_t>(P.prog.ops.size(), 1) * sizeof(KOp), P.prog.steps.size() * sizeof(KStep),
-                             P.groups.size() * sizeof(KGroup),   P.prog.init.size() * sizeof(KOp),
+    const size_t sizes[7] = {std::max<size_t>(P.prog->ops.size(), 1) * sizeof(KOp), P.prog->steps.size() * sizeof(KStep),
+                             P.groups.size() * sizeof(KGroup),   P.prog->init.size() * sizeof(KOp),
                              P.df.size() * sizeof(double),       P.tab.size() * sizeof(float),
                              P.thr.size() * sizeof(float)};
     for (int t = 0; t < 7; ++t) {
@@ -192,7 +192,7 @@ PriceBookHandle launch_price_book(kt_engine* E, const kt_job* jobs, uint64_t n_j
     A.df = (const double*)(dev + P.off[4]);
     A.tab = (const float*)(dev + P.off[5]);
     A.thr = (const float*)(dev + P.off[6]);
-    A.n_init = (int)P.prog.init.size();
+    A.n_init = (int)P.prog->init.size();
     A.n_grid = (int)P.grid.size();
     A.n_groups = (int)P.groups.size();
     A.n_uniform = uniform_group_prefix(P.groups.data(), A.n_groups);
@@ -206,11 +206,11 @@ PriceBookHandle launch_price_book(kt_engine* E, const kt_job* jobs, uint64_t n_j
     A.err = errw + i;
     pj.variant = P.variant;
     std::memcpy(H + i * sizeof(PortfolioJob), &pj, sizeof(PortfolioJob));
-    const Blob blobs[7] = {{P.prog.ops.empty() ? &kNop : P.prog.ops.data(),
-                            std::max<size_t>(P.prog.ops.size(), 1) * sizeof(KOp)},
-                           {P.prog.steps.data(), P.prog.steps.size() * sizeof(KStep)},
+    const Blob blobs[7] = {{P.prog->ops.empty() ? &kNop : P.prog->ops.data(),
+                            std::max<size_t>(P.prog->ops.size(), 1) * sizeof(KOp)},
+                           {P.prog->steps.data(), P.prog->steps.size() * sizeof(KStep)},
                            {P.groups.data(), P.groups.size() * sizeof(KGroup)},
-                           {P.prog.init.data(), P.prog.init.size() * sizeof(KOp)},
+                           {P.prog->init.data(), P.prog->init.size() * sizeof(KOp)},
                            {P.df.data(), P.df.size() * sizeof(double)},
                            {P.tab.data(), P.tab.size() * sizeof(float)},
                            {P.thr.data(), P.thr.size() * sizeof(float)}};
@@ -284,7 +284,7 @@ void finish_price_book(kt_engine* E, PriceBookRun& R) {
     const kt_mc_config& c = R.jobs[j].cfg;
     if (P.con)
       return price_vr(E, *P.con, P.md, P.grid, c.rate, c.n_paths, c.seed, c.antithetic != 0, c.control_variate != 0);
-    return result_from_stats(run_program(E, P.prog, P.md, P.grid, c.rate, c.n_paths, c.seed, nullptr, nullptr)[0]);
+    return result_from_stats(run_program(E, *P.prog, P.md, P.grid, c.rate, c.n_paths, c.seed, nullptr, nullptr)[0]);
   };
   if (R.book.empty()) {
     for (uint64_t j : R.solo) R.out[j] = price_solo(j);
@@ -310,7 +310,7 @@ void finish_price_book(kt_engine* E, PriceBookRun& R) {
     R.out[j] = price_solo(j);
   }
   if (first_code)
-    raise(first_code, error_text(first_code, R.plans[first_err].prog) + std::string(" (job ") + jid(first_err) + ")");
+    raise(first_code, error_text(first_code, *R.plans[first_err].prog) + std::string(" (job ") + jid(first_err) + ")");
   for (size_t i = 0; i < nb; ++i) R.out[R.book[i]] = result_from_stats(st[i]);
 }
 

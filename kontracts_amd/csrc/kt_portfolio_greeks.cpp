@@ -26,7 +26,7 @@ struct AdjJob {
   uint64_t index = 0;  // position in the caller's book
   ModelDesc md;
   std::vector<double> grid;
-  Program prog;
+  std::shared_ptr<const Program> prog;  // the engine's memoised lowering (shared, not copied)
   AdjPlan plan;
   double rate = 0.0;
   AdjJobHost H;
@@ -72,8 +72,9 @@ void price_portfolio_greeks(kt_engine* E, const kt_job* jobs, uint64_t n_jobs, k
     if (!J.contract_json) raise(KT_ERR_INVALID_ARGUMENT, "null contract in job " + std::to_string(j));
     ModelDesc md = model_from_abi(&J.model);
     const bool vr = J.cfg.antithetic || J.cfg.control_variate;
-    PreparedContract pc = prepare(J.contract_json, md, &J.cfg, vr);
-    Program prog = *lower_cached(E, J.contract_json, *pc.con, pc.grid, md.names);
+    PreparedContract pc = prepare(J.contract_json, md, &J.cfg, vr, E);
+    std::shared_ptr<const Program> prog_ptr = lower_cached(E, J.contract_json, *pc.con, pc.grid, md.names);
+    const Program& prog = *prog_ptr;
     bool ok = !vr;
     for (const KOp& op : prog.ops) ok = ok && op.code != OP_FLAGERR;  // path errors: the pricer raises them
     AdjPlan plan;
@@ -109,7 +110,7 @@ void price_portfolio_greeks(kt_engine* E, const kt_job* jobs, uint64_t n_jobs, k
     a.index = j;
     a.md = std::move(md);
     a.grid = pc.grid;
-    a.prog = std::move(prog);
+    a.prog = std::move(prog_ptr);
     a.plan = std::move(plan);
     a.rate = J.cfg.rate;
     a.variant = variant;
@@ -117,7 +118,7 @@ void price_portfolio_greeks(kt_engine* E, const kt_job* jobs, uint64_t n_jobs, k
     a.n_blocks = (J.cfg.n_paths + kBlock - 1) / kBlock;
     if (a.n_blocks > 0xffffffffull) raise(KT_ERR_INVALID_ARGUMENT, "too many paths in job " + std::to_string(j));
     const double per_point = variant == APV_GBM ? 60.0 : variant == APV_BASKET5 ? 260.0 : 250.0;
-    a.cost = per_point * (double)a.grid.size() + 4.0 * (double)a.prog.ops.size();
+    a.cost = per_point * (double)a.grid.size() + 4.0 * (double)a.prog->ops.size();
     adj.push_back(std::move(a));
   }
 
@@ -132,7 +133,7 @@ void price_portfolio_greeks(kt_engine* E, const kt_job* jobs, uint64_t n_jobs, k
     for (uint64_t j : price_only) po_jobs.push_back(jobs[j]);
     po_run = launch_price_book(E, po_jobs.data(), po_jobs.size(), po_out.data(), price_only.data(), 1);
   }
-  for (AdjJob& a : adj) a.H = adjoint_job_host(a.prog, a.md, a.grid, a.rate, std::move(a.plan));
+  for (AdjJob& a : adj) a.H = adjoint_job_host(*a.prog, a.md, a.grid, a.rate, std::move(a.plan));
 
   if (!adj.empty()) {
     const size_t n_adj = adj.size();
@@ -167,7 +168,7 @@ void price_portfolio_greeks(kt_engine* E, const kt_job* jobs, uint64_t n_jobs, k
     cursor += a256(n_adj * sizeof(int));
     std::vector<std::vector<std::pair<const void*, size_t>>> blobs(n_adj);
     for (size_t i = 0; i < n_adj; ++i) {
-      blobs[i] = adjoint_blobs(adj[i].H, adj[i].prog);
+      blobs[i] = adjoint_blobs(adj[i].H, *adj[i].prog);
       for (int t = 0; t < 10; ++t) {
         adj[i].off[t] = cursor;
         cursor += a256(std::max<size_t>(blobs[i][t].second, 1));
@@ -212,7 +213,7 @@ void price_portfolio_greeks(kt_engine* E, const kt_job* jobs, uint64_t n_jobs, k
         if (blobs[i][t].second) std::memcpy(H + a.off[t], blobs[i][t].first, blobs[i][t].second);
       }
       AdjArgs A{};
-      fill_adjoint_args(A, a.H, a.prog, a.md, a.grid, J.cfg.seed, dptr);
+      fill_adjoint_args(A, a.H, *a.prog, a.md, a.grid, J.cfg.seed, dptr);
       A.p.n_paths = J.cfg.n_paths;
       A.p.path_base = 0;
       A.p.partials = partials + a.part_off;

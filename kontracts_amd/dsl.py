@@ -633,7 +633,33 @@ def _dbl(xs) -> C.Array:
     return (C.c_double * len(a)).from_buffer(a)
 
 
+def _model_key(model):
+    """The values _model_struct reads from a model (its cache key)."""
+    if isinstance(model, GBM):
+        return (model.s0, model.sigma, model.r, model.q, model.asset)
+    p = model.params
+    if model.kind == N.KT_MODEL_CORRELATED_GBM:
+        return (tuple((f.name, f.s0, f.mu, f.sigma) for f in p["factors"]),
+                tuple(tuple(row) for row in p["corr"]))
+    return tuple(p.items())
+
+
 def _model_struct(model) -> _Struct:
+    """The model's kt_model struct, kept on the model object and rebuilt
+    when any value it was built from has changed (a book priced again
+    marshals each model once)."""
+    if not isinstance(model, (GBM, Model)):
+        raise TypeError("expected a GBM or Model")
+    key = _model_key(model)
+    cached = getattr(model, "_kt_struct", None)
+    if cached is not None and cached[0] == key:
+        return cached[1]
+    st = _build_model_struct(model)
+    model._kt_struct = (key, st)
+    return st
+
+
+def _build_model_struct(model) -> _Struct:
     if isinstance(model, GBM):
         names = (C.c_char_p * 1)(model.asset.encode())
         s0, mu, sg = _dbl([model.s0]), _dbl([model.r - model.q]), _dbl([model.sigma])
