@@ -136,7 +136,7 @@ def bench_book(args, eng, K, B, R, T, threads):
     lane_ops = sum(job[2] * (len(job[5]) - 1) * R.book_job(kind, model, n, nf, greeks=True)
                    for job, (kind, model, n, nf) in zip(jobs, profiles))
     if args.no_tune:
-        tuned = {"best": (0, 1, 0, 0), "device_ms": {}}
+        tuned = {"best": T.DEFAULT_TILE, "device_ms": {}}
     else:
         tuned = T.autotune_portfolio(jobs, repeats=1, greeks=True)
     log(f"config4: autotuned tile {tuned['best']} ({', '.join(f'{k}: {v:.1f} ms' for k, v in tuned['device_ms'].items())})")
@@ -169,7 +169,10 @@ def bench_book(args, eng, K, B, R, T, threads):
                     "50/50; pathwise Greeks on the 750 Lipschitz contracts, barriers priced; persistent launches "
                     "under the autotuned tile",
         "contracts": len(jobs), "path_steps": path_steps, "kinds": kinds, "with_greeks": n_greeks,
-        "tile": {"best": list(tuned["best"]), "device_ms": tuned["device_ms"]},
+        "tile": {"best": dict(zip(("blocks_per_cu", "queue_chunk", "price_waves_per_simd", "greeks_waves_per_simd",
+                                   "groups_per_iteration", "lanes_per_workgroup"), tuned["best"])),
+                 "paths_per_lane": 256 // tuned["best"][5],
+                 "device_ms": tuned["device_ms"]},
         "parallelism": "single GPU",
     }
     if args.no_compare:

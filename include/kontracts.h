@@ -275,6 +275,11 @@ kt_status kt_engine_kernel_time(kt_engine* engine, double* total_ms, uint64_t* n
  *                      measured faster on ROCm for books of 10-100 jobs.
  *   "portfolio_waves": register budget of the persistent price kernel: 0
  *                      (default: the compiler's), 6 or 8 waves per SIMD.
+ *   "portfolio_unroll": 4-point groups per iteration of the persistent price
+ *                      kernel's uniform-run loops: 1 (default) or 2.
+ *   "portfolio_lanes": lanes per workgroup of the persistent price kernel:
+ *                      256 (default, one path per lane) or 128 (each lane
+ *                      prices two paths of a 256-path block in turn).
  *   "portfolio_adjoint_waves": register budget of the persistent Greeks
  *                      kernels: 0 (default: 102 VGPRs for baskets and Heston
  *                      with running accumulators, 128 for Heston, the

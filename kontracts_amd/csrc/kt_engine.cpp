@@ -855,7 +855,13 @@ kt_status kt_engine_set_option(kt_engine* E, const char* name, int64_t value) {
     E->pf_use_graph = value == 1;
   } else if (key == "portfolio_waves") {
     if (value != 0 && value != 6 && value != 8) raise(KT_ERR_INVALID_ARGUMENT, "portfolio_waves must be 0, 6 or 8");
-    E->pf_waves = (int)value;
+    E->pf_tile.waves = (int)value;
+  } else if (key == "portfolio_unroll") {
+    if (value != 1 && value != 2) raise(KT_ERR_INVALID_ARGUMENT, "portfolio_unroll must be 1 or 2");
+    E->pf_tile.unroll = (int)value;
+  } else if (key == "portfolio_lanes") {
+    if (value != 128 && value != 256) raise(KT_ERR_INVALID_ARGUMENT, "portfolio_lanes must be 128 or 256");
+    E->pf_tile.lanes = (int)value;
   } else if (key == "portfolio_adjoint_waves") {
     if (value != 0 && value != 5) raise(KT_ERR_INVALID_ARGUMENT, "portfolio_adjoint_waves must be 0 or 5");
     E->pf_adj_waves = (int)value;

@@ -45,7 +45,7 @@ struct kt_engine {
   // persistent portfolio launch: tile choice and the captured launch graph
   int pf_blocks_per_cu = 0;  // 0: the kernel's occupancy limit
   int pf_chunk = 1;
-  int pf_waves = 0;          // register-budget tile of the price kernel: 0, 6 or 8 waves per SIMD
+  kt::PortfolioTile pf_tile;  // compiled tile of the price kernel (register budget, unroll, lanes)
   int pf_adj_waves = 0;      // ... of the Greeks kernels: 0 (per-variant default) or 5
   // Direct launches by default: on this stack a hipGraphLaunch of the
   // three-kernel sequence costs more host time than the three launches.
@@ -70,7 +70,9 @@ struct kt_engine {
   // calls unless a call was interrupted.
   DevBuf errp;
   bool errp_zero = false;
-  int occupancy_portfolio[3] = {0, 0, 0};  // cached occupancies (blocks per CU): price kernel at 0 / 6 / 8 waves
+  // cached occupancies (workgroups per CU) of the price kernel's tiles:
+  // [waves 0 / 6 / 8][unroll 1 / 2][lanes 256 / 128]
+  int occupancy_portfolio[3][2][2] = {};
   int occupancy_surface[4] = {0, 0, 0, 0};  // Greek surface kernel: 8 / 16 registers, then their serial-tail variants
   int occupancy_adjoint[2][kt::APV_COUNT] = {};  // Greeks kernels at the default budget / 5 waves
   bool pathwise_tangent = false;  // Heston pathwise Greeks: forward tangents instead of the reverse sweep

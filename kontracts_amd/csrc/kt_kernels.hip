@@ -159,7 +159,12 @@ constexpr bool uniform_loop_built() {
 // seven-scenario kernel, 2% on the 5-asset basket, 2-3% on the Heston
 // European.  Heston runs under a running sum or extreme keep reading the
 // table (7% slower with the copies: more VGPRs).
-template <int KIND, int MODEL, int K, int SC, bool DEBUG, class PA>
+// UNROLL: groups per loop iteration (a tile knob: 4 or 8 grid points per
+// iteration, the normals of both groups drawn before their steps; the same
+// operations on the same values either way).  Written out by hand: the
+// normal generator's wave ballot is convergent, which keeps the compiler
+// from unrolling a loop with a run-time trip count.
+template <int KIND, int MODEL, int K, int SC, bool DEBUG, int UNROLL, class PA>
 __device__ __forceinline__ void uniform_groups(const PA& A, const KGroup gd, int n, uint32_t p_lo, uint32_t p_hi,
                                                uint32_t flip, float (&x)[SC][K], float& v, float (&acc)[SC],
                                                float (&comp)[SC], bool live, uint64_t local) {
@@ -176,7 +181,18 @@ __device__ __forceinline__ void uniform_groups(const PA& A, const KGroup gd, int
         fast_points<KIND, MODEL, K, SC, DEBUG>(A, gd, 0, z, x, v, acc, comp, ext, live, local);
       }
       const StepRow<MODEL, K, SC> R = vgpr_row<MODEL, K, SC>(A, 1);
-      for (int g = 1; g < n; ++g) {
+      int g = 1;
+      if constexpr (UNROLL == 2) {
+        // eight points per iteration: both groups' normals, then their steps
+        for (; g + 1 < n; g += 2) {
+          float z0[4 * M], z1[4 * M];
+          group_normals<M>(g, p_lo, p_hi, A.seed_lo, A.seed_hi, flip, z0);
+          group_normals<M>(g + 1, p_lo, p_hi, A.seed_lo, A.seed_hi, flip, z1);
+          fast_points<KIND, MODEL, K, SC, DEBUG, true>(A, gd, 4 * g, z0, x, v, acc, comp, ext, live, local, &R);
+          fast_points<KIND, MODEL, K, SC, DEBUG, true>(A, gd, 4 * g + 4, z1, x, v, acc, comp, ext, live, local, &R);
+        }
+      }
+      for (; g < n; ++g) {
         float z[4 * M];
         group_normals<M>(g, p_lo, p_hi, A.seed_lo, A.seed_hi, flip, z);
         fast_points<KIND, MODEL, K, SC, DEBUG, true>(A, gd, 4 * g, z, x, v, acc, comp, ext, live, local, &R);
@@ -184,7 +200,17 @@ __device__ __forceinline__ void uniform_groups(const PA& A, const KGroup gd, int
       g_table = n;
     }
   }
-  for (int g = g_table; g < n; ++g) {
+  int g = g_table;
+  if constexpr (UNROLL == 2) {
+    for (; g + 1 < n; g += 2) {
+      float z0[4 * M], z1[4 * M];
+      group_normals<M>(g, p_lo, p_hi, A.seed_lo, A.seed_hi, flip, z0);
+      group_normals<M>(g + 1, p_lo, p_hi, A.seed_lo, A.seed_hi, flip, z1);
+      fast_points<KIND, MODEL, K, SC, DEBUG>(A, gd, 4 * g, z0, x, v, acc, comp, ext, live, local);
+      fast_points<KIND, MODEL, K, SC, DEBUG>(A, gd, 4 * g + 4, z1, x, v, acc, comp, ext, live, local);
+    }
+  }
+  for (; g < n; ++g) {
     float z[4 * M];
     group_normals<M>(g, p_lo, p_hi, A.seed_lo, A.seed_hi, flip, z);
     fast_points<KIND, MODEL, K, SC, DEBUG>(A, gd, 4 * g, z, x, v, acc, comp, ext, live, local);
@@ -214,13 +240,25 @@ struct OneScenario {
 // keeps 0.5 * (pv_base + pv_anti) (src/variance_reduction.rs:196-201).
 // Complementing every bit of a Philox word maps the uniform u of its cell to
 // 1 - u, so normal_from_word(~w) == -normal_from_word(w) exactly.
+// Tile knobs (the persistent portfolio kernel's variants): UNROLL groups per
+// iteration of the uniform-run loops, and LANES lanes per workgroup: with
+// LANES = 128 each lane prices two paths of the 256-path block one after the
+// other (pass p takes paths p * 128 .. p * 128 + 127), and each wave's
+// statistics land in the slot of the 64 paths it covered, so the block's
+// partials are combined in the same order as with 256 lanes: the results do
+// not depend on the tile.
 template <int MODEL, int K, int NS, int NOUT, int SC, int NSTAT, bool DEBUG, bool ANTI = false, bool SERIAL = false,
-          class PA>
+          int UNROLL = 1, int LANES = kBlock, class PA>
 __device__ __forceinline__ void path_block(const PA& A, uint32_t blk, float (&r)[SC][NS]) {
   static_assert(!ANTI || (SC == 1 && !DEBUG), "antithetic pairs: one scenario, no debug output");
   static_assert(!SERIAL || (SC > 1 && !DEBUG && MODEL == MK_GBM && K == 1), "scenario-serial tail: GBM scenarios");
+  static_assert(LANES == kBlock || (LANES == kBlock / 2 && !SERIAL), "lanes per workgroup: 256, or 128 x 2 passes");
   constexpr int M = MODEL == MK_GBM ? 1 : (MODEL == MK_HESTON ? 2 : K);
-  const uint64_t local = (uint64_t)blk * kBlock + threadIdx.x;
+  __shared__ KStats sh[NSTAT][kBlock / 64];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll 1
+  for (int pass = 0; pass < kBlock / LANES; ++pass) {
+  const uint64_t local = (uint64_t)blk * kBlock + (uint64_t)pass * LANES + threadIdx.x;
   const bool live = local < A.n_paths;
   const uint64_t path = A.path_base + local;
   const uint32_t p_lo = (uint32_t)path, p_hi = (uint32_t)(path >> 32);
@@ -324,8 +362,8 @@ __device__ __forceinline__ void path_block(const PA& A, uint32_t blk, float (&r)
   case KIND:                                                                                          \
     if constexpr (uniform_loop_built<KIND, MODEL>()) {                                                \
       enter();                                                                                        \
-      uniform_groups<KIND, MODEL, K, SC, DEBUG>(A, gu, nu, p_lo, p_hi, flip, x, v, macc, mcomp, live, \
-                                                local);                                               \
+      uniform_groups<KIND, MODEL, K, SC, DEBUG, UNROLL>(A, gu, nu, p_lo, p_hi, flip, x, v, macc, mcomp,  \
+                                                        live, local);                                 \
     }                                                                                                 \
     break;
         KT_UNIFORM(G_NONE)
@@ -523,14 +561,14 @@ __device__ __forceinline__ void path_block(const PA& A, uint32_t blk, float (&r)
       y[0] = t;
     }
   }
-  // Block statistics per output: wave shifted sums, then Chan across waves.
-  __shared__ KStats sh[NSTAT][kBlock / 64];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  // Block statistics per output: wave shifted sums (slot: the wave's 64
+  // paths within the block), then Chan across the slots.
 #pragma unroll
   for (int j = 0; j < NSTAT; ++j) {
     const KStats st = wave_stats(y[j], live);
-    if (lane == 0) sh[j][wave] = st;
+    if (lane == 0) sh[j][pass * (LANES / 64) + wave] = st;
   }
+  }  // pass
   __syncthreads();
   if (threadIdx.x < NSTAT) {
     KStats acc = sh[threadIdx.x][0];
@@ -618,8 +656,8 @@ __global__ void portfolio_reset_kernel(unsigned* counter, unsigned* err, int n_j
 
 // WAVES > 0: the register budget for WAVES waves per SIMD (a tile the
 // autotuner may pick); 0 leaves it to the compiler.
-template <int NS, int WAVES>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WAVES > 0 ? WAVES : 1))) void portfolio_kernel(
+template <int NS, int WAVES, int UNROLL, int LANES>
+__global__ __launch_bounds__(LANES) __attribute__((amdgpu_waves_per_eu(WAVES > 0 ? WAVES : 1))) void portfolio_kernel(
     const PortfolioArgs Q) {
   __shared__ uint32_t s_first;
   float r[1][NS];
@@ -636,12 +674,12 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WAVES > 
       const uint32_t j = items[2 * (size_t)it], blk = items[2 * (size_t)it + 1];
       const KT_CONST PathArgs& A = jobs[j].a;
       switch (jobs[j].variant) {
-        case PV_GBM: path_block<MK_GBM, 1, NS, 1, 1, 1, false>(A, blk, r); break;
-        case PV_HESTON: path_block<MK_HESTON, 1, NS, 1, 1, 1, false>(A, blk, r); break;
-        case PV_BASKET2: path_block<MK_BASKET, 2, NS, 1, 1, 1, false>(A, blk, r); break;
-        case PV_BASKET3: path_block<MK_BASKET, 3, NS, 1, 1, 1, false>(A, blk, r); break;
-        case PV_BASKET4: path_block<MK_BASKET, 4, NS, 1, 1, 1, false>(A, blk, r); break;
-        default: path_block<MK_BASKET, 5, NS, 1, 1, 1, false>(A, blk, r); break;
+        case PV_GBM: path_block<MK_GBM, 1, NS, 1, 1, 1, false, false, false, UNROLL, LANES>(A, blk, r); break;
+        case PV_HESTON: path_block<MK_HESTON, 1, NS, 1, 1, 1, false, false, false, UNROLL, LANES>(A, blk, r); break;
+        case PV_BASKET2: path_block<MK_BASKET, 2, NS, 1, 1, 1, false, false, false, UNROLL, LANES>(A, blk, r); break;
+        case PV_BASKET3: path_block<MK_BASKET, 3, NS, 1, 1, 1, false, false, false, UNROLL, LANES>(A, blk, r); break;
+        case PV_BASKET4: path_block<MK_BASKET, 4, NS, 1, 1, 1, false, false, false, UNROLL, LANES>(A, blk, r); break;
+        default: path_block<MK_BASKET, 5, NS, 1, 1, 1, false, false, false, UNROLL, LANES>(A, blk, r); break;
       }
     }
   }
@@ -1271,30 +1309,47 @@ hipError_t launch_diff_stats(const double* a, const double* b, double scale, uin
   return hipGetLastError();
 }
 
-int portfolio_occupancy_blocks_per_cu(int waves) {
+// The compiled tiles of the persistent price kernel: register budget (0:
+// the compiler's, 6 or 8 waves per SIMD) x groups per loop iteration (1, 2)
+// x lanes per workgroup (256, or 128 with two paths per lane).
+#define KT_PF_TILES(X) \
+  X(0, 1, 256) X(6, 1, 256) X(8, 1, 256) X(0, 2, 256) X(6, 2, 256) X(8, 2, 256) \
+  X(0, 1, 128) X(6, 1, 128) X(8, 1, 128) X(0, 2, 128) X(6, 2, 128) X(8, 2, 128)
+
+bool portfolio_tile_valid(const PortfolioTile& t) {
+#define KT_PF_VALID(W, U, L) \
+  if (t.waves == W && t.unroll == U && t.lanes == L) return true;
+  KT_PF_TILES(KT_PF_VALID)
+#undef KT_PF_VALID
+  return false;
+}
+
+int portfolio_occupancy_blocks_per_cu(const PortfolioTile& t) {
   int n = 0;
-  hipError_t e;
-  switch (waves) {
-    case 6: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, portfolio_kernel<16, 6>, kBlock, 0); break;
-    case 8: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, portfolio_kernel<16, 8>, kBlock, 0); break;
-    default: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, portfolio_kernel<16, 0>, kBlock, 0); break;
-  }
+  hipError_t e = hipErrorInvalidValue;
+#define KT_PF_OCC(W, U, L)                                                                        \
+  if (t.waves == W && t.unroll == U && t.lanes == L)                                              \
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, portfolio_kernel<16, W, U, L>, L, 0);
+  KT_PF_TILES(KT_PF_OCC)
+#undef KT_PF_OCC
   if (e != hipSuccess) return 4;
   return n > 0 ? n : 1;
 }
 
 hipError_t launch_portfolio(const PortfolioArgs& q, const int* n_blocks, int n_jobs, int grid, unsigned* err,
-                            KStats* out, unsigned* err_copy, hipStream_t s, int waves) {
+                            KStats* out, unsigned* err_copy, hipStream_t s, const PortfolioTile& t) {
+  if (!portfolio_tile_valid(t)) return hipErrorInvalidValue;
   hipLaunchKernelGGL(portfolio_reset_kernel, dim3(std::max(1, std::min(64, (n_jobs + 255) / 256))), dim3(256), 0, s,
                      q.counter, err, n_jobs);
-  switch (waves) {
-    case 6: hipLaunchKernelGGL((portfolio_kernel<16, 6>), dim3(grid), dim3(kBlock), 0, s, q); break;
-    case 8: hipLaunchKernelGGL((portfolio_kernel<16, 8>), dim3(grid), dim3(kBlock), 0, s, q); break;
-    default: hipLaunchKernelGGL((portfolio_kernel<16, 0>), dim3(grid), dim3(kBlock), 0, s, q); break;
-  }
+#define KT_PF_LAUNCH(W, U, L)                      \
+  if (t.waves == W && t.unroll == U && t.lanes == L) \
+    hipLaunchKernelGGL((portfolio_kernel<16, W, U, L>), dim3(grid), dim3(L), 0, s, q);
+  KT_PF_TILES(KT_PF_LAUNCH)
+#undef KT_PF_LAUNCH
   hipLaunchKernelGGL(portfolio_finalize_kernel, dim3(n_jobs), dim3(256), 0, s, q.jobs, n_blocks, out, err_copy);
   return hipGetLastError();
 }
+#undef KT_PF_TILES
 
 int surface_occupancy_blocks_per_cu(int ns, bool serial) {
   int n = 0;

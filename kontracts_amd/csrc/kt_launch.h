@@ -176,12 +176,19 @@ struct PortfolioArgs {
 // Reset the queue and the jobs' error words err[n_jobs], run the persistent
 // grid, finalize each job into out[job] and copy its error word to
 // err_copy[job].
-// waves: the register-budget tile of the persistent price kernel: 0 for the
-// compiler's choice, 6 or 8 waves per SIMD (at most 80 / 64 VGPRs).
+// The tile of the persistent price kernel (compiled variants): waves, the
+// register budget (0 for the compiler's choice, 6 or 8 waves per SIMD: at
+// most 80 / 64 VGPRs); unroll, 4-point groups per iteration of the uniform
+// runs (1 or 2); lanes per workgroup (256, or 128 with two paths per lane).
+struct PortfolioTile {
+  int waves = 0, unroll = 1, lanes = kBlock;
+};
+bool portfolio_tile_valid(const PortfolioTile& t);
 hipError_t launch_portfolio(const PortfolioArgs& q, const int* n_blocks, int n_jobs, int grid, unsigned* err,
-                            KStats* out, unsigned* err_copy, hipStream_t s, int waves = 0);
-// Resident 256-lane blocks of the persistent kernel per compute unit.
-int portfolio_occupancy_blocks_per_cu(int waves = 0);
+                            KStats* out, unsigned* err_copy, hipStream_t s, const PortfolioTile& t);
+// Resident workgroups (t.lanes lanes each) of the persistent kernel per
+// compute unit.
+int portfolio_occupancy_blocks_per_cu(const PortfolioTile& t);
 
 // partials[block] = statistics of (a[i] - b[i]) * scale over the block's paths.
 hipError_t launch_diff_stats(const double* a, const double* b, double scale, uint64_t n, KStats* partials,

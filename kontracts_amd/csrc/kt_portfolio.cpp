@@ -228,8 +228,10 @@ PriceBookHandle launch_price_book(kt_engine* E, const kt_job* jobs, uint64_t n_j
   const int* nblk_dev = (const int*)(dev + jobs_bytes + items_bytes);
   int n_cu = 0;
   check_hip(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, E->device), "device attribute");
-  int& occ = E->occupancy_portfolio[E->pf_waves == 6 ? 1 : E->pf_waves == 8 ? 2 : 0];
-  if (occ <= 0) occ = portfolio_occupancy_blocks_per_cu(E->pf_waves);
+  const PortfolioTile& tile = E->pf_tile;
+  int& occ = E->occupancy_portfolio[tile.waves == 6 ? 1 : tile.waves == 8 ? 2 : 0][tile.unroll == 2 ? 1 : 0]
+                                   [tile.lanes == kBlock ? 0 : 1];
+  if (occ <= 0) occ = portfolio_occupancy_blocks_per_cu(tile);
   const int bpc = E->pf_blocks_per_cu > 0 ? E->pf_blocks_per_cu : occ;
   const uint64_t slots = (uint64_t)n_cu * bpc;
   // A queue chunk of c items saves atomics only where items outnumber the
@@ -243,11 +245,11 @@ PriceBookHandle launch_price_book(kt_engine* E, const kt_job* jobs, uint64_t n_j
   // (slot 0 only; slot 1 launches directly).
   const std::vector<uint64_t> key = {(uint64_t)dev, (uint64_t)partials, (uint64_t)stats, (uint64_t)errw,
                                      n_items, book.size(), (uint64_t)grid, (uint64_t)Q.chunk,
-                                     (uint64_t)E->pf_waves};
+                                     (uint64_t)tile.waves, (uint64_t)tile.unroll, (uint64_t)tile.lanes};
   const bool graph = E->pf_use_graph && slot == 0;
   E->prof_begin();
   if (!graph) {
-    check_hip(launch_portfolio(Q, nblk_dev, (int)book.size(), grid, errw, stats, err_copy, E->stream, E->pf_waves),
+    check_hip(launch_portfolio(Q, nblk_dev, (int)book.size(), grid, errw, stats, err_copy, E->stream, tile),
               "portfolio launch");
   } else if (!E->pf_exec || E->pf_key != key) {
     if (E->pf_exec) (void)hipGraphExecDestroy(E->pf_exec);
@@ -257,7 +259,7 @@ PriceBookHandle launch_price_book(kt_engine* E, const kt_job* jobs, uint64_t n_j
     E->pf_key.clear();
     check_hip(hipStreamBeginCapture(E->stream, hipStreamCaptureModeThreadLocal), "begin capture");
     hipError_t le =
-        launch_portfolio(Q, nblk_dev, (int)book.size(), grid, errw, stats, err_copy, E->stream, E->pf_waves);
+        launch_portfolio(Q, nblk_dev, (int)book.size(), grid, errw, stats, err_copy, E->stream, tile);
     hipGraph_t g = nullptr;
     hipError_t ce = hipStreamEndCapture(E->stream, &g);
     check_hip(le, "portfolio launch");

@@ -88,7 +88,8 @@ def test_portfolio_tile_configs_and_graph_replay():
         eng.set_option("portfolio_chunk", 1)
         eng.set_option("portfolio_graph", 0)
     for bad in (("portfolio_chunk", 0), ("portfolio_blocks_per_cu", -1), ("portfolio_graph", 2),
-                ("portfolio_waves", 5), ("portfolio_adjoint_waves", 4), ("lsm_graph", 2)):
+                ("portfolio_waves", 5), ("portfolio_adjoint_waves", 4), ("lsm_graph", 2),
+                ("portfolio_unroll", 4), ("portfolio_lanes", 64)):
         with pytest.raises(N.KontractError):
             eng.set_option(*bad)
 
@@ -106,11 +107,22 @@ def test_portfolio_register_budget_tiles():
             eng.set_option("portfolio_waves", pw)
             assert_identical(K.price_portfolio(jobs), want)
         eng.set_option("portfolio_waves", 0)
+        # step unroll (8 points per iteration) and 128-lane workgroups with
+        # two paths per lane: same statistics, combined in the same order
+        for unroll, lanes in ((2, 256), (1, 128), (2, 128)):
+            eng.set_option("portfolio_unroll", unroll)
+            eng.set_option("portfolio_lanes", lanes)
+            assert_identical(K.price_portfolio(jobs), want)
+            eng.set_option("portfolio_waves", 8)
+            assert_identical(K.price_portfolio(jobs), want)
+            eng.set_option("portfolio_waves", 0)
         eng.set_option("portfolio_adjoint_waves", 5)
         assert_greeks_identical(K.price_portfolio(gjobs, greeks=True), gwant)
     finally:
         eng.set_option("portfolio_waves", 0)
         eng.set_option("portfolio_adjoint_waves", 0)
+        eng.set_option("portfolio_unroll", 1)
+        eng.set_option("portfolio_lanes", 256)
 
 
 def test_autotuner_sets_a_measured_tile():
@@ -120,15 +132,13 @@ def test_autotuner_sets_a_measured_tile():
     want = price_each(jobs)
     try:
         r = T.autotune_portfolio(jobs, candidates=((0, 1), (2, 4)), repeats=1)
-        assert r["best"] in {tuple(int(x.strip("waw")) for x in k.replace("x", "/").split("/"))
-                             for k in r["device_ms"]}
-        assert len(r["device_ms"]) == 2 + 2  # two (grid, chunk) pairs, then price budgets 6 and 8
+        assert T.tile_name(r["best"]) in r["device_ms"]
+        # two (grid, chunk) pairs, price budgets 6 and 8, three (unroll, lanes) variants
+        assert len(r["device_ms"]) == 2 + 2 + 3
         assert all(v > 0 for v in r["device_ms"].values())
         assert_identical(K.price_portfolio(jobs), want)  # under the tuned tile
     finally:
-        for name in ("portfolio_blocks_per_cu", "portfolio_waves", "portfolio_adjoint_waves"):
-            eng.set_option(name, 0)
-        eng.set_option("portfolio_chunk", 1)
+        T.set_tile(eng, T.DEFAULT_TILE)
 
 
 def test_portfolio_errors_like_a_loop():
