@@ -72,12 +72,19 @@ def gbm_asian() -> float:
 # 4 FMAs) or, under Heston, the reverse sweep's seed (exp2, mul, 2 FMAs).
 FIXING = 2 + 4
 FIXING_PATHWISE = FIXING + 4
-# Heston reverse sweep per point, no running accumulators: vp = sv^2, the
-# correlated normal (2), the floor's cut (2), the adjoint of the log step and
-# of the variance step (6), the four parameter accumulators (1 + 3 + 3 + 4),
-# d sqrt(v) / dv through rcp (3) and its cut (2), vbar (2) = 28; with running
-# accumulators the sweep also walks the log-spot back (4).
-HESTON_REVERSE = 28.0
+# Heston reverse sweep per point.  The recompute of a segment regenerates
+# both normals and, without running accumulators, steps the variance alone:
+# max, sqrt and the 7 variance instructions = 9 (with accumulators the full
+# step, 13).  The sweep on a grid of equal steps without accumulators, its
+# per-point constants folded out of the loop: vp = sv^2 (1), the correlated
+# normal (2), the floor's cut of vbar (2), vbar sv (1), the kappa, theta,
+# sigma_v and rho sums (2 + 1 + 1 + 2), half of d/d sqrt(v) (3), its
+# reciprocal with the cut (3), d/dv of the step (2), vbar (2) = 22.  The
+# general sweep (tables read per point, or running accumulators) forms every
+# term per point: 28, plus the log-spot walk-back (4) with accumulators.
+HESTON_RECOMPUTE_VARIANCE = 9.0
+HESTON_REVERSE = 22.0
+HESTON_REVERSE_GENERAL = 28.0
 HESTON_REVERSE_FEEDS = 4.0
 
 
@@ -96,8 +103,12 @@ def basket_pathwise(k: int) -> float:
 
 def heston_pathwise(feeds: bool = False) -> float:
     """Forward pass, recompute of every segment from its checkpoint (both
-    normals regenerated) and the reverse sweep."""
-    return 2 * heston_european() + HESTON_REVERSE + (HESTON_REVERSE_FEEDS if feeds else 0.0)
+    normals regenerated) and the reverse sweep: the equal-step sweep without
+    running accumulators, or the general sweep with the log-spot recomputed
+    and walked back."""
+    if feeds:
+        return 2 * heston_european() + HESTON_REVERSE_GENERAL + HESTON_REVERSE_FEEDS
+    return heston_european() + 2 * NORMAL + HESTON_RECOMPUTE_VARIANCE + HESTON_REVERSE
 
 
 def book_job(kind: str, model: str, n_steps: int, n_fixings: int = 0, greeks: bool = False) -> float:
