@@ -173,19 +173,49 @@ const char* error_text(int code) {
 
 std::shared_ptr<const Program> lower_cached(kt_engine* E, const char* json, const Con& con,
                                             const std::vector<double>& grid, const std::vector<std::string>& names) {
+  // key: the text, the names and a 64-bit mix of the grid's bits (the grid
+  // itself is compared on a hit, so a collision only costs a re-lowering)
+  uint64_t h = 0x9E3779B97F4A7C15ull ^ grid.size();
+  for (double t : grid) {
+    uint64_t b;
+    std::memcpy(&b, &t, sizeof b);
+    h = (h ^ b) * 0xBF58476D1CE4E5B9ull;
+    h ^= h >> 31;
+  }
   std::string key(json);
-  key.push_back('\0');
-  key.append((const char*)grid.data(), grid.size() * sizeof(double));
   for (const std::string& n : names) {
     key.push_back('\0');
     key += n;
   }
+  key.push_back('\0');
+  key.append((const char*)&h, sizeof h);
   auto it = E->lower_cache.find(key);
-  if (it != E->lower_cache.end()) return it->second;
+  if (it != E->lower_cache.end() && it->second.grid == grid) return it->second.prog;
   auto p = std::make_shared<const Program>(lower({&con}, grid, names));
-  if (E->lower_cache.size() >= 16384) E->lower_cache.clear();
-  E->lower_cache.emplace(std::move(key), p);
+  if (E->lower_cache.size() >= 16384) {
+    E->lower_cache.clear();
+    E->adj_plan_cache.clear();
+  }
+  if (it != E->lower_cache.end()) {
+    E->adj_plan_cache.erase(it->second.prog.get());
+    it->second = kt_engine::Lowered{grid, p};
+  } else {
+    E->lower_cache.emplace(std::move(key), kt_engine::Lowered{grid, p});
+  }
   return p;
+}
+
+std::shared_ptr<const AdjPlan> adjoint_plan_cached(kt_engine* E, const std::shared_ptr<const Program>& prog) {
+  auto it = E->adj_plan_cache.find(prog.get());
+  if (it != E->adj_plan_cache.end()) return it->second;
+  std::shared_ptr<const AdjPlan> plan;
+  try {
+    plan = std::make_shared<const AdjPlan>(classify_adjoint(*prog));
+  } catch (const Error& e) {
+    if (e.code != KT_ERR_UNSUPPORTED) throw;
+  }
+  E->adj_plan_cache.emplace(prog.get(), plan);
+  return plan;
 }
 
 std::string error_text(int code, const Program& prog) {

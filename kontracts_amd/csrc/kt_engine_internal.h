@@ -92,7 +92,16 @@ struct kt_engine {
   int adj_seg = kt::kAdjSeg;  // Heston adjoint checkpoint interval (kt_engine_set_option)
   // Lowered programs of recent (contract JSON, grid, asset names): a book
   // priced again (new market data, same contracts) skips the lowering.
-  std::unordered_map<std::string, std::shared_ptr<const kt::Program>> lower_cache;
+  // Keyed by the contract text, the asset names and a hash of the grid; an
+  // entry keeps its grid, compared on every hit.
+  struct Lowered {
+    std::vector<double> grid;
+    std::shared_ptr<const kt::Program> prog;
+  };
+  std::unordered_map<std::string, Lowered> lower_cache;
+  // classify_adjoint of cached programs (null plan: not pathwise-Greek-able),
+  // cleared with lower_cache.
+  std::unordered_map<const kt::Program*, std::shared_ptr<const kt::AdjPlan>> adj_plan_cache;
   // Parsed contracts and their plans of recent contract JSON texts.
   struct Parsed {
     std::shared_ptr<const kt::Con> con;
@@ -161,6 +170,9 @@ std::string error_text(int code, const Program& prog);
 // on the engine (the program depends on nothing else).
 std::shared_ptr<const Program> lower_cached(kt_engine* E, const char* json, const Con& con,
                                             const std::vector<double>& grid, const std::vector<std::string>& names);
+// classify_adjoint of a program returned by lower_cached, memoised with it;
+// null when the program has no pathwise Greeks (Unsupported).
+std::shared_ptr<const AdjPlan> adjoint_plan_cached(kt_engine* E, const std::shared_ptr<const Program>& prog);
 
 // allow_vr: the caller handles cfg's variance-reduction flags; otherwise
 // they are rejected.

@@ -79,12 +79,9 @@ void price_portfolio_greeks(kt_engine* E, const kt_job* jobs, uint64_t n_jobs, k
     for (const KOp& op : prog.ops) ok = ok && op.code != OP_FLAGERR;  // path errors: the pricer raises them
     AdjPlan plan;
     if (ok) {
-      try {
-        plan = classify_adjoint(prog);
-      } catch (const Error& e) {
-        if (e.code != KT_ERR_UNSUPPORTED) throw;
-        ok = false;
-      }
+      const std::shared_ptr<const AdjPlan> cached = adjoint_plan_cached(E, prog_ptr);
+      if (cached) plan = *cached;
+      else ok = false;
     }
     const int kind = model_kernel_kind(md), K = (int)md.names.size();
     int ns = 0;
@@ -109,7 +106,7 @@ void price_portfolio_greeks(kt_engine* E, const kt_job* jobs, uint64_t n_jobs, k
     AdjJob a;
     a.index = j;
     a.md = std::move(md);
-    a.grid = pc.grid;
+    a.grid = std::move(pc.grid);
     a.prog = std::move(prog_ptr);
     a.plan = std::move(plan);
     a.rate = J.cfg.rate;
