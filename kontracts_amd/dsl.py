@@ -11,6 +11,7 @@ import array
 import ctypes as C
 import json
 import math
+import threading
 from typing import Iterable, Sequence
 
 import numpy as np
@@ -220,6 +221,15 @@ class PriceResult:
         self.ci95_low = r.ci95_low
         self.ci95_high = r.ci95_high
         self.n_paths = int(r.n_paths)
+
+    @classmethod
+    def _from_fields(cls, t) -> "PriceResult":
+        """From a kt_price_result's fields in declaration order (price,
+        sample_std, std_error, ci95_low, ci95_high, n_paths)."""
+        r = cls.__new__(cls)
+        r.price, r.sample_std, r.std_error, r.ci95_low, r.ci95_high = t[0], t[1], t[2], t[3], t[4]
+        r.n_paths = int(t[5])
+        return r
 
     def paths_for_tolerance(self, tol: float) -> int:
         """src/pricer.rs:81-86."""
@@ -711,8 +721,32 @@ def price_batch(contracts: Iterable[Contract], model, n_paths: int = 100_000, se
     return [PriceResult(r) for r in out]
 
 
+class _JobsMemo:
+    """The last book's kt_job array.  A book priced again with the same job
+    tuples (tuples are immutable, and so are contracts) reuses it when every
+    model's struct is unchanged (models are mutable: _model_struct rebuilds
+    the struct of a changed model, and the array is then built afresh).  The
+    array is only read by the engine."""
+
+    def __init__(self):
+        self.jobs = None
+        self.arr = None
+        self.models = None
+        self.keep = None
+
+
+_jobs_memo = _JobsMemo()
+_jobs_lock = threading.Lock()
+
+
 def _jobs_array(jobs):
+    with _jobs_lock:
+        memo = _jobs_memo
+        if memo.jobs is not None and len(memo.jobs) == len(jobs) and all(a is b for a, b in zip(jobs, memo.jobs)) \
+                and all(_model_struct(job[1]) is m for job, m in zip(jobs, memo.models)):
+            return memo.arr, memo.keep
     keep = []
+    models = []
     arr = (N.KtJob * len(jobs))()
     for i, job in enumerate(jobs):
         con, model, n_paths, seed, spy = job[:5]
@@ -723,9 +757,12 @@ def _jobs_array(jobs):
         cfg = _cfg(n_paths, seed, spy, model.rate, grid, anti, cv)
         js = con.to_json().encode()
         keep += [m, cfg, js]
+        models.append(m)
         arr[i].contract_json = js
         arr[i].model = m.s
         arr[i].cfg = cfg.s
+    with _jobs_lock:
+        memo.jobs, memo.arr, memo.models, memo.keep = list(jobs), arr, models, keep
     return arr, keep
 
 
@@ -749,21 +786,24 @@ def price_portfolio(jobs: Sequence[tuple], device: int | None = None, greeks: bo
     eng = N.engine(device)
     if not greeks:
         N.check(N.lib().kt_price_portfolio(eng.handle, arr, len(jobs), out), eng.handle)
-        return [PriceResult(r) for r in out]
+        return [PriceResult._from_fields(t) for t in np.ctypeslib.as_array(out).tolist()]
     stride = 24
     g = (C.c_double * (stride * len(jobs)))()
     se = (C.c_double * (stride * len(jobs)))()
     N.check(N.lib().kt_price_portfolio_greeks(eng.handle, arr, len(jobs), out, g, se, stride), eng.handle)
+    # the outputs as Python lists in one conversion each
+    prices = np.ctypeslib.as_array(out).tolist()
+    grows = np.ctypeslib.as_array(g).reshape(len(jobs), stride).tolist()
+    srows = np.ctypeslib.as_array(se).reshape(len(jobs), stride).tolist()
     res = []
     for j, job in enumerate(jobs):
-        pr = PriceResult(out[j])
+        pr = PriceResult._from_fields(prices[j])
         names = _param_names(job[1])
-        row = list(g[j * stride:j * stride + len(names)])
-        if any(math.isnan(x) for x in row):
+        row = grows[j][:len(names)]
+        if any(x != x for x in row):  # NaN: no pathwise Greeks for this job
             res.append((pr, None))
             continue
-        srow = list(se[j * stride:j * stride + len(names)])
-        res.append((pr, ModelGreeks(names, [pr.price] + row, [pr.std_error] + srow, "pathwise")))
+        res.append((pr, ModelGreeks(names, [pr.price] + row, [pr.std_error] + srows[j][:len(names)], "pathwise")))
     return res
 
 
