@@ -195,6 +195,7 @@ std::shared_ptr<const Program> lower_cached(kt_engine* E, const char* json, cons
   if (E->lower_cache.size() >= 16384) {
     E->lower_cache.clear();
     E->adj_plan_cache.clear();
+    it = E->lower_cache.end();
   }
   if (it != E->lower_cache.end()) {
     E->adj_plan_cache.erase(it->second.prog.get());
