@@ -628,12 +628,17 @@ class _Struct:
         self.keep = keep
 
 
+def _shared_f64(xs) -> bool:
+    """True when _dbl shares xs's buffer instead of copying it."""
+    return isinstance(xs, np.ndarray) and xs.dtype == np.float64 and xs.ndim == 1 and bool(xs.size) \
+        and xs.flags.c_contiguous and xs.flags.writeable
+
+
 def _dbl(xs) -> C.Array:
     """A C double array of xs: a float64 numpy array is shared without a
     copy; anything else is copied (array.array converts a list in one C
     loop).  The ctypes array keeps the buffer alive."""
-    if isinstance(xs, np.ndarray) and xs.dtype == np.float64 and xs.ndim == 1 and xs.size \
-            and xs.flags.c_contiguous and xs.flags.writeable:
+    if _shared_f64(xs):
         return (C.c_double * xs.size).from_buffer(xs)
     if isinstance(xs, (list, tuple)) and len(xs) <= 32:
         return (C.c_double * max(len(xs), 1))(*xs)
@@ -725,7 +730,9 @@ class _JobsMemo:
     """The last book's kt_job array.  A book priced again with the same job
     tuples (tuples are immutable, and so are contracts) reuses it when every
     model's struct is unchanged (models are mutable: _model_struct rebuilds
-    the struct of a changed model, and the array is then built afresh).  The
+    the struct of a changed model, and the array is then built afresh).  Only
+    books whose grids are absent, tuples, or float64 arrays the engine reads
+    in place are kept (a copied list could change behind the copy).  The
     array is only read by the engine."""
 
     def __init__(self):
@@ -761,8 +768,9 @@ def _jobs_array(jobs):
         arr[i].contract_json = js
         arr[i].model = m.s
         arr[i].cfg = cfg.s
-    with _jobs_lock:
-        memo.jobs, memo.arr, memo.models, memo.keep = list(jobs), arr, models, keep
+    if all(len(job) <= 5 or job[5] is None or isinstance(job[5], tuple) or _shared_f64(job[5]) for job in jobs):
+        with _jobs_lock:
+            memo.jobs, memo.arr, memo.models, memo.keep = list(jobs), arr, models, keep
     return arr, keep
 
 

@@ -172,3 +172,35 @@ def test_portfolio_book_generator():
     for j in jobs[:8]:
         first = K.describe_program(j[0], j[1], grid=j[5]).splitlines()[0]
         assert int(first.split()[1].rstrip(",")) <= 16
+
+
+def test_job_array_reused_only_while_the_book_is_unchanged():
+    """price_portfolio marshals a book once and reuses the kt_job array while
+    the same job tuples come back with unchanged models; a changed model, a
+    different job or a grid the engine cannot read in place (a list, copied
+    on marshalling) gives a fresh array with the current values."""
+    from kontracts_amd import book as B
+    from kontracts_amd import dsl as D
+    jobs = B.book(12)
+    a1, _ = D._jobs_array(jobs)
+    a2, _ = D._jobs_array(jobs)
+    assert a1 is a2
+    g = next(i for i, j in enumerate(jobs) if isinstance(j[1], D.GBM))
+    h = next(i for i, j in enumerate(jobs) if isinstance(j[1], D.Model) and j[1].kind == N.KT_MODEL_HESTON)
+    jobs[g][1].sigma = 0.31
+    a3, _ = D._jobs_array(jobs)
+    assert a3 is not a2 and a3[g].model.sigma[0] == 0.31
+    jobs[h][1].params["kappa"] = 2.5
+    a4, _ = D._jobs_array(jobs)
+    assert a4 is not a3 and a4[h].model.kappa == 2.5
+    other = list(jobs)
+    other[0] = other[1]
+    a5, _ = D._jobs_array(other)
+    assert a5 is not a4 and a5[0].cfg.seed == jobs[1][3]
+    listed = [j[:5] + (list(j[5]),) for j in jobs]
+    b1, _ = D._jobs_array(listed)
+    b2, _ = D._jobs_array(listed)
+    assert b1 is not b2
+    listed[0][5][1] = 0.5
+    b3, _ = D._jobs_array(listed)
+    assert b3[0].cfg.grid[1] == 0.5
