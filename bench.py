@@ -441,6 +441,74 @@ def bench_config5(eng, K):
     return out
 
 
+# ---------------------------------------------------------------------------
+# The next rows built on the same kernels: Greek surface, variance
+# reduction, calibration-sized calls
+# ---------------------------------------------------------------------------
+def bench_next(eng, K, R):
+    out = {}
+    # Greek surface: 20 x 20 (spot, vol) cells of the config-2 barrier at
+    # 2^16 paths x 252 steps in one launch (src/surface.rs:104-138), against a
+    # loop of greeks calls over 100 of the cells (scaled to 400)
+    bar = K.up_and_out_call("X", 100.0, 200.0, 1.0)
+    spots = [150.0 + 2.5 * i for i in range(20)]
+    vols = [0.15 + 0.01 * j for j in range(20)]
+    surf = lambda: K.greek_surface(bar, "X", spots, vols, n_paths=1 << 16, seed=42, steps_per_year=252)
+    _, t_s, k_s = median_call(eng, surf, reps=3)
+    t0 = time.perf_counter()
+    for s_ in spots[:5]:
+        for v_ in vols:
+            bar.greeks(K.GBM(s_, v_, 0.05, asset="X"), n_paths=1 << 16, seed=42, steps_per_year=252)
+    t_loop = (time.perf_counter() - t0) * 4
+    sps = 400 * 7 * (1 << 16) * 252
+    out["greek_surface"] = {
+        "workload": "20 x 20 (spot 150-197.5, vol 0.15-0.34) cells of the up-and-out call, 65,536 paths x 252 "
+                    "steps, seven CRN scenarios per cell, one launch",
+        "ms": t_s * 1e3, "kernel_ms": k_s, "loop_of_greeks_calls_ms": t_loop * 1e3,
+        "scenario_path_steps_per_s": sps / t_s,
+        "roofline": roof(400 * (1 << 16) * 252 * R.gbm_barrier(7, 5), k_s, "surface_kernel (7 CRN scenarios per cell)"),
+    }
+    # variance reduction on a daily-average Asian call (1,000,000 paths x 252)
+    m1 = K.GBM(100.0, 0.2, 0.05, asset="X")
+    asian = K.scale((K.average(K.spot("X")) - 100.0).clip(0.0), K.one("USD")) @ K.at(1.0)
+    g252 = [i / 252 for i in range(253)]
+    vr = {"workload": "arithmetic-average Asian call (every grid point, incl. t = 0), GBM s0=K=100 sigma=0.2 "
+                      "r=0.05 T=1, 1,000,000 paths x 252 steps (src/variance_reduction.rs estimators)"}
+    for label, anti, cv in (("plain", False, False), ("antithetic", True, False), ("control_variate", False, True),
+                            ("both", True, True)):
+        r, t, k = median_call(eng, lambda: asian.price(m1, n_paths=1_000_000, seed=42, grid=g252, antithetic=anti,
+                                                       control_variate=cv), reps=3)
+        vr[label] = {"price": r.price, "std_error": r.std_error, "ms": t * 1e3, "kernel_ms": k}
+    out["variance_reduction"] = vr
+    # calibration-sized calls: a 1,000-path x 50-step knock-out price, and ten
+    # such quotes as one portfolio call against a loop of ten calls
+    ko = K.up_and_out_call("X", 100.0, 130.0, 1.0)
+    for _ in range(20):
+        ko.price(K.GBM(100.0, 0.2, 0.05, asset="X"), n_paths=1000, seed=1)
+    t0 = time.perf_counter()
+    for i in range(200):
+        ko.price(K.GBM(100.0, 0.2 + 1e-4 * i, 0.05, asset="X"), n_paths=1000, seed=1)
+    single_us = (time.perf_counter() - t0) / 200 * 1e6
+    quotes = [(ko, K.GBM(float(s_), 0.2, 0.05, asset="X"), 1000, 1, 50) for s_ in range(90, 110, 2)]
+    for _ in range(10):
+        K.price_portfolio(quotes)
+    t0 = time.perf_counter()
+    for _ in range(100):
+        K.price_portfolio(quotes)
+    pf_us = (time.perf_counter() - t0) / 100 * 1e6
+    t0 = time.perf_counter()
+    for _ in range(20):
+        for q in quotes:
+            q[0].price(q[1], n_paths=q[2], seed=q[3], steps_per_year=q[4])
+    loop_us = (time.perf_counter() - t0) / 20 * 1e6
+    out["calibration_calls"] = {"workload": "up-and-out call, 1,000 paths x 50 steps per quote",
+                                "single_price_us": single_us, "ten_quotes_portfolio_us": pf_us,
+                                "ten_quotes_loop_us": loop_us}
+    log(f"next rows: surface {t_s * 1e3:.1f} ms ({k_s:.1f} ms kernel) vs loop {t_loop * 1e3:.0f} ms; "
+        f"single price {single_us:.0f} us; 10 quotes {pf_us:.0f} us vs {loop_us:.0f} us")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -452,7 +520,7 @@ def main():
     ap.add_argument("--no-compare", action="store_true",
                     help="book: skip the per-contract and launch-bound comparisons (profiling runs)")
     ap.add_argument("--no-tune", action="store_true", help="book: default tile instead of the autotuner (profiling runs)")
-    ap.add_argument("--only", choices=["book", "config2", "config3", "config1", "config5"], default=None,
+    ap.add_argument("--only", choices=["book", "config2", "config3", "config1", "config5", "next"], default=None,
                     help="run one workload only (profiling runs)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run the plain copy kernel on MIB MiB (profiler byte-counter calibration)")
@@ -494,8 +562,9 @@ def main():
         configs["config1"] = bench_config1(eng, K, R)
     if args.only in (None, "config5"):
         configs["config5"] = bench_config5(eng, K)
+    nxt = bench_next(eng, K, R) if args.only in (None, "next") else None
     if args.only not in (None, "book"):
-        print(json.dumps({"configs": configs}))
+        print(json.dumps({"configs": configs, "next_rows": nxt}))
         return
     value, step_s, info, roofline, extra, cpu = bench_book(args, eng, K, B, R, T, threads)
     line = {
@@ -516,6 +585,7 @@ def main():
         "cpu_baseline": cpu,
         "book": extra,
         "configs": configs,
+        "next_rows": nxt,
         "host": {"cpu_model": model, "affinity_cores": affinity, "threads_used": threads},
     }
     print(json.dumps(line))
