@@ -238,6 +238,11 @@ __device__ __forceinline__ void adjoint_gbm_block(const AA& A, uint32_t blk) {
   auto op_free = [&](int g) {
     return g + 1 < P.n_groups && ld_u32(P.groups, (size_t)g * 4) == (uint32_t)G_NONE;
   };
+  // one past the run of op-free groups starting at g (g itself when g is
+  // not op-free): an op-free group's slot names the first group after its run
+  auto run_end = [&](int g) {
+    return op_free(g) ? min((int)ld_u32(P.groups, (size_t)g * 4 + 1), P.n_groups - 1) : g;
+  };
   int g = 0;
   // (GBM only: with the basket's 26 copies the kernel spills; its price
   // kernel gained 2% from them)
@@ -275,10 +280,11 @@ __device__ __forceinline__ void adjoint_gbm_block(const AA& A, uint32_t blk) {
   }
   for (;; ++g) {
     if (kRuns) {
+      const int e = run_end(g);
       if (urow)
-        for (; op_free(g); ++g) quiet_group(g, row_coef);
+        for (; g < e; ++g) quiet_group(g, row_coef);
       else
-        for (; op_free(g); ++g) quiet_group(g, table_coef);
+        for (; g < e; ++g) quiet_group(g, table_coef);
     }
     if (g >= P.n_groups) break;
     const bool quiet = !kRuns && op_free(g);

@@ -1189,6 +1189,15 @@ void classify_groups(Program& p) {
     }
     p.groups.push_back(grp);
   }
+  // each op-free group names where its run ends, so that a kernel finds the
+  // run's length with one read instead of reading every group's kind
+  int next = (int)p.groups.size();
+  for (int g = (int)p.groups.size() - 1; g >= 0; --g) {
+    if (p.groups[g].kind == G_NONE)
+      p.groups[g].slot = next;
+    else
+      next = g;
+  }
 }
 
 Program lower_exercise(const Con& contract, const std::vector<double>& grid,

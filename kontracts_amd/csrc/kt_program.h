@@ -71,7 +71,9 @@ struct KStep {
 // code with no interpretation; every other group is interpreted point by point.
 enum GroupKind : int32_t {
   G_GENERIC = 0,  // interpret each point's op list
-  G_NONE = 1,     // no ops at any of the four points
+  G_NONE = 1,     // no ops at any of the four points; slot = index of the first
+                  // group after this run of G_NONE groups (the group count when
+                  // the run reaches the end)
   G_KO_GE = 2,    // r[slot] = max(r[slot], S_asset >= imm) at each point (until/knock-out)
   G_KO_GT = 3,
   G_KO_LE = 4,
