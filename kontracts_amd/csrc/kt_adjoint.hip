@@ -170,6 +170,16 @@ __device__ __forceinline__ void init_registers(const PA& P, float (&r)[1][NS]) {
 
 // GBM (K = 1) and correlated GBM (K >= 2): block blk of the job (paths
 // blk*256 .. blk*256+255).  Outputs: price, d/ds0_a, d/dsigma_a, d/dmu_a.
+// One past the run of op-free groups that starts at group g, or g itself
+// when g has ops or is the last group (which may be partial and always takes
+// the per-point path): an op-free group's slot names the first group after
+// its run, so the run's length costs one read.
+template <class PA>
+__device__ __forceinline__ int op_free_run_end(const PA& P, int g) {
+  if (g + 1 >= P.n_groups || ld_u32(P.groups, (size_t)g * 4) != (uint32_t)G_NONE) return g;
+  return min((int)ld_u32(P.groups, (size_t)g * 4 + 1), P.n_groups - 1);
+}
+
 template <int K, int NS, class AA>
 __device__ __forceinline__ void adjoint_gbm_block(const AA& A, uint32_t blk) {
   constexpr int MODEL = K == 1 ? MK_GBM : MK_BASKET;
@@ -238,11 +248,6 @@ __device__ __forceinline__ void adjoint_gbm_block(const AA& A, uint32_t blk) {
   auto op_free = [&](int g) {
     return g + 1 < P.n_groups && ld_u32(P.groups, (size_t)g * 4) == (uint32_t)G_NONE;
   };
-  // one past the run of op-free groups starting at g (g itself when g is
-  // not op-free): an op-free group's slot names the first group after its run
-  auto run_end = [&](int g) {
-    return op_free(g) ? min((int)ld_u32(P.groups, (size_t)g * 4 + 1), P.n_groups - 1) : g;
-  };
   int g = 0;
   // (GBM only: with the basket's 26 copies the kernel spills; its price
   // kernel gained 2% from them)
@@ -280,7 +285,7 @@ __device__ __forceinline__ void adjoint_gbm_block(const AA& A, uint32_t blk) {
   }
   for (;; ++g) {
     if (kRuns) {
-      const int e = run_end(g);
+      const int e = op_free_run_end(P, g);
       if (urow)
         for (; g < e; ++g) quiet_group(g, row_coef);
       else
@@ -672,6 +677,23 @@ __device__ __forceinline__ void adjoint_heston_block(const AA& A, uint32_t blk, 
     }
   }
   for (int g = n_quiet; g < P.n_groups; ++g) {
+    // later runs of groups without ops (between the fixings of a running
+    // average): on a grid of equal steps from VGPR copies of row 1 as well
+    if (P.tab_uniform && g > 0) {
+      const int e = op_free_run_end(P, g);
+      if (e > g) {
+        const StepRow<MK_HESTON, 1, 1> R = vgpr_row<MK_HESTON, 1, 1>(P, 1);
+        for (; g < e; ++g) {
+          float z[8];
+          group_normals<2>(g, p_lo, p_hi, P.seed_lo, P.seed_hi, 0u, z);
+#pragma unroll
+          for (int s = 0; s < 4; ++s) {
+            checkpoint(4 * g + s);
+            advance_row<MK_HESTON, 1, 1>(R, &z[2 * s], x, v);
+          }
+        }
+      }
+    }
     float z[8];
     group_normals<2>(g, p_lo, p_hi, P.seed_lo, P.seed_hi, 0u, z);
     // a later group without ops (full: only the last one can be partial)
@@ -738,6 +760,10 @@ __device__ __forceinline__ void adjoint_heston_block(const AA& A, uint32_t blk, 
   // are multiplied once at the end by the row's kappa dt, sqrt(dt) and t5.
   auto sweep = [&](auto urow) {
     constexpr bool U = decltype(urow)::value && !FEEDS;
+    // with running accumulators on a grid of equal steps: the general
+    // expressions, every point's rows read from row 1 (loads the compiler
+    // hoists out of the loops)
+    constexpr bool UF = decltype(urow)::value && FEEDS;
     float u_e0 = 0.0f, u_d1 = 0.0f, u_d2 = 0.0f, u_rho = 0.0f, u_rhop = 0.0f, u_lo = 0.0f;
     float u_c1 = 0.0f, u_c2h = 0.0f, u_t3 = 0.0f, u_t5h = 0.0f, u_dt = 0.0f, u_dtth = 0.0f, u_rr = 0.0f;
     float s_vnb = 0.0f, s_sv = 0.0f, s_rho = 0.0f;
@@ -786,7 +812,11 @@ __device__ __forceinline__ void adjoint_heston_block(const AA& A, uint32_t blk, 
           z1s[i] = z[2 * s];
           z2s[i] = z[2 * s + 1];
           if (4 * g + s <= n) {
-            if constexpr (FEEDS) advance<MK_HESTON, 1, 1>(P, 4 * g + s, &z[2 * s], xs, vs);
+            if constexpr (UF) {
+              // point 0's zero row leaves the state as it is
+              if (4 * g + s == 0) continue;
+              advance<MK_HESTON, 1, 1>(P, 1, &z[2 * s], xs, vs);
+            } else if constexpr (FEEDS) advance<MK_HESTON, 1, 1>(P, 4 * g + s, &z[2 * s], xs, vs);
             else if constexpr (U) {
               // point 0 has the zero row (no step); the others share row 1
               if (4 * g + s == 0) continue;
@@ -836,13 +866,14 @@ __device__ __forceinline__ void adjoint_heston_block(const AA& A, uint32_t blk, 
             sv_next = sv;
           } else if (p >= 1) {
             // one 32-byte row per point: a single scalar load
-            const float t1 = ld_f32(A.rev, 8 * (size_t)p + 0);
-            const float t2 = ld_f32(A.rev, 8 * (size_t)p + 1);
-            const float t3 = ld_f32(A.rev, 8 * (size_t)p + 2);
-            const float t5 = ld_f32(A.rev, 8 * (size_t)p + 3);
-            const float dt = ld_f32(A.rev, 8 * (size_t)p + 4);
-            const float sq = ld_f32(A.rev, 8 * (size_t)p + 5);
-            const float kdt = ld_f32(A.rev, 8 * (size_t)p + 6);
+            const size_t pr = UF ? 1 : (size_t)p;
+            const float t1 = ld_f32(A.rev, 8 * pr + 0);
+            const float t2 = ld_f32(A.rev, 8 * pr + 1);
+            const float t3 = ld_f32(A.rev, 8 * pr + 2);
+            const float t5 = ld_f32(A.rev, 8 * pr + 3);
+            const float dt = ld_f32(A.rev, 8 * pr + 4);
+            const float sq = ld_f32(A.rev, 8 * pr + 5);
+            const float kdt = ld_f32(A.rev, 8 * pr + 6);
             const float sv = svs[i], z1 = z1s[i], z2 = z2s[i];
             const float vp = sv * sv;  // max(v, 0) to rounding
             const float w = fmaf(rho, z1, rho_perp * z2);
@@ -859,7 +890,7 @@ __device__ __forceinline__ void adjoint_heston_block(const AA& A, uint32_t blk, 
             vpb = sv > 0.0f ? fmaf(svb, 0.5f * __builtin_amdgcn_rcpf(sv), vpb) : vpb;
             vbar = vnb + ((!ft || sv > 0.0f) ? vpb : 0.0f);
             if constexpr (FEEDS) {
-              const float t0 = ld_f32(P.tab, (size_t)kHestonStride * p + 0);
+              const float t0 = ld_f32(P.tab, (size_t)kHestonStride * pr + 0);
               xs[0][0] = xs[0][0] - (fmaf(t1, vp, t0) + t2 * sv * z1);
             }
             sv_next = sv;
@@ -874,12 +905,14 @@ __device__ __forceinline__ void adjoint_heston_block(const AA& A, uint32_t blk, 
       g_rho = ld_f32(A.rev, 8 + 3) * s_rho;
     }
   };
-  // (one instance only with running accumulators: U needs none)
+  // (with running accumulators U would fold a constant xb: the equal-step
+  // instance is UF)
   if constexpr (!FEEDS) {
     if (P.tab_uniform) sweep(std::integral_constant<bool, true>{});
     else sweep(std::integral_constant<bool, false>{});
   } else {
-    sweep(std::integral_constant<bool, false>{});
+    if (P.tab_uniform) sweep(std::integral_constant<bool, true>{});
+    else sweep(std::integral_constant<bool, false>{});
   }
 
   double vals[NOUT] = {pv[0][0], (double)g_s0, (double)vbar, (double)g_k, (double)g_th, (double)g_sv, (double)g_rho};

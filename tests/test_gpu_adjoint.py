@@ -239,26 +239,36 @@ def test_heston_tangent_mode_equals_reverse_sweep():
 
 
 def test_heston_sweep_on_equal_and_unequal_steps():
-    """The reverse sweep reads its step rows from registers on a grid of equal
-    steps and from the tables otherwise; on either grid it must agree with the
-    forward-tangent mode (same paths, FP32 rounding only), and on the equal
-    grid the two paths give the same price as the plain pricer."""
+    """The reverse sweep reads its step rows from registers (or rows hoisted
+    out of its loops, with running averages) on a grid of equal steps and
+    from the tables otherwise, and the forward pass runs the op-free stretches
+    between an average's fixings from registers on the equal grid; on either
+    grid it must agree with the forward-tangent mode (same paths, FP32
+    rounding only), and the price with the plain pricer's."""
     eng = N.engine(0)
     m = K.heston(100.0, 0.04, 2.0, 0.04, 0.3, -0.5, 0.05, asset="X")
-    c = call("X", 100.0)
     equal = [i / 96 for i in range(97)]
     unequal = sorted(set([i / 96 for i in range(97)] + [0.013, 0.377, 0.5011, 0.9]))
+    # fixings every 8th point of the equal grid (runs of op-free groups
+    # between them), and one off the 4-point groups' boundaries
+    fix = [equal[k] for k in range(8, 97, 8)] + [equal[43]]
+    monthly = K.scale((K.average_at(K.spot("X"), sorted(fix)) - 100.0).clip(0.0), K.one("USD")) @ K.at(1.0)
+    # (contract, relative and standard-error tolerances): with running
+    # averages the log-spot is walked back from checkpoints, so the two modes
+    # differ by FP32 rounding over a checkpoint interval
+    cases = [(call("X", 100.0), 1e-5, 1e-3), (monthly, 1e-4, 2e-3), (asian_call("X", 100.0), 1e-4, 2e-3)]
     try:
-        for grid in (equal, unequal):
-            eng.set_option("pathwise_mode", 0)
-            a = c.pathwise_greeks(m, n_paths=1 << 15, seed=13, grid=grid)
-            eng.set_option("pathwise_mode", 1)
-            b = c.pathwise_greeks(m, n_paths=1 << 15, seed=13, grid=grid)
-            assert a.price == b.price
-            for name, x, y, se in zip(a.names, a.values, b.values, a.std_errors):
-                assert abs(x - y) <= 1e-5 * abs(x) + 1e-3 * se, (len(grid), name, x, y)
-            p = c.price(m, n_paths=1 << 15, seed=13, grid=grid)
-            assert a.price == pytest.approx(p.price, rel=1e-6)
+        for c, rel, sef in cases:
+            for grid in (equal, unequal):
+                eng.set_option("pathwise_mode", 0)
+                a = c.pathwise_greeks(m, n_paths=1 << 15, seed=13, grid=grid)
+                eng.set_option("pathwise_mode", 1)
+                b = c.pathwise_greeks(m, n_paths=1 << 15, seed=13, grid=grid)
+                assert a.price == b.price
+                for name, x, y, se in zip(a.names, a.values, b.values, a.std_errors):
+                    assert abs(x - y) <= rel * abs(x) + sef * se, (len(grid), name, x, y)
+                p = c.price(m, n_paths=1 << 15, seed=13, grid=grid)
+                assert a.price == pytest.approx(p.price, rel=1e-6)
     finally:
         eng.set_option("pathwise_mode", 0)
 
