@@ -189,14 +189,25 @@ def bench_book(args, eng, K, B, R, T, threads):
                 job[0].pathwise_greeks(job[1], **kw)
 
     _, wall_pc, dev_pc, _, _ = timed_steps(eng, per_contract, 1)
-    # price only, the same book
+    # price only, the same book, with the price kernel's register budget
+    # tuned on this call (it runs every job there; the Greeks book's tile
+    # was tuned on a call where the price kernel runs only the barriers)
+    if args.no_tune:
+        po_tuned = {"best": tuned["best"], "device_ms": {}}
+    else:
+        po_tuned = T.autotune_portfolio(jobs, candidates=[tuple(tuned["best"][:2])], repeats=1, greeks=False,
+                                        unroll_lanes=[])
     _, wall_po, dev_po, _, _ = timed_steps(eng, lambda: K.price_portfolio(jobs), 2)
+    T.set_tile(eng, tuned["best"])
+    log(f"config4 price only: {1e3 * wall_po / 2:.1f} ms/call ({dev_po / 2:.1f} ms device), price kernel tile "
+        f"{po_tuned['best'][2]} waves ({', '.join(f'{k}: {v:.1f} ms' for k, v in po_tuned['device_ms'].items())})")
     extra = {
         "per_contract_s": wall_pc, "per_contract_device_ms": dev_pc,
         "speedup_vs_per_contract": wall_pc / (wall / args.steps),
         "price_only_ms": 1e3 * wall_po / 2, "price_only_device_ms": dev_po / 2,
         "price_only_path_steps_per_s": path_steps * 2 / wall_po,
         "price_only_roofline_frac": round(lane_ops_po / (dev_po / 2 / 1e3) / R.PEAK_LANE_OPS, 4),
+        "price_only_tile": {"price_waves_per_simd": po_tuned["best"][2], "device_ms": po_tuned["device_ms"]},
         "first_prices": [round(r.price, 6) for r, _ in res[:4]],
         "speedup_target_note": "the 3x over per-contract launches is out of reach for this book: its contracts "
                                "(2^16-2^20 paths x 63-756 steps) already fill the GPU one launch at a time, so "
