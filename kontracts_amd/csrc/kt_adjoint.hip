@@ -172,10 +172,11 @@ __device__ __forceinline__ void init_registers(const PA& P, const RF& R) {
 // interpreter is a load or a store at a wave-uniform slot index; in VGPRs
 // the interpreter's writes at such an index merge whole copies of the file.
 // The GBM and Heston adjoints interpret ops at every fixing of a running
-// average and take the LDS file (the book's GBM Greeks 30.3 to 29.0 ms, its
-// Heston Asians 65.7 to 64.0 ms); the basket adjoint interprets only its
-// payoff and the tangent kernel reads its file on every point, and they
-// keep VGPRs (in LDS: the basket 0.3% slower, with more spills).  Each lane
+// average and take the LDS file (measured 4% faster on GBM Asians and
+// Europeans together, 2% on Heston Asians); the basket adjoint interprets
+// only its payoff and the tangent kernel reads its file on every point,
+// and they keep VGPRs (in LDS the basket ran 0.3% slower, with more
+// spills).  Each lane
 // touches only its own column, so no barrier is needed between work items.
 template <bool IN_LDS, int N>
 __device__ __forceinline__ auto adj_register_file(float* lds, float (&arr)[1][N]) {
@@ -189,8 +190,6 @@ __device__ __forceinline__ auto adj_register_file(float* lds, float (&arr)[1][N]
 
 }  // namespace
 
-// GBM (K = 1) and correlated GBM (K >= 2): block blk of the job (paths
-// blk*256 .. blk*256+255).  Outputs: price, d/ds0_a, d/dsigma_a, d/dmu_a.
 // One past the run of op-free groups that starts at group g, or g itself
 // when g has ops or is the last group (which may be partial and always takes
 // the per-point path): an op-free group's slot names the first group after
@@ -201,6 +200,8 @@ __device__ __forceinline__ int op_free_run_end(const PA& P, int g) {
   return min((int)ld_u32(P.groups, (size_t)g * 4 + 1), P.n_groups - 1);
 }
 
+// GBM (K = 1) and correlated GBM (K >= 2): block blk of the job (paths
+// blk*256 .. blk*256+255).  Outputs: price, d/ds0_a, d/dsigma_a, d/dmu_a.
 template <int K, int NS, class AA>
 __device__ __forceinline__ void adjoint_gbm_block(const AA& A, uint32_t blk) {
   constexpr int MODEL = K == 1 ? MK_GBM : MK_BASKET;
