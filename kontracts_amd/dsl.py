@@ -271,6 +271,16 @@ class ModelGreeks:
         self.std_errors = list(se[1:1 + len(names)])
         self.method = method
 
+    @classmethod
+    def _from_rows(cls, names, price, price_std_error, values, std_errors, method) -> "ModelGreeks":
+        """From the price, its error and fresh lists of the derivatives and
+        their errors (taken as they are, not copied)."""
+        g = cls.__new__(cls)
+        g.price, g.price_std_error = price, price_std_error
+        g.names = list(names)
+        g.values, g.std_errors, g.method = values, std_errors, method
+        return g
+
     def __getitem__(self, name: str) -> float:
         return self.values[self.names.index(name)]
 
@@ -282,14 +292,24 @@ class ModelGreeks:
         return f"ModelGreeks({self.method}: price={self.price:.6f}, {body})"
 
 
-def _param_names(model) -> list:
+_GBM_NAMES = ("s0", "sigma", "mu")
+_HESTON_NAMES = ("s0", "v0", "kappa", "theta", "sigma_v", "rho")
+_BASKET_NAMES: dict = {}  # factor names -> parameter names
+
+
+def _param_names(model) -> tuple:
     if isinstance(model, GBM):
-        return ["s0", "sigma", "mu"]
+        return _GBM_NAMES
     if isinstance(model, Model) and model.kind == N.KT_MODEL_HESTON:
-        return ["s0", "v0", "kappa", "theta", "sigma_v", "rho"]
+        return _HESTON_NAMES
     if isinstance(model, Model) and model.kind == N.KT_MODEL_CORRELATED_GBM:
-        fs = [f.name for f in model.params["factors"]]
-        return [f"s0[{n}]" for n in fs] + [f"sigma[{n}]" for n in fs] + [f"mu[{n}]" for n in fs]
+        fs = tuple(f.name for f in model.params["factors"])
+        names = _BASKET_NAMES.get(fs)
+        if names is None:
+            names = tuple([f"s0[{n}]" for n in fs] + [f"sigma[{n}]" for n in fs] + [f"mu[{n}]" for n in fs])
+            if len(_BASKET_NAMES) < 4096:
+                _BASKET_NAMES[fs] = names
+        return names
     raise TypeError("expected a GBM or Model")
 
 
@@ -807,11 +827,12 @@ def price_portfolio(jobs: Sequence[tuple], device: int | None = None, greeks: bo
     for j, job in enumerate(jobs):
         pr = PriceResult._from_fields(prices[j])
         names = _param_names(job[1])
-        row = grows[j][:len(names)]
+        n = len(names)
+        row = grows[j][:n]
         if any(x != x for x in row):  # NaN: no pathwise Greeks for this job
             res.append((pr, None))
             continue
-        res.append((pr, ModelGreeks(names, [pr.price] + row, [pr.std_error] + srows[j][:len(names)], "pathwise")))
+        res.append((pr, ModelGreeks._from_rows(names, pr.price, pr.std_error, row, srows[j][:n], "pathwise")))
     return res
 
 
