@@ -171,18 +171,32 @@ const char* error_text(int code) {
   }
 }
 
-std::shared_ptr<const Program> lower_cached(kt_engine* E, const char* json, const Con& con,
+std::shared_ptr<const Program> lower_cached(kt_engine* E, const std::shared_ptr<const Con>& con,
                                             const std::vector<double>& grid, const std::vector<std::string>& names) {
-  // key: the text, the names and a 64-bit mix of the grid's bits (the grid
-  // itself is compared on a hit, so a collision only costs a re-lowering)
-  uint64_t h = 0x9E3779B97F4A7C15ull ^ grid.size();
-  for (double t : grid) {
+  // key: the parsed contract (the parse cache keeps one per contract text;
+  // the entry holds it, so its address is not reused while the entry
+  // lives), the names and a 64-bit mix of the grid's bits in four
+  // independent lanes (the grid itself is compared on a hit, so a collision
+  // only costs a re-lowering)
+  uint64_t lane[4] = {0x9E3779B97F4A7C15ull ^ grid.size(), 0x6A09E667F3BCC909ull, 0xBB67AE8584CAA73Bull,
+                      0x3C6EF372FE94F82Bull};
+  const size_t n4 = grid.size() & ~(size_t)3;
+  auto mix = [](uint64_t h, double t) {
     uint64_t b;
     std::memcpy(&b, &t, sizeof b);
     h = (h ^ b) * 0xBF58476D1CE4E5B9ull;
-    h ^= h >> 31;
+    return h ^ (h >> 31);
+  };
+  for (size_t i = 0; i < n4; i += 4)
+    for (int l = 0; l < 4; ++l) lane[l] = mix(lane[l], grid[i + l]);
+  for (size_t i = n4; i < grid.size(); ++i) lane[0] = mix(lane[0], grid[i]);
+  uint64_t h = lane[0];
+  for (int l = 1; l < 4; ++l) {
+    h = (h ^ lane[l]) * 0x94D049BB133111EBull;
+    h ^= h >> 29;
   }
-  std::string key(json);
+  const Con* cp = con.get();
+  std::string key((const char*)&cp, sizeof cp);
   for (const std::string& n : names) {
     key.push_back('\0');
     key += n;
@@ -191,7 +205,7 @@ std::shared_ptr<const Program> lower_cached(kt_engine* E, const char* json, cons
   key.append((const char*)&h, sizeof h);
   auto it = E->lower_cache.find(key);
   if (it != E->lower_cache.end() && it->second.grid == grid) return it->second.prog;
-  auto p = std::make_shared<const Program>(lower({&con}, grid, names));
+  auto p = std::make_shared<const Program>(lower({con.get()}, grid, names));
   if (E->lower_cache.size() >= 16384) {
     E->lower_cache.clear();
     E->adj_plan_cache.clear();
@@ -199,9 +213,9 @@ std::shared_ptr<const Program> lower_cached(kt_engine* E, const char* json, cons
   }
   if (it != E->lower_cache.end()) {
     E->adj_plan_cache.erase(it->second.prog.get());
-    it->second = kt_engine::Lowered{grid, p};
+    it->second = kt_engine::Lowered{grid, p, con};
   } else {
-    E->lower_cache.emplace(std::move(key), kt_engine::Lowered{grid, p});
+    E->lower_cache.emplace(std::move(key), kt_engine::Lowered{grid, p, con});
   }
   return p;
 }

@@ -90,13 +90,15 @@ struct kt_engine {
   uint64_t prof_launches = 0;
   void prof_resolve();
   int adj_seg = kt::kAdjSeg;  // Heston adjoint checkpoint interval (kt_engine_set_option)
-  // Lowered programs of recent (contract JSON, grid, asset names): a book
-  // priced again (new market data, same contracts) skips the lowering.
-  // Keyed by the contract text, the asset names and a hash of the grid; an
-  // entry keeps its grid, compared on every hit.
+  // Lowered programs of recent (contract, grid, asset names): a book priced
+  // again (new market data, same contracts) skips the lowering.  Keyed by
+  // the parse cache's contract (its address, held by the entry), the asset
+  // names and a hash of the grid; an entry keeps its grid, compared on every
+  // hit.
   struct Lowered {
     std::vector<double> grid;
     std::shared_ptr<const kt::Program> prog;
+    std::shared_ptr<const kt::Con> con;
   };
   std::unordered_map<std::string, Lowered> lower_cache;
   // classify_adjoint of cached programs (null plan: not pathwise-Greek-able),
@@ -166,9 +168,10 @@ const char* error_text(int code);
 // date ("...: <date>"), as KontractError::TimeOutOfRange(f64) carries it.
 std::string error_text(int code, const Program& prog);
 
-// lower({con}, grid, names) of the contract whose JSON is `json`, memoised
-// on the engine (the program depends on nothing else).
-std::shared_ptr<const Program> lower_cached(kt_engine* E, const char* json, const Con& con,
+// lower({con}, grid, names), memoised on the engine (the program depends on
+// nothing else); con comes from prepare() with the engine, so one contract
+// text maps to one con while its parse-cache entry lives.
+std::shared_ptr<const Program> lower_cached(kt_engine* E, const std::shared_ptr<const Con>& con,
                                             const std::vector<double>& grid, const std::vector<std::string>& names);
 // classify_adjoint of a program returned by lower_cached, memoised with it;
 // null when the program has no pathwise Greeks (Unsupported).

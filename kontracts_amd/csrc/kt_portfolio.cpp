@@ -104,7 +104,7 @@ PriceBookHandle launch_price_book(kt_engine* E, const kt_job* jobs, uint64_t n_j
       solo.push_back(j);
       continue;
     }
-    P.prog = lower_cached(E, J.contract_json, *pc.con, P.grid, P.md.names);
+    P.prog = lower_cached(E, pc.con, P.grid, P.md.names);
     P.variant = variant_of(P.md, *P.prog);
     if (J.cfg.n_paths == 0) {
       out[j] = result_from_stats(KStats{0.0, 0.0, 0.0});

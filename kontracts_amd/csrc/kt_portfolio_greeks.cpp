@@ -73,7 +73,7 @@ void price_portfolio_greeks(kt_engine* E, const kt_job* jobs, uint64_t n_jobs, k
     ModelDesc md = model_from_abi(&J.model);
     const bool vr = J.cfg.antithetic || J.cfg.control_variate;
     PreparedContract pc = prepare(J.contract_json, md, &J.cfg, vr, E);
-    std::shared_ptr<const Program> prog_ptr = lower_cached(E, J.contract_json, *pc.con, pc.grid, md.names);
+    std::shared_ptr<const Program> prog_ptr = lower_cached(E, pc.con, pc.grid, md.names);
     const Program& prog = *prog_ptr;
     bool ok = !vr;
     for (const KOp& op : prog.ops) ok = ok && op.code != OP_FLAGERR;  // path errors: the pricer raises them
