@@ -4,6 +4,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <limits>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -405,6 +406,23 @@ void fill_model_args(PathArgs& A, const ModelDesc& md, uint64_t seed) {
   A.s0_scen[0] = (float)md.s0[0];
 }
 
+// exp(-rate t_k) at every grid point whose op list pays a flow (OP_EMIT, the
+// only op that reads the discount table) and NaN elsewhere, so that a read
+// of an unused entry would show; out has grid.size() entries.
+void discount_factors(const Program& prog, const std::vector<double>& grid, double rate, double* out) {
+  const double nan = std::numeric_limits<double>::quiet_NaN();
+  for (size_t k = 0; k < grid.size(); ++k) {
+    out[k] = nan;
+    if (k >= prog.steps.size()) continue;
+    const KStep& st = prog.steps[k];
+    for (int i = 0; i < st.count; ++i)
+      if (prog.ops[(size_t)st.begin + i].code == OP_EMIT) {
+        out[k] = std::exp(-rate * grid[k]);
+        break;
+      }
+  }
+}
+
 // Knock-out thresholds log2(H/s0) per group and scenario; groups whose spot
 // or level is not positive are interpreted instead.
 std::vector<KGroup> knockout_tables(const Program& prog, const std::vector<ModelDesc>& models,
@@ -466,8 +484,7 @@ LaunchTables launch_tables(const Program& prog, const std::vector<ModelDesc>& mo
   }
   if (discount) {
     T.df.resize(distinct.size() * grid.size());
-    for (size_t t = 0; t < distinct.size(); ++t)
-      for (size_t k = 0; k < grid.size(); ++k) T.df[t * grid.size() + k] = std::exp(-distinct[t] * grid[k]);
+    for (size_t t = 0; t < distinct.size(); ++t) discount_factors(prog, grid, distinct[t], &T.df[t * grid.size()]);
   }
   T.groups = knockout_tables(prog, models, &T.thr);
   return T;

@@ -190,6 +190,9 @@ std::vector<KStats> run_program(kt_engine* E, const Program& prog, const ModelDe
 
 std::vector<KGroup> knockout_tables(const Program& prog, const std::vector<ModelDesc>& models,
                                     std::vector<float>* thr);
+// exp(-rate t_k) at the grid points where the program pays a flow, NaN at
+// the others (no kernel reads them); out[0 .. grid.size()).
+void discount_factors(const Program& prog, const std::vector<double>& grid, double rate, double* out);
 
 // Model-dependent tables of one launch of the path kernel: per-point model
 // table (scenario-interleaved when SC > 1), discount tables per distinct

@@ -349,7 +349,7 @@ AdjJobHost adjoint_job_host(const Program& prog, const ModelDesc& md, const std:
   AdjJobHost H;
   H.tab = model_table(md, grid, &H.stride);
   H.df.resize(grid.size());
-  for (size_t k = 0; k < grid.size(); ++k) H.df[k] = std::exp(-rate * grid[k]);
+  discount_factors(prog, grid, rate, H.df.data());
   // GBM / basket: {sqrt(dt), t log2(e)} per point (d x / d sigma is
   // (W_t - sigma t) log2(e), W_t the sum of sqrt(dt) w; d x / d mu is t log2(e))
   H.adj.assign(2 * grid.size(), 0.0f);

@@ -96,7 +96,7 @@ PriceBookHandle launch_price_book(kt_engine* E, const kt_job* jobs, uint64_t n_j
     JobPlan& P = plans[j];
     P.md = model_from_abi(&J.model);
     PreparedContract pc = prepare(J.contract_json, P.md, &J.cfg, true, E);
-    P.grid = pc.grid;
+    P.grid = std::move(pc.grid);
     out[j] = kt_price_result{};
     if (J.cfg.antithetic || J.cfg.control_variate) {
       // variance reduction runs its own launches, like a loop's price_gbm
@@ -116,7 +116,7 @@ PriceBookHandle launch_price_book(kt_engine* E, const kt_job* jobs, uint64_t n_j
     }
     P.tab = model_table(P.md, P.grid, &P.stride);
     P.df.resize(P.grid.size());
-    for (size_t k = 0; k < P.grid.size(); ++k) P.df[k] = std::exp(-J.cfg.rate * P.grid[k]);
+    discount_factors(*P.prog, P.grid, J.cfg.rate, P.df.data());
     P.groups = knockout_tables(*P.prog, {P.md}, &P.thr);
     P.n_blocks = (J.cfg.n_paths + kBlock - 1) / kBlock;
     if (P.n_blocks > 0xffffffffull) raise(KT_ERR_INVALID_ARGUMENT, "too many paths in job " + jid(j));
