@@ -101,6 +101,28 @@ def test_gbm_vs_oracle_crn_fd(kind):
     check_close(g, ref, [2e-3 * abs(x) + 1e-4 for x in ref], kind)
 
 
+@pytest.mark.parametrize("grid_kind", ["equal", "unequal"])
+def test_gbm_fixings_vs_oracle_crn_fd(grid_kind):
+    """An average_at over fixings 12 points apart: the op-free groups between
+    them run as loops of their own (one read of their run's end), from VGPR
+    step rows on the equal grid and from the table on the unequal one, and
+    each fixing goes through the op interpreter; against central CRN
+    differences of the oracle on the same Philox stream."""
+    s0, sg, r = 100.0, 0.3, 0.04
+    mu = r - 0.01
+    equal = [k / 96 for k in range(97)]
+    fix = sorted([equal[k] for k in range(12, 97, 12)] + [equal[43]])
+    grid = equal if grid_kind == "equal" else sorted(set(equal + [0.013, 0.377, 0.5011, 0.9]))
+    c = K.scale((K.average_at(K.spot("X"), fix) - 100.0).clip(0.0), K.one("USD")) @ K.at(1.0)
+    m = K.GBM(s0, sg, r, q=r - mu, asset="X")
+    n, seed = 65536, 17
+    g = c.pathwise_greeks(m, n_paths=n, seed=seed, grid=grid)
+    make = lambda p: O.gbm("X", p[0], p[2], p[1])
+    h = [1e-3 * s0, 1e-4, 1e-4]
+    ref = oracle_fd(c.to_json(), make, [s0, sg, mu], h, n, seed, 50, r, grid=grid)
+    check_close(g, ref, [2e-3 * abs(x) + 1e-4 for x in ref], f"fixings-{grid_kind}")
+
+
 def test_basket_vs_oracle_crn_fd():
     names = ["A", "B", "C"]
     s0 = [100.0, 90.0, 110.0]
