@@ -173,10 +173,10 @@ __device__ __forceinline__ void init_registers(const PA& P, const RF& R) {
 // the interpreter's writes at such an index merge whole copies of the file.
 // The GBM and Heston adjoints interpret ops at every fixing of a running
 // average and take the LDS file (measured 4% faster on GBM Asians and
-// Europeans together, 2% on Heston Asians); the basket adjoint interprets
-// only its payoff and the tangent kernel reads its file on every point,
-// and they keep VGPRs (in LDS the basket ran 0.3% slower, with more
-// spills).  Each lane
+// Europeans together, 2% on Heston Asians); the basket adjoint takes it
+// too, for the registers it frees (at 6 waves per SIMD its spills shrink
+// from 320 to 196 B per lane: 1% faster than the VGPR file at 5 waves);
+// the tangent kernel reads its file on every point and keeps VGPRs.  Each lane
 // touches only its own column, so no barrier is needed between work items.
 template <bool IN_LDS, int N>
 __device__ __forceinline__ auto adj_register_file(float* lds, float (&arr)[1][N]) {
@@ -213,7 +213,7 @@ __device__ __forceinline__ void adjoint_gbm_block(const AA& A, uint32_t blk) {
   const uint64_t path = P.path_base + local;
   const uint32_t p_lo = (uint32_t)path, p_hi = (uint32_t)(path >> 32);
   const int n = P.n_grid - 1;
-  KT_ADJ_REGISTER_FILE(NS, r, K == 1);
+  KT_ADJ_REGISTER_FILE(NS, r, true);
   init_registers<NS>(P, r);
   // ws[a] = sum of sqrt(dt) w_a over the steps so far: d x_a / d sigma_a at
   // time t is (ws[a] - sigma_a t) log2(e) and d x_a / d mu_a is t log2(e),
@@ -994,14 +994,16 @@ __device__ __forceinline__ void portfolio_adjoint_body(const AdjPortfolioArgs& Q
 
 // The register budget per variant: the Heston variants need a few registers
 // more than 128, which would cost a quarter of their occupancy (3 waves per
-// SIMD instead of 4), so they are capped at 128 (4 waves); the basket and the
-// Heston variant with running accumulators run faster still at 102 (5 waves,
-// spilling a few values around their loops: measured 1.6% and 6% on the
-// 1,000-contract book); GBM takes the compiler's choice.  WAVES = 5 caps
-// every variant at 102 VGPRs (a tile the autotuner may pick).
+// SIMD instead of 4), so they are capped at 128 (4 waves); the Heston
+// variant with running accumulators runs faster still at 102 (5 waves,
+// spilling a few values around its loops: 6% on the 1,000-contract book)
+// and the basket, with its register file in LDS, at 85 (6 waves: 1% against
+// 5 waves); GBM takes the compiler's choice.  WAVES = 5 caps every variant
+// at 102 VGPRs (a tile the autotuner may pick).
 template <int VARIANT>
 constexpr int adjoint_default_waves() {
-  return (VARIANT == APV_BASKET5 || VARIANT == APV_HESTON_FEEDS)   ? 5
+  return VARIANT == APV_BASKET5                                     ? 6
+         : VARIANT == APV_HESTON_FEEDS                              ? 5
          : (VARIANT == APV_HESTON || VARIANT == APV_HESTON_FEEDS_T) ? 4
                                                                      : 1;
 }
