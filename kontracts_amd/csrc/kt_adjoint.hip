@@ -992,18 +992,20 @@ __device__ __forceinline__ void portfolio_adjoint_body(const AdjPortfolioArgs& Q
   }
 }
 
-// The register budget per variant: the Heston variants need a few registers
-// more than 128, which would cost a quarter of their occupancy (3 waves per
-// SIMD instead of 4), so they are capped at 128 (4 waves); the Heston
-// variant with running accumulators runs faster still at 102 (5 waves,
-// spilling a few values around its loops: 6% on the 1,000-contract book)
-// and the basket, with its register file in LDS, at 85 (6 waves: 1% against
-// 5 waves); GBM takes the compiler's choice.  WAVES = 5 caps every variant
-// at 102 VGPRs (a tile the autotuner may pick).
+// The register budget per variant, measured on the 1,000-contract book: the
+// Heston variants need a few registers more than 128, which would cost a
+// quarter of their occupancy (3 waves per SIMD instead of 4), so they are
+// capped at 128 (4 waves; 5 waves was 3% slower); the Heston variant with
+// running accumulators runs faster still at 102 (5 waves, spilling a few
+// values around its loops: 6%; 6 waves 9% slower), GBM with its register
+// file in LDS at 102 too (5 waves, its spills outside the group loop: 6%
+// against the compiler's 116 VGPRs), and the basket, with its register
+// file in LDS, at 85 (6 waves: 1% against 5 waves).  WAVES = 5 caps every
+// variant at 102 VGPRs (a tile the autotuner may pick).
 template <int VARIANT>
 constexpr int adjoint_default_waves() {
   return VARIANT == APV_BASKET5                                     ? 6
-         : VARIANT == APV_HESTON_FEEDS                              ? 5
+         : (VARIANT == APV_HESTON_FEEDS || VARIANT == APV_GBM)      ? 5
          : (VARIANT == APV_HESTON || VARIANT == APV_HESTON_FEEDS_T) ? 4
                                                                      : 1;
 }
