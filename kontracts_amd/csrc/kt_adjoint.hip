@@ -190,16 +190,6 @@ __device__ __forceinline__ auto adj_register_file(float* lds, float (&arr)[1][N]
 
 }  // namespace
 
-// One past the run of op-free groups that starts at group g, or g itself
-// when g has ops or is the last group (which may be partial and always takes
-// the per-point path): an op-free group's slot names the first group after
-// its run, so the run's length costs one read.
-template <class PA>
-__device__ __forceinline__ int op_free_run_end(const PA& P, int g) {
-  if (g + 1 >= P.n_groups || ld_u32(P.groups, (size_t)g * 4) != (uint32_t)G_NONE) return g;
-  return min((int)ld_u32(P.groups, (size_t)g * 4 + 1), P.n_groups - 1);
-}
-
 // GBM (K = 1) and correlated GBM (K >= 2): block blk of the job (paths
 // blk*256 .. blk*256+255).  Outputs: price, d/ds0_a, d/dsigma_a, d/dmu_a.
 template <int K, int NS, class AA>

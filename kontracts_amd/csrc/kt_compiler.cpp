@@ -1186,6 +1186,26 @@ void classify_groups(Program& p) {
         ok = a.count == b.count && (a.count == 0 || a.begin == b.begin);
       }
       if (ok) grp = first;
+      // fixings of a running sum at some of the four points, nothing at the
+      // others (an average_at between its dates): one fast group with the
+      // fixing points' mask
+      if (!ok) {
+        uint32_t mask = 0;
+        KGroup fix{G_GENERIC, 0, 0, 0.0f};
+        bool fixes = true;
+        for (int s = 0; fixes && s < 4; ++s) {
+          const KStep& st = p.steps[g0 + s];
+          if (st.count == 0) continue;
+          KGroup m;
+          fixes = single_monitor(p, st, m) && m.kind == G_AVG && (mask == 0 || (m.slot == fix.slot && m.asset == fix.asset));
+          fix = m;
+          mask |= 1u << s;
+        }
+        if (fixes && mask != 0) {
+          grp = KGroup{G_AVG_AT, fix.slot, fix.asset, 0.0f};
+          std::memcpy(&grp.imm, &mask, sizeof mask);
+        }
+      }
     }
     p.groups.push_back(grp);
   }
@@ -1298,11 +1318,11 @@ std::string Program::describe() const {
   std::ostringstream os;
   os << "slots " << n_slots << ", outputs " << n_outputs << ", ops " << ops.size() << ", max/step "
      << max_ops_per_step << (time_only ? ", time-only" : ", price-dependent") << "\n";
-  int ng[9] = {0};
+  int ng[10] = {0};
   for (const KGroup& g : groups) ng[g.kind]++;
   os << "groups: generic " << ng[G_GENERIC] << ", none " << ng[G_NONE] << ", knock-out "
-     << ng[G_KO_GE] + ng[G_KO_GT] + ng[G_KO_LE] + ng[G_KO_LT] << ", avg " << ng[G_AVG] << ", rmax/rmin "
-     << ng[G_RMAX] + ng[G_RMIN] << "\n";
+     << ng[G_KO_GE] + ng[G_KO_GT] + ng[G_KO_LE] + ng[G_KO_LT] << ", avg " << ng[G_AVG] << ", avg at points "
+     << ng[G_AVG_AT] << ", rmax/rmin " << ng[G_RMAX] + ng[G_RMIN] << "\n";
   os << "init:";
   for (const KOp& o : init) os << " " << op_name(o.code) << "(" << (int)o.d << "," << o.imm << ")";
   os << "\n";

@@ -101,6 +101,8 @@ __device__ __forceinline__ void fast_points(const PA& A, const KGroup gd, int k0
   constexpr bool KO = knock_out_kind<KIND>();
   constexpr bool UP = KIND == G_KO_GE || KIND == G_KO_GT;
   const uint32_t asset = (uint32_t)gd.asset < (uint32_t)K ? (uint32_t)gd.asset : 0u;
+  // G_AVG_AT: the points that fix (wave-uniform)
+  const uint32_t fix_mask = KIND == G_AVG_AT ? __float_as_uint(gd.imm) : 0xfu;
   float s0v[SC];
   FOR_SC s0v[sc] = scen_s0<K, SC>(A, sc, asset);
 #pragma unroll
@@ -109,13 +111,14 @@ __device__ __forceinline__ void fast_points(const PA& A, const KGroup gd, int k0
     else advance<MODEL, K, SC>(A, k0 + s, &z[s * M], x, v);
     store_normals<MODEL, K, SC, DEBUG>(A, k0 + s, &z[s * M], live, local);
     if constexpr (KIND != G_NONE) {
+      if (KIND == G_AVG_AT && !((fix_mask >> s) & 1u)) continue;
       FOR_SC {
         if (KO && xsrc<SC>(sc) != sc) continue;  // shares the extreme of its source scenario
         const float xs = K == 1 ? x[xsrc<SC>(sc)][0] : x[xsrc<SC>(sc)][asset];
         if constexpr (KO) ext[sc] = UP ? fmaxf(ext[sc], xs) : fminf(ext[sc], xs);
-        if constexpr (KIND == G_AVG || KIND == G_RMAX || KIND == G_RMIN) {
+        if constexpr (KIND == G_AVG || KIND == G_AVG_AT || KIND == G_RMAX || KIND == G_RMIN) {
           const float S = s0v[sc] * exp2_fast(xs);
-          if constexpr (KIND == G_AVG) {
+          if constexpr (KIND == G_AVG || KIND == G_AVG_AT) {
             const float y = S - comp[sc];
             const float t = acc[sc] + y;
             comp[sc] = (t - acc[sc]) - y;
@@ -321,7 +324,7 @@ __device__ __forceinline__ void path_block(const PA& A, uint32_t blk, float (&r)
       const uint32_t s0i = (uint32_t)mslot & (NS - 1), s1i = (s0i + 1) & (NS - 1);
       FOR_SC {
         put(sc, s0i, macc[sc]);
-        if (mkind == G_AVG) put(sc, s1i, mcomp[sc]);
+        if (mkind == G_AVG || mkind == G_AVG_AT) put(sc, s1i, mcomp[sc]);
       }
       mslot = -1;
     }
@@ -428,7 +431,7 @@ __device__ __forceinline__ void path_block(const PA& A, uint32_t blk, float (&r)
         }
         if (mslot >= 0) {
           rf(0, s0i) = m_acc;
-          if (mkind == G_AVG) rf(0, s1i) = m_comp;
+          if (mkind == G_AVG || mkind == G_AVG_AT) rf(0, s1i) = m_comp;
         }
         float xs[1][1] = {{x_start}};
         double ps[1][NOUT];
@@ -465,6 +468,23 @@ __device__ __forceinline__ void path_block(const PA& A, uint32_t blk, float (&r)
   if constexpr (!SERIAL) {
   init_file();
   for (int g = g_begin; g < A.n_groups; ++g) {
+    // a later run of op-free groups (between the fixings of an average_at)
+    // on a grid of equal steps: a loop of its own, from VGPR copies of row 1
+    if constexpr (!DEBUG && MODEL != MK_BASKET && SC == 1) {
+      if (A.tab_uniform && g > 0) {
+        const int e = op_free_run_end(A, g);
+        if (e > g) {
+          const StepRow<MODEL, K, SC> R = vgpr_row<MODEL, K, SC>(A, 1);
+          const KGroup none{G_NONE, 0, 0, 0.0f};
+          float ext[SC];
+          for (; g < e; ++g) {
+            float z[4 * M];
+            group_normals<M>(g, p_lo, p_hi, A.seed_lo, A.seed_hi, flip, z);
+            fast_points<G_NONE, MODEL, K, SC, DEBUG, true>(A, none, 4 * g, z, x, v, macc, mcomp, ext, live, local, &R);
+          }
+        }
+      }
+    }
     const KGroup gd = load_group(g);
     float z[4 * M];
     group_normals<M>(g, p_lo, p_hi, A.seed_lo, A.seed_hi, flip, z);
@@ -474,7 +494,7 @@ __device__ __forceinline__ void path_block(const PA& A, uint32_t blk, float (&r)
       const uint32_t s0i = (uint32_t)gd.slot & (NS - 1), s1i = (s0i + 1) & (NS - 1);
       FOR_SC {
         macc[sc] = get(sc, s0i);
-        mcomp[sc] = gd.kind == G_AVG ? get(sc, s1i) : 0.0f;
+        mcomp[sc] = (gd.kind == G_AVG || gd.kind == G_AVG_AT) ? get(sc, s1i) : 0.0f;
       }
       mslot = gd.slot;
       mkind = gd.kind;
@@ -488,6 +508,7 @@ __device__ __forceinline__ void path_block(const PA& A, uint32_t blk, float (&r)
       KT_FAST(G_KO_LE)
       KT_FAST(G_KO_LT)
       KT_FAST(G_AVG)
+      KT_FAST(G_AVG_AT)
       KT_FAST(G_RMAX)
       KT_FAST(G_RMIN)
 #undef KT_FAST

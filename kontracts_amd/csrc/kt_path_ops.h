@@ -158,6 +158,16 @@ __device__ __forceinline__ void run_ops(const PA& A, int k, int begin, int count
   run_ops_rf<NS, NOUT, K, SC>(A, k, begin, count, RegArray<SC, NS>{r}, pv, x, live);
 }
 
+// One past the run of op-free groups that starts at group g, or g itself
+// when g has ops or is the last group (which may be partial and always takes
+// the per-point path): an op-free group's slot names the first group after
+// its run, so the run's length costs one read.
+template <class PA>
+__device__ __forceinline__ int op_free_run_end(const PA& P, int g) {
+  if (g + 1 >= P.n_groups || ld_u32(P.groups, (size_t)g * 4) != (uint32_t)G_NONE) return g;
+  return min((int)ld_u32(P.groups, (size_t)g * 4 + 1), P.n_groups - 1);
+}
+
 // Advance every scenario of one path to grid point k with that point's
 // normals zk[0..M).
 // The 4 M normals of group g (M Philox blocks), words XORed with flip.

@@ -81,6 +81,8 @@ enum GroupKind : int32_t {
   G_AVG = 6,      // Kahan sum of S_asset into r[slot], r[slot+1]
   G_RMAX = 7,     // r[slot] = max(r[slot], S_asset)
   G_RMIN = 8,
+  G_AVG_AT = 9,   // G_AVG at the points whose bit (1 << point in the group)
+                  // is set in imm's bits; the other points have no ops
 };
 
 struct KGroup {

@@ -181,6 +181,15 @@ PARITY_CASES = {
     # AverageAt extension: monthly fixings on a daily grid, under Heston
     "heston_asian_monthly": (K.scale((K.average_at(K.spot("X"), [i / 12 for i in range(1, 13)]) - 100.0).clip(0.0),
                                      K.one("USD")) @ K.at(1.0), HESTON_X, 252, None, 100.0, 0),
+    # ... with the fixings on an equal-step grid (every 21st of 252 points,
+    # one off a 4-point group boundary): fixing groups of one point each
+    # between op-free runs, which the pricing kernel runs from VGPR step rows
+    "gbm_asian_monthly_equal": (K.scale((K.average_at(K.spot("X"), [k / 252 for k in range(21, 253, 21)])
+                                         - 100.0).clip(0.0), K.one("USD")) @ K.at(1.0), GBM_X, 0,
+                                [i / 252 for i in range(253)], 100.0, 0),
+    "heston_asian_monthly_equal": (K.scale((K.average_at(K.spot("X"), [k / 252 for k in range(21, 253, 21)])
+                                            - 100.0).clip(0.0), K.one("USD")) @ K.at(1.0), HESTON_X, 0,
+                                   [i / 252 for i in range(253)], 100.0, 0),
 }
 
 
@@ -232,10 +241,11 @@ def test_ragged_path_counts(eng, name, n):
 def test_pricing_kernel_equals_dispatching_kernel(eng, name):
     """The pricing kernels run a leading run of groups of one fast kind (and
     one slot, asset and level) in a loop of their own, with a knock-out's
-    compare deferred to the end of the run; the debug kernel, whose per-path
-    values are checked against the oracle above, takes every group through
-    the dispatching loop.  Same paths and arithmetic: the price and spread
-    agree to the order of the reduction."""
+    compare deferred to the end of the run, and later runs of op-free groups
+    on an equal-step grid from VGPR step rows; the debug kernel, whose
+    per-path values are checked against the oracle above, takes every group
+    through the dispatching loop.  Same paths and arithmetic: the price and
+    spread agree to the order of the reduction."""
     contract, model, spy, grid, _, _ = PARITY_CASES[name]
     n = 4096
     pv, _, _, _ = debug_paths(eng, contract, model, n, 42, spy, grid)
