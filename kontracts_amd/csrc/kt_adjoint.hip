@@ -156,6 +156,40 @@ __device__ __forceinline__ void run_point_ops(const AA& A, int k, const RF& R, d
     if (((fm >> q) & 1u) && A.acc_kind[q] != OP_AVG && R(0, A.acc_slot[q] & (NS - 1)) != prev[q]) argstep[q] = k;
 }
 
+// A point of a G_AVG_AT group whose mask bit is set: its op list is the
+// spot of one asset and that spot's Kahan update of a running sum; run_ops'
+// SPOT and AVG expressions on the same slots without the dispatch, rounded
+// the same way (the spot's own slot is not written: no op reads it after
+// that point's list).  An AVG
+// accumulator never moves an argstep, so run_point_ops' bookkeeping has
+// nothing to do here.
+template <int NS, int K, class AA, class RF>
+__device__ __forceinline__ void fix_point(const AA& A, const RF& R, uint32_t slot, uint32_t asset,
+                                          const float (&x)[1][K]) {
+  // the interpreter rounds the spot to a register slot before the update
+  // reads it: no fused multiply-subtract here either
+#pragma clang fp contract(off)
+  const uint32_t ai = asset < (uint32_t)K ? asset : 0u;
+  const float S = scen_s0<K, 1>(A.p, 0, ai) * exp2_fast(x[0][ai]);
+  const uint32_t d = slot & (NS - 1), dc = (d + 1) & (NS - 1);
+  const float y = S - R(0, dc);
+  const float t = R(0, d) + y;
+  R(0, dc) = (t - R(0, d)) - y;
+  R(0, d) = t;
+}
+
+// The mask, slot and asset of group g when it is a G_AVG_AT group, else a
+// zero mask (its points take run_point_ops).
+struct FixGroup {
+  uint32_t mask, slot, asset;
+};
+template <class PA>
+__device__ __forceinline__ FixGroup fix_group(const PA& P, int g) {
+  if (ld_u32(P.groups, (size_t)g * 4) != (uint32_t)G_AVG_AT) return FixGroup{0u, 0u, 0u};
+  return FixGroup{ld_u32(P.groups, (size_t)g * 4 + 3), ld_u32(P.groups, (size_t)g * 4 + 1),
+                  ld_u32(P.groups, (size_t)g * 4 + 2)};
+}
+
 template <int NS, class PA, class RF>
 __device__ __forceinline__ void init_registers(const PA& P, const RF& R) {
 #pragma unroll
@@ -305,6 +339,9 @@ __device__ __forceinline__ void adjoint_gbm_block(const AA& A, uint32_t blk) {
     }
     if (g >= P.n_groups) break;
     const bool quiet = !kRuns && op_free(g);
+    // (one asset only: the basket's extra code path costs it spills, and its
+    // fixings take the interpreter)
+    const FixGroup fg = K == 1 ? fix_group(P, g) : FixGroup{0u, 0u, 0u};
     float z[4 * M];
 #pragma unroll
     for (int bk = 0; bk < M; ++bk) {
@@ -334,7 +371,8 @@ __device__ __forceinline__ void adjoint_gbm_block(const AA& A, uint32_t blk) {
         int before[kAdjMaxAcc];
 #pragma unroll
         for (int q = 0; q < kAdjMaxAcc; ++q) before[q] = argstep[q];
-        run_point_ops<NS, K>(A, k, r, pv, x, live, fm, argstep);
+        if (K > 1 || fg.mask == 0u) run_point_ops<NS, K>(A, k, r, pv, x, live, fm, argstep);
+        else if ((fg.mask >> s) & 1u) fix_point<NS, K>(A, r, fg.slot, fg.asset, x);
         if (fm) {
 #pragma unroll
           for (int q = 0; q < kAdjMaxAcc; ++q) {
@@ -717,13 +755,17 @@ __device__ __forceinline__ void adjoint_heston_block(const AA& A, uint32_t blk, 
       }
       continue;
     }
+    const FixGroup fg = fix_group(P, g);
 #pragma unroll
     for (int s = 0; s < 4; ++s) {
       const int k = 4 * g + s;
       if (k <= n) {
         checkpoint(k);
         advance<MK_HESTON, 1, 1>(P, k, &z[2 * s], x, v);
-        run_point_ops<NS, 1>(A, k, r, pv, x, live, FEEDS ? ld_u32(A.feed, (size_t)k) : 0u, argstep);
+        if (fg.mask == 0u)
+          run_point_ops<NS, 1>(A, k, r, pv, x, live, FEEDS ? ld_u32(A.feed, (size_t)k) : 0u, argstep);
+        else if ((fg.mask >> s) & 1u)
+          fix_point<NS, 1>(A, r, fg.slot, fg.asset, x);
       }
     }
   }
