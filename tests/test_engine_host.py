@@ -152,6 +152,28 @@ def test_average_at_plan_and_lowering():
     assert K.Contract.from_json(c.to_json()) == c
 
 
+def test_fixing_groups_classified():
+    """A group of four points whose points either have no ops or fix one
+    running sum (spot, then its Kahan update) is the fast kind "avg at
+    points"; two sums in one group, or a fixing beside a knock-out monitor,
+    stay interpreted; the partial last group always is."""
+    gbm = K.GBM(100.0, 0.2, 0.05, asset="X")
+    grid = [i / 16 for i in range(17)]
+
+    def groups(c):
+        line = K.describe_program(c, gbm, grid=grid).splitlines()[1]
+        return {k.strip(): int(v) for k, v in (part.rsplit(" ", 1) for part in line[len("groups: "):].split(","))}
+
+    avg_at = lambda dates: K.average_at(K.spot("X"), dates)
+    # fixings at points 1 and 2 (group 0) and 6 (group 1); the payoff at 16
+    g = groups(K.scale((avg_at([1 / 16, 2 / 16, 6 / 16]) - 100.0).clip(0.0), K.one("USD")) @ K.at(1.0))
+    assert g == {"generic": 1, "none": 2, "knock-out": 0, "avg": 0, "avg at points": 2, "rmax/rmin": 0}
+    g = groups(K.scale((avg_at([1 / 16]) - avg_at([2 / 16])).clip(0.0), K.one("USD")) @ K.at(1.0))
+    assert g["avg at points"] == 0 and g["generic"] == 2
+    g = groups(K.until(K.spot("X") >= 150.0, K.scale(avg_at([5 / 16]), K.one("USD")) @ K.at(1.0)))
+    assert g["avg at points"] == 0 and g["knock-out"] == 3 and g["generic"] == 2
+
+
 def test_portfolio_book_generator():
     """The batched-portfolio book: deterministic, 250 of each kind, daily grids
     of 63..756 steps, 2^16..2^20 paths, single-asset models split GBM/Heston,
