@@ -281,9 +281,9 @@ kt_status kt_engine_kernel_time(kt_engine* engine, double* total_ms, uint64_t* n
  *                      256 (default, one path per lane) or 128 (each lane
  *                      prices two paths of a 256-path block in turn).
  *   "portfolio_adjoint_waves": register budget of the persistent Greeks
- *                      kernels: 0 (default: 102 VGPRs for baskets and Heston
- *                      with running accumulators, 128 for Heston, the
- *                      compiler's for GBM) or 5 waves per SIMD for all.
+ *                      kernels: 0 (default: 102 VGPRs for GBM and Heston
+ *                      with running accumulators, 85 for baskets, 128 for
+ *                      Heston) or 5 waves per SIMD for all.
  *   "lsm_graph":       1 (default) replays kt_price_american's launch chain
  *                      (paths kernel, backward passes, solves) from a
  *                      captured hipGraph while its launch arguments are
