@@ -379,10 +379,18 @@ def _lookback_knock_out():
     return K.up_and_out_call("X", 100.0, 200.0, 1.0) + K.until(S >= 230.0, K.scale(e, K.one("USD")) @ K.at(1.0))
 
 
+def _monthly_asian():
+    # fixings on the daily grid's points 21, 42, ... (fast fixing groups
+    # between op-free runs in the scenario kernel and in the pricer)
+    return K.scale((K.average_at(K.spot("X"), [k / 12 for k in range(1, 13)]) - 170.0).clip(0.0),
+                   K.one("USD")) @ K.at(1.0)
+
+
 @pytest.mark.parametrize("n", [65536, 1000])
-@pytest.mark.parametrize("which", ["up_and_out_8_registers", "lookback_knock_out_16_registers"])
+@pytest.mark.parametrize("which", ["up_and_out_8_registers", "lookback_knock_out_16_registers", "monthly_asian"])
 def test_greeks_scenario_kernel_equals_separate_pricings(which, n):
-    c = K.up_and_out_call("X", 100, 200, 1.0) if which.startswith("up_and_out") else _lookback_knock_out()
+    c = {"up_and_out_8_registers": lambda: K.up_and_out_call("X", 100, 200, 1.0),
+         "lookback_knock_out_16_registers": _lookback_knock_out, "monthly_asian": _monthly_asian}[which]()
     s0, sg, r, spy = 180.0, 0.25, 0.05, 252
     hs, hv, hr = 1.8, 0.01, 1e-4
     g = c.greeks(K.GBM(s0, sg, r, asset="X"), n_paths=n, seed=7, steps_per_year=spy, bumps=(hs, hv, hr))
